@@ -1,0 +1,163 @@
+"""Throughput of the real-time SU(3) leapfrog + NRQCD Crank-Nicolson hot path.
+
+Workload (one GPU): 64^3 lattice, spacings (a0, a1, a2, a3) = (0.05, 1, 1, 1),
+transverse-polarised box initial condition with Qs = 0.5, amplitude 1,
+coupling 1, ranlxd seed 2; FP64 links and E; heavy quark point source at
+lattice index 1, Ma = 2, c = (1, 0, 0, 0), CN solved by GMRES(30) to an FP64
+true relative residual of 1e-10.
+
+A "step" is one leapfrog step of the whole lattice (3V link-updates).  The
+timed region runs `--steps` leapfrog steps back to back on the library's
+stream with the inputs resident in HBM; CN steps are timed separately.
+
+Prints one JSON line.  Options: --gpus N (only 1), --steps K, --warmup W,
+--size L, --cn-steps C, --no-cpu (skip the CPU baseline).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+METRIC = "link-updates/s & CN steps/s at FP64 rel.res 1e-10: learned+mixed vs plain"
+HBM_PEAK_GBS = 8000.0
+
+
+def _cpu_baseline(size: int, steps: int) -> dict:
+    """Oracle leapfrog (OpenMP FP64 C restatement) on a bounded sample."""
+    from oracle.pyoracle import Oracle
+
+    orc = Oracle()
+    orc.lattice_init((size, size, size), (0.05, 1.0, 1.0, 1.0))
+    U, E = orc.box_init(2, 0.5, 1.0, 1.0)
+    orc.leapfrog(U, E, 1.0, 1)  # warm-up
+    t0 = time.perf_counter()
+    orc.leapfrog(U, E, 1.0, steps)
+    dt = time.perf_counter() - t0
+    return {
+        "value": 3 * size ** 3 * steps / dt,
+        "unit": "link-updates/s",
+        "cores": orc.num_threads(),
+        "kind": "port",
+        "sample": f"oracle leapfrog FP64, {size}^3, {steps} steps after 1 warm-up step ({dt:.1f} s)",
+    }
+
+
+def main() -> None:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=20)
+    ap.add_argument("--size", type=int, default=64)
+    ap.add_argument("--cn-steps", type=int, default=20)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--cpu-steps", type=int, default=4)
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the lattice is not "
+              "sharded across GPUs", file=sys.stderr)
+        sys.exit(2)
+
+    import nrqcd_amd as nrq
+
+    L = args.size
+    V = L ** 3
+    nrq.lattice_init((L, L, L), (0.05, 1.0, 1.0, 1.0))
+    nrq.random_init(2)
+    g = nrq.GaugeConfiguration(nrq.NRQ_PRECISION_FP64)
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    obs0 = g.observables()
+
+    # ---- leapfrog ------------------------------------------------------
+    g.Update(1.0, args.warmup)
+    nrq.synchronize()
+    nrq.event_record(0)
+    t0 = time.perf_counter()
+    g.Update(1.0, args.steps)
+    nrq.event_record(1)
+    ms = nrq.event_elapsed_ms(0, 1)
+    wall = time.perf_counter() - t0
+    obs1 = g.observables()
+    lu_per_s = 3 * V * args.steps / (ms * 1e-3)
+    ms_step = ms / args.steps
+    # dominant kernel: the fused E+link pass (steps-1 launches, plus one
+    # links-only and one E-only launch at the ends of each Update call)
+    bytes_per_site = 1248.0  # U, E read once and written once, FP64
+    achieved = bytes_per_site * V / (ms_step * 1e-3) / 1e9
+
+    # ---- CN ----------------------------------------------------------------
+    nrq.nrqcd_init(2.0, [1, 0, 0, 0], 1.0, 1e-10)
+    nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_GMRES_FP64, 30, 10000)
+    h = nrq.NRQCDField()
+    h.InitSinglePoint(1, 1)
+    h.Update(g, 2.0, [1, 0, 0, 0])  # warm-up (workspace allocation)
+    nrq.synchronize()
+    its = 0
+    worst = 0.0
+    nrq.event_record(2)
+    for _ in range(args.cn_steps):
+        st = h.Update(g, 2.0, [1, 0, 0, 0])
+        its += st.iterations
+        worst = max(worst, st.max_true_relres)
+    nrq.event_record(3)
+    cn_ms = nrq.event_elapsed_ms(2, 3)
+    cn_per_s = args.cn_steps / (cn_ms * 1e-3)
+    nq, na = h.norms()
+    nrq.nrqcd_finalize()
+
+    cpu = None
+    if not args.no_cpu:
+        cpu = _cpu_baseline(min(L, 32), args.cpu_steps)
+
+    out = {
+        "metric": METRIC,
+        "value": lu_per_s,
+        "unit": "link-updates/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic",
+        "config": {
+            "workload": f"{L}^3 SU(3) leapfrog (FP64) + NRQCD CN point source, box IC Qs=0.5 seed 2",
+            "lattice": [L, L, L],
+            "spacings": [0.05, 1.0, 1.0, 1.0],
+            "cn_solver": "GMRES(30) FP64, true rel.res 1e-10",
+            "mass": 2.0,
+        },
+        "cn_steps_per_s": cn_per_s,
+        "cn_ms_per_step": cn_ms / args.cn_steps,
+        "cn_krylov_iterations_per_step": its / args.cn_steps,
+        "cn_worst_true_relres": worst,
+        "cn_norms": [nq, na],
+        "energy_drift_rel": (obs1["energy"] - obs0["energy"]) / obs0["energy"],
+        "gauss_max_drift": obs1["gauss_max"] - obs0["gauss_max"],
+        "wall_s_leapfrog": wall,
+        "roofline": {
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "kernel": "k_fused<double> (E update + next link update per (site, dir))",
+            "bytes_per_launch": bytes_per_site * V,
+        },
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    nrq.lattice_finalize()
+
+
+if __name__ == "__main__":
+    main()

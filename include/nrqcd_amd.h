@@ -1,0 +1,205 @@
+/*
+ * nrqcd_amd.h -- C ABI of the MI355X-native real-time SU(3) leapfrog and
+ * NRQCD Crank-Nicolson stepper.
+ *
+ * This is the drop-in boundary for the reference's Fortran module interfaces
+ * `lattice`, `random`, `gaugeconfiguration_su3` and `nrqcd`: the Fortran
+ * modules under nrqcd_amd/fortran/ keep the reference's module, type and
+ * type-bound-procedure names and bind these functions with ISO_C_BINDING.
+ * Each declaration cites the reference interface it replaces.
+ *
+ * Conventions
+ *  - Every function returns NRQ_OK (0) or a negative NRQ_ERR_* code; the
+ *    message of the last failure is available from nrq_last_error().
+ *  - Lattice indices are 1-based int64 with x fastest (lattice.f90:883-931);
+ *    directions are +-1..3 (gaugeconfiguration_su3.f90:193-199).
+ *  - Complex numbers are (re, im) pairs of doubles.  Host arrays exchanged
+ *    with the library use the reference's Fortran layouts:
+ *      links  (3,3,3,V) complex128  gaugeconfiguration_su3.f90:35,134-137
+ *      efield (8,3,V)   real64
+ *      prop   (6,6,V)   complex128  nrqcd.f90:60-63
+ *  - Device state stays resident in HBM between calls.  All work is ordered
+ *    on one HIP stream; calls that return host values synchronise it.
+ *  - One host thread; handles are not thread-safe.
+ */
+#ifndef NRQCD_AMD_H
+#define NRQCD_AMD_H
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define NRQ_OK 0
+#define NRQ_ERR_ARG (-1)          /* invalid argument */
+#define NRQ_ERR_STATE (-2)        /* module not initialised / already initialised */
+#define NRQ_ERR_HIP (-3)          /* HIP runtime or hipFFT failure */
+#define NRQ_ERR_UNSUPPORTED (-4)  /* feature outside this build (e.g. c2..c4 != 0) */
+#define NRQ_ERR_NOCONV (-5)       /* Krylov solver hit maxit before the tolerance */
+#define NRQ_ERR_ALLOC (-6)        /* device or host allocation failed */
+
+#define NRQ_PRECISION_FP64 64
+#define NRQ_PRECISION_FP32 32
+
+/* Copy the last error message (NUL-terminated, truncated to n). Returns the
+ * code of the last failure (0 if none). */
+int nrq_last_error(char *buf, size_t n);
+
+/* Library version string. */
+const char *nrq_version(void);
+
+/* Timing and ordering on the library's HIP stream (no reference
+ * counterpart; the reference times nothing).  nrq_event_record records HIP
+ * event `slot` (0..15) on the stream; nrq_event_elapsed waits for `stop` and
+ * returns the milliseconds between two recorded slots; nrq_synchronize waits
+ * for all queued work. */
+int nrq_event_record(int slot);
+int nrq_event_elapsed(int start, int stop, double *ms);
+int nrq_synchronize(void);
+
+/* ---- lattice (lattice.f90) ----------------------------------------- */
+
+/* lattice::InitModule(LatticeExtensions_(3) int64, LatticeSpacings_(0:3))
+ * lattice.f90:124-228.  spacing = (a0, a1, a2, a3). Single process: the
+ * partition and halo of lattice.f90:151-192 do not exist. */
+int nrq_lattice_init(const int64_t ext[3], const double spacing[4]);
+/* Releases the module state (no reference counterpart; lets a process
+ * re-initialise with another lattice). */
+int nrq_lattice_finalize(void);
+/* lattice::GetLatticeSize, lattice.f90 (product of extents) */
+int nrq_lattice_size(int64_t *vol);
+/* lattice::GetLatticeIndex, lattice.f90:476-484 */
+int nrq_lattice_index(const int64_t pos[3], int64_t *idx);
+/* lattice::GetLatticePosition, lattice.f90:794-804 */
+int nrq_lattice_position(int64_t idx, int64_t pos[3]);
+/* lattice::GetNeib_G, lattice.f90:531-543 (dir in -3..3) */
+int nrq_lattice_neib(int dir, int64_t idx, int64_t *out);
+/* lattice::GetNegativeLatticeIndex, lattice.f90:495-505 */
+int nrq_lattice_negative_index(int64_t idx, int64_t *out);
+/* lattice::GetMomentum_G (backward derivative), lattice.f90:963-979;
+ * p = 3 complex numbers */
+int nrq_lattice_momentum(int64_t idx, double p[6]);
+/* lattice::GetNorm2Momentum_G, lattice.f90:1048-1058 */
+int nrq_lattice_norm2_momentum(int64_t idx, double *out);
+
+/* ---- random (random.f90 / ranlxd_generator.f90) --------------------- */
+
+/* random::InitModule(CustomSeed) -> ranlxd_init(2, seed), random.f90:126-140 */
+int nrq_random_init(int64_t seed);
+/* random::GetRandomUniformReal(n) -> ranlxd(r), random.f90:148-175 */
+int nrq_random_uniform(double *r, int64_t n);
+
+/* ---- gaugeconfiguration_su3 ------------------------------------------ */
+
+typedef struct nrq_gauge nrq_gauge;
+
+/* GaugeConfiguration%Allocate, gaugeconfiguration_su3.f90:128-138.
+ * precision: NRQ_PRECISION_FP64 or NRQ_PRECISION_FP32 (link/E storage). */
+int nrq_gauge_create(int precision, nrq_gauge **h);
+/* GaugeConfiguration%Deallocate, gaugeconfiguration_su3.f90:145-151 */
+int nrq_gauge_destroy(nrq_gauge *h);
+/* Intrinsic assignment GaugeConf_t2 = GaugeConf_t1 (main.f90:2320): deep
+ * device-to-device copy; dst is re-allocated to src's precision. */
+int nrq_gauge_copy(nrq_gauge *dst, const nrq_gauge *src);
+/* GaugeConfiguration%ColdInit, gaugeconfiguration_su3.f90:363-378 */
+int nrq_gauge_cold(nrq_gauge *h);
+/* GaugeConfiguration%TransversePolarisedOccupiedInit_Box(SaturationScale,
+ * Amplitude, Coupling), gaugeconfiguration_su3.f90:906-951 and 1263-1446.
+ * Draws from the library's ranlxd stream (nrq_random_init). */
+int nrq_gauge_box_ic(nrq_gauge *h, double saturation_scale, double amplitude, double coupling);
+/* Upload / download in the reference layouts links(3,3,3,V) complex128 and
+ * efield(8,3,V) real64 (gaugeconfiguration_su3.f90:35-37).  Either pointer may
+ * be NULL to skip that field. */
+int nrq_gauge_upload(nrq_gauge *h, const double *links, const double *efield);
+int nrq_gauge_download(const nrq_gauge *h, double *links, double *efield);
+/* GaugeConfiguration%GetLink_G(i, LatticeIndex), gaugeconfiguration_su3.f90:
+ * 160-196: dir 0 -> identity, dir < 0 -> U_{-dir}(x-dir)^dagger.
+ * link = 3x3 complex, column-major. */
+int nrq_gauge_get_link(const nrq_gauge *h, int dir, int64_t idx, double link[18]);
+/* GetEfield_G(a, i, LatticeIndex), gaugeconfiguration_su3.f90:239-254;
+ * e = 8 reals */
+int nrq_gauge_get_efield(const nrq_gauge *h, int dir, int64_t idx, double e[8]);
+/* GaugeConfiguration%Update([StepWidth]) -> Update_Leapfrog applied nsteps
+ * times, gaugeconfiguration_su3.f90:2082-2111 (links then E for
+ * stepwidth > 0, E then links otherwise). */
+int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps);
+/* GaugeConfiguration%GetEnergy(), gaugeconfiguration_su3.f90:1769-1817 */
+int nrq_gauge_energy(const nrq_gauge *h, double *energy);
+/* GaugeConfiguration%GetDeviationFromGaussLaw(), gaugeconfiguration_su3.f90:
+ * 1480-1531 (sum of 2|Im Tr(T^a D)|; zero up to rounding by construction) */
+int nrq_gauge_gauss_ref(const nrq_gauge *h, double *gauss);
+/* Gauss-law constraint max_{x,a} 2|Tr(T^a G(x))| of GetG/GetGdeviation,
+ * gaugeconfiguration_su3.f90:724-813 (no charge density) */
+int nrq_gauge_gauss_max(const nrq_gauge *h, double *gauss);
+/* (1/3V) sum_{x,i<j} Re Tr P_ij(x)/3 built from GetPlaquette_M,
+ * gaugeconfiguration_su3.f90:1722-1737 */
+int nrq_gauge_mean_plaquette(const nrq_gauge *h, double *plaq);
+/* All four observables in one pass: out = {energy, gauss_ref, gauss_max,
+ * mean_plaquette}. */
+int nrq_gauge_observables(const nrq_gauge *h, double out[4]);
+/* GaugeConfiguration%GetPlaquette_G(i, j, LatticeIndex) for spatial i, j,
+ * gaugeconfiguration_su3.f90:1591-1615 (3x3 complex column-major) */
+int nrq_gauge_plaquette(const nrq_gauge *h, int i, int j, int64_t idx, double P[18]);
+/* Projects every link back onto SU(3) in FP64 (Gram-Schmidt of rows 1-2,
+ * row 3 = conj(row1 x row2)).  No reference counterpart: the reference keeps
+ * FP64 links and never reunitarises; used by the FP32 storage mode. */
+int nrq_gauge_reunitarise(nrq_gauge *h);
+
+/* ---- nrqcd ------------------------------------------------------------ */
+
+typedef struct nrq_heavy nrq_heavy;
+
+typedef struct {
+    int64_t iterations;      /* Krylov iterations summed over the solved columns */
+    int64_t solves;          /* columns with a non-zero right-hand side */
+    int64_t restarts;        /* GMRES restart cycles (incl. refinement passes) */
+    double max_true_relres;  /* worst FP64 true relative residual ||b-Ax||/||b|| */
+} nrq_solve_stats;
+
+#define NRQ_SOLVER_GMRES_FP64 0   /* GMRES(m), classical Gram-Schmidt, FP64 */
+
+/* nrqcd::InitModule(Mass, WilsonCoeffs(4), StepWidth, kspTol),
+ * nrqcd.f90:167-192 with KSPSetTolerances(rtol=kspTol), nrqcd.f90:803-804.
+ * c = 4 complex Wilson coefficients.  The solver judges convergence on the
+ * FP64 true relative residual (stricter than PETSc's preconditioned test). */
+int nrq_cn_init(double mass, const double c[8], double stepwidth, double ksp_tol);
+/* nrqcd::FinalizeModule, nrqcd.f90:199-212 */
+int nrq_cn_finalize(void);
+/* Solver selection: kind NRQ_SOLVER_*, GMRES restart length (PETSc default
+ * 30) and iteration cap (PETSc default maxits 1e4, nrqcd.f90:803-804). */
+int nrq_cn_set_solver(int kind, int restart, int64_t maxit);
+
+/* NRQCDField() constructor / Destructor, nrqcd.f90:234-260 */
+int nrq_heavy_create(nrq_heavy **h);
+int nrq_heavy_destroy(nrq_heavy *h);
+/* NRQCDField assignment(=) -> assign, nrqcd.f90:219-228 */
+int nrq_heavy_copy(nrq_heavy *dst, const nrq_heavy *src);
+/* NRQCDField%InitSinglePoint(LatticeIndex_quark, LatticeIndex_antiq),
+ * nrqcd.f90:324-350: 1_6/sqrt(6) at the given sites */
+int nrq_heavy_init_point(nrq_heavy *h, int64_t idx_quark, int64_t idx_antiq);
+/* NRQCDField%InitSinglePointSingleDoF, nrqcd.f90:357-393 (spin 1..2,
+ * colour 1..3, spin-colour index colour + 3(spin-1) as nrqcd.f90:437-444) */
+int nrq_heavy_init_point_dof(nrq_heavy *h, int spin_q, int colour_q, int64_t idx_q,
+                             int spin_a, int colour_a, int64_t idx_a);
+/* Direct access to the public QuarkProp / AntiQprop components
+ * (nrqcd.f90:60-63) in the (6,6,V) complex128 layout; NULL skips. */
+int nrq_heavy_upload(nrq_heavy *h, const double *quark, const double *antiq);
+int nrq_heavy_download(const nrq_heavy *h, double *quark, double *antiq);
+/* NRQCDField%Update(GaugeConf, Mass, WilsonCoeffs, [StepWidth]) ->
+ * Update_CrankNicholson, nrqcd.f90:1053-1168: rhs = S psi, solve
+ * S^dagger psi' = rhs with S = 1 - i (StepWidth a0/2) H, H = -c1 D^2/2M; the
+ * antiquark uses -M, conj(c).  Only c1 is supported (c2..c4 must be 0).
+ * stats may be NULL. */
+int nrq_heavy_update(nrq_heavy *h, const nrq_gauge *g, double mass, const double c[8],
+                     double stepwidth, nrq_solve_stats *stats);
+/* GetNorm_Quark / GetNorm_AntiQ, nrqcd.f90:547-611 */
+int nrq_heavy_norms(const nrq_heavy *h, double *norm_quark, double *norm_antiq);
+/* GetMesoncorrelator_{O1_1S0, O1_3S1, O8_1S0, O8_3S1}_ZeroMomentum,
+ * nrqcd.f90:1484-1661; out = 4 complex numbers in that order */
+int nrq_heavy_correlators(const nrq_heavy *h, double out[8]);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

@@ -1,0 +1,434 @@
+"""nrqcd_amd -- MI355X-native real-time SU(3) leapfrog + NRQCD Crank-Nicolson.
+
+Python host layer over the C ABI of ``libnrqcd_amd.so`` (declared in
+``include/nrqcd_amd.h``).  The class and method names mirror the reference's
+Fortran modules so that scripts read like the reference drivers:
+
+* :mod:`lattice` functions -> :func:`lattice_init` (``lattice::InitModule``,
+  lattice.f90:124-228) and the index/neighbour/momentum accessors;
+* ``random::InitModule`` -> :func:`random_init` (random.f90:126-140);
+* ``type(GaugeConfiguration)`` -> :class:`GaugeConfiguration`
+  (gaugeconfiguration_su3.f90:33-116);
+* ``nrqcd::InitModule`` -> :func:`nrqcd_init` (nrqcd.f90:167-192) and
+  ``type(NRQCDField)`` -> :class:`NRQCDField` (nrqcd.f90:58-117).
+
+Arrays exchanged with the library use the reference's Fortran layouts as
+numpy arrays in Fortran order: links ``(3,3,3,V)`` complex128, efield
+``(8,3,V)`` float64, propagators ``(6,6,V)`` complex128.
+
+The library needs a HIP device; there is no CPU fallback.  Loading fails
+loudly when the shared library is missing.
+"""
+from __future__ import annotations
+
+import ctypes
+import os
+from pathlib import Path
+
+import numpy as np
+
+_PKG = Path(__file__).resolve().parent
+LIBRARY_PATH = _PKG / "libnrqcd_amd.so"
+
+NRQ_OK = 0
+NRQ_PRECISION_FP64 = 64
+NRQ_PRECISION_FP32 = 32
+NRQ_SOLVER_GMRES_FP64 = 0
+
+_ERRORS = {
+    -1: "NRQ_ERR_ARG",
+    -2: "NRQ_ERR_STATE",
+    -3: "NRQ_ERR_HIP",
+    -4: "NRQ_ERR_UNSUPPORTED",
+    -5: "NRQ_ERR_NOCONV",
+    -6: "NRQ_ERR_ALLOC",
+}
+
+
+class NRQError(RuntimeError):
+    """Raised when a C-ABI call returns a non-zero status."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{_ERRORS.get(code, code)}: {message}")
+        self.code = code
+
+
+class SolveStats(ctypes.Structure):
+    _fields_ = [
+        ("iterations", ctypes.c_int64),
+        ("solves", ctypes.c_int64),
+        ("restarts", ctypes.c_int64),
+        ("max_true_relres", ctypes.c_double),
+    ]
+
+
+_lib = None
+
+
+def library() -> ctypes.CDLL:
+    """Load ``libnrqcd_amd.so`` (built in-tree by ``nrqcd_amd.build``)."""
+    global _lib
+    if _lib is None:
+        path = os.environ.get("NRQCD_AMD_LIB", str(LIBRARY_PATH))
+        if not Path(path).exists():
+            raise ImportError(
+                f"nrqcd_amd: native library {path} is missing; build it with "
+                "`python -m nrqcd_amd.build` (there is no CPU fallback)")
+        _lib = ctypes.CDLL(path)
+        _declare(_lib)
+    return _lib
+
+
+def _declare(lib: ctypes.CDLL) -> None:
+    P = ctypes.c_void_p
+    I64 = ctypes.c_int64
+    D = ctypes.c_double
+    I = ctypes.c_int
+    sig = {
+        "nrq_last_error": (I, [ctypes.c_char_p, ctypes.c_size_t]),
+        "nrq_version": (ctypes.c_char_p, []),
+        "nrq_event_record": (I, [I]),
+        "nrq_event_elapsed": (I, [I, I, P]),
+        "nrq_synchronize": (I, []),
+        "nrq_lattice_init": (I, [P, P]),
+        "nrq_lattice_finalize": (I, []),
+        "nrq_lattice_size": (I, [P]),
+        "nrq_lattice_index": (I, [P, P]),
+        "nrq_lattice_position": (I, [I64, P]),
+        "nrq_lattice_neib": (I, [I, I64, P]),
+        "nrq_lattice_negative_index": (I, [I64, P]),
+        "nrq_lattice_momentum": (I, [I64, P]),
+        "nrq_lattice_norm2_momentum": (I, [I64, P]),
+        "nrq_random_init": (I, [I64]),
+        "nrq_random_uniform": (I, [P, I64]),
+        "nrq_gauge_create": (I, [I, P]),
+        "nrq_gauge_destroy": (I, [P]),
+        "nrq_gauge_copy": (I, [P, P]),
+        "nrq_gauge_cold": (I, [P]),
+        "nrq_gauge_box_ic": (I, [P, D, D, D]),
+        "nrq_gauge_upload": (I, [P, P, P]),
+        "nrq_gauge_download": (I, [P, P, P]),
+        "nrq_gauge_get_link": (I, [P, I, I64, P]),
+        "nrq_gauge_get_efield": (I, [P, I, I64, P]),
+        "nrq_gauge_leapfrog": (I, [P, D, I64]),
+        "nrq_gauge_energy": (I, [P, P]),
+        "nrq_gauge_gauss_ref": (I, [P, P]),
+        "nrq_gauge_gauss_max": (I, [P, P]),
+        "nrq_gauge_mean_plaquette": (I, [P, P]),
+        "nrq_gauge_observables": (I, [P, P]),
+        "nrq_gauge_plaquette": (I, [P, I, I, I64, P]),
+        "nrq_gauge_reunitarise": (I, [P]),
+        "nrq_cn_init": (I, [D, P, D, D]),
+        "nrq_cn_finalize": (I, []),
+        "nrq_cn_set_solver": (I, [I, I, I64]),
+        "nrq_heavy_create": (I, [P]),
+        "nrq_heavy_destroy": (I, [P]),
+        "nrq_heavy_copy": (I, [P, P]),
+        "nrq_heavy_init_point": (I, [P, I64, I64]),
+        "nrq_heavy_init_point_dof": (I, [P, I, I, I64, I, I, I64]),
+        "nrq_heavy_upload": (I, [P, P, P]),
+        "nrq_heavy_download": (I, [P, P, P]),
+        "nrq_heavy_update": (I, [P, P, D, P, D, P]),
+        "nrq_heavy_norms": (I, [P, P, P]),
+        "nrq_heavy_correlators": (I, [P, P]),
+    }
+    for name, (res, args) in sig.items():
+        f = getattr(lib, name)
+        f.restype = res
+        f.argtypes = args
+
+
+def _ptr(a: np.ndarray | None):
+    return None if a is None else a.ctypes.data_as(ctypes.c_void_p)
+
+
+def _check(rc: int) -> None:
+    if rc != NRQ_OK:
+        buf = ctypes.create_string_buffer(512)
+        library().nrq_last_error(buf, 512)
+        raise NRQError(rc, buf.value.decode(errors="replace"))
+
+
+def event_record(slot: int) -> None:
+    """Record HIP event ``slot`` on the library stream."""
+    _check(library().nrq_event_record(slot))
+
+
+def event_elapsed_ms(start: int, stop: int) -> float:
+    ms = ctypes.c_double()
+    _check(library().nrq_event_elapsed(start, stop, ctypes.byref(ms)))
+    return ms.value
+
+
+def synchronize() -> None:
+    _check(library().nrq_synchronize())
+
+
+# ---------------------------------------------------------------------------
+# lattice / random
+# ---------------------------------------------------------------------------
+def lattice_init(extensions, spacings) -> None:
+    """``lattice::InitModule(LatticeExtensions_, LatticeSpacings_)``."""
+    ext = np.ascontiguousarray(extensions, dtype=np.int64)
+    sp = np.ascontiguousarray(spacings, dtype=np.float64)
+    if ext.shape != (3,) or sp.shape != (4,):
+        raise ValueError("extensions must have 3 entries, spacings 4 (a0..a3)")
+    _check(library().nrq_lattice_init(_ptr(ext), _ptr(sp)))
+
+
+def lattice_finalize() -> None:
+    _check(library().nrq_lattice_finalize())
+
+
+def lattice_size() -> int:
+    v = ctypes.c_int64()
+    _check(library().nrq_lattice_size(ctypes.byref(v)))
+    return v.value
+
+
+def GetLatticeIndex(position) -> int:
+    pos = np.ascontiguousarray(position, dtype=np.int64)
+    out = ctypes.c_int64()
+    _check(library().nrq_lattice_index(_ptr(pos), ctypes.byref(out)))
+    return out.value
+
+
+def GetLatticePosition(index: int) -> np.ndarray:
+    pos = np.zeros(3, np.int64)
+    _check(library().nrq_lattice_position(index, _ptr(pos)))
+    return pos
+
+
+def GetNeib_G(direction: int, index: int) -> int:
+    out = ctypes.c_int64()
+    _check(library().nrq_lattice_neib(direction, index, ctypes.byref(out)))
+    return out.value
+
+
+def GetNegativeLatticeIndex(index: int) -> int:
+    out = ctypes.c_int64()
+    _check(library().nrq_lattice_negative_index(index, ctypes.byref(out)))
+    return out.value
+
+
+def GetMomentum_G(index: int) -> np.ndarray:
+    p = np.zeros(3, np.complex128)
+    _check(library().nrq_lattice_momentum(index, _ptr(p)))
+    return p
+
+
+def GetNorm2Momentum_G(index: int) -> float:
+    out = ctypes.c_double()
+    _check(library().nrq_lattice_norm2_momentum(index, ctypes.byref(out)))
+    return out.value
+
+
+def random_init(seed: int) -> None:
+    """``random::InitModule(CustomSeed)`` -> ranlxd level 2."""
+    _check(library().nrq_random_init(seed))
+
+
+def random_uniform(n: int) -> np.ndarray:
+    r = np.zeros(n)
+    _check(library().nrq_random_uniform(_ptr(r), n))
+    return r
+
+
+# ---------------------------------------------------------------------------
+# gaugeconfiguration_su3
+# ---------------------------------------------------------------------------
+class GaugeConfiguration:
+    """Device-resident ``type(GaugeConfiguration)``.
+
+    ``precision`` selects FP64 (reference) or FP32 link/E storage."""
+
+    def __init__(self, precision: int = NRQ_PRECISION_FP64):
+        self._h = ctypes.c_void_p()
+        _check(library().nrq_gauge_create(precision, ctypes.byref(self._h)))
+        self.precision = precision
+
+    # -- lifetime --------------------------------------------------------
+    def Deallocate(self) -> None:
+        if self._h:
+            library().nrq_gauge_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.Deallocate()
+        except Exception:
+            pass
+
+    def copy(self) -> "GaugeConfiguration":
+        """Deep copy (the reference's intrinsic assignment)."""
+        other = GaugeConfiguration.__new__(GaugeConfiguration)
+        other._h = ctypes.c_void_p()
+        _check(library().nrq_gauge_create(self.precision, ctypes.byref(other._h)))
+        other.precision = self.precision
+        _check(library().nrq_gauge_copy(other._h, self._h))
+        return other
+
+    # -- initialisation --------------------------------------------------
+    def ColdInit(self) -> None:
+        _check(library().nrq_gauge_cold(self._h))
+
+    def TransversePolarisedOccupiedInit_Box(self, SaturationScale: float, Amplitude: float,
+                                            Coupling: float) -> None:
+        _check(library().nrq_gauge_box_ic(self._h, SaturationScale, Amplitude, Coupling))
+
+    # -- state exchange ----------------------------------------------------
+    def upload(self, links: np.ndarray | None = None, efield: np.ndarray | None = None) -> None:
+        v = lattice_size()
+        lk = None if links is None else np.asfortranarray(links, dtype=np.complex128).reshape((3, 3, 3, v), order="F")
+        ef = None if efield is None else np.asfortranarray(efield, dtype=np.float64).reshape((8, 3, v), order="F")
+        _check(library().nrq_gauge_upload(self._h, _ptr(lk), _ptr(ef)))
+
+    def download(self):
+        v = lattice_size()
+        lk = np.zeros((3, 3, 3, v), np.complex128, order="F")
+        ef = np.zeros((8, 3, v), np.float64, order="F")
+        _check(library().nrq_gauge_download(self._h, _ptr(lk), _ptr(ef)))
+        return lk, ef
+
+    def GetLink_G(self, i: int, LatticeIndex: int) -> np.ndarray:
+        out = np.zeros((3, 3), np.complex128, order="F")
+        _check(library().nrq_gauge_get_link(self._h, i, LatticeIndex, _ptr(out)))
+        return out
+
+    def GetEfield_G(self, i: int, LatticeIndex: int) -> np.ndarray:
+        out = np.zeros(8)
+        _check(library().nrq_gauge_get_efield(self._h, i, LatticeIndex, _ptr(out)))
+        return out
+
+    # -- evolution -------------------------------------------------------
+    def Update(self, StepWidth: float = 1.0, nsteps: int = 1) -> None:
+        """``GaugeConf%Update([StepWidth])`` applied ``nsteps`` times."""
+        _check(library().nrq_gauge_leapfrog(self._h, StepWidth, nsteps))
+
+    def Reunitarise(self) -> None:
+        _check(library().nrq_gauge_reunitarise(self._h))
+
+    # -- observables -----------------------------------------------------
+    def observables(self) -> dict:
+        out = np.zeros(4)
+        _check(library().nrq_gauge_observables(self._h, _ptr(out)))
+        return {"energy": out[0], "gauss_ref": out[1], "gauss_max": out[2], "plaquette": out[3]}
+
+    def GetEnergy(self) -> float:
+        return self.observables()["energy"]
+
+    def GetDeviationFromGaussLaw(self) -> float:
+        return self.observables()["gauss_ref"]
+
+    def GetGaussMax(self) -> float:
+        return self.observables()["gauss_max"]
+
+    def GetMeanPlaquette(self) -> float:
+        return self.observables()["plaquette"]
+
+    def GetPlaquette_G(self, i: int, j: int, LatticeIndex: int) -> np.ndarray:
+        out = np.zeros((3, 3), np.complex128, order="F")
+        _check(library().nrq_gauge_plaquette(self._h, i, j, LatticeIndex, _ptr(out)))
+        return out
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+
+# ---------------------------------------------------------------------------
+# nrqcd
+# ---------------------------------------------------------------------------
+def _coeffs(WilsonCoeffs) -> np.ndarray:
+    c = np.zeros(4, np.complex128)
+    c[: len(WilsonCoeffs)] = WilsonCoeffs
+    return c
+
+
+def nrqcd_init(Mass: float, WilsonCoeffs, StepWidth: float, kspTol: float) -> None:
+    """``nrqcd::InitModule(Mass, WilsonCoeffs, StepWidth, kspTol)``."""
+    c = _coeffs(WilsonCoeffs)
+    _check(library().nrq_cn_init(Mass, _ptr(c), StepWidth, kspTol))
+
+
+def nrqcd_finalize() -> None:
+    _check(library().nrq_cn_finalize())
+
+
+def nrqcd_set_solver(kind: int = NRQ_SOLVER_GMRES_FP64, restart: int = 30, maxit: int = 10000) -> None:
+    _check(library().nrq_cn_set_solver(kind, restart, maxit))
+
+
+class NRQCDField:
+    """Device-resident ``type(NRQCDField)`` (quark and antiquark propagator)."""
+
+    def __init__(self):
+        self._h = ctypes.c_void_p()
+        _check(library().nrq_heavy_create(ctypes.byref(self._h)))
+
+    def Destructor(self) -> None:
+        if self._h:
+            library().nrq_heavy_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.Destructor()
+        except Exception:
+            pass
+
+    def copy(self) -> "NRQCDField":
+        other = NRQCDField()
+        _check(library().nrq_heavy_copy(other._h, self._h))
+        return other
+
+    def InitSinglePoint(self, LatticeIndex_quark: int, LatticeIndex_antiq: int) -> None:
+        _check(library().nrq_heavy_init_point(self._h, LatticeIndex_quark, LatticeIndex_antiq))
+
+    def InitSinglePointSingleDoF(self, spin_quark, colour_quark, LatticeIndex_quark, spin_antiq, colour_antiq,
+                                 LatticeIndex_antiq) -> None:
+        _check(library().nrq_heavy_init_point_dof(self._h, spin_quark, colour_quark, LatticeIndex_quark,
+                                                  spin_antiq, colour_antiq, LatticeIndex_antiq))
+
+    def upload(self, QuarkProp=None, AntiQProp=None) -> None:
+        v = lattice_size()
+        q = None if QuarkProp is None else np.asfortranarray(QuarkProp, dtype=np.complex128).reshape((6, 6, v), order="F")
+        a = None if AntiQProp is None else np.asfortranarray(AntiQProp, dtype=np.complex128).reshape((6, 6, v), order="F")
+        _check(library().nrq_heavy_upload(self._h, _ptr(q), _ptr(a)))
+
+    def download(self):
+        v = lattice_size()
+        q = np.zeros((6, 6, v), np.complex128, order="F")
+        a = np.zeros((6, 6, v), np.complex128, order="F")
+        _check(library().nrq_heavy_download(self._h, _ptr(q), _ptr(a)))
+        return q, a
+
+    def Update(self, GaugeConf: GaugeConfiguration, Mass: float, WilsonCoeffs, StepWidth: float = 1.0) -> SolveStats:
+        c = _coeffs(WilsonCoeffs)
+        st = SolveStats()
+        _check(library().nrq_heavy_update(self._h, GaugeConf.handle, Mass, _ptr(c), StepWidth, ctypes.byref(st)))
+        return st
+
+    def norms(self):
+        q = ctypes.c_double()
+        a = ctypes.c_double()
+        _check(library().nrq_heavy_norms(self._h, ctypes.byref(q), ctypes.byref(a)))
+        return q.value, a.value
+
+    def GetNorm_Quark(self) -> float:
+        return self.norms()[0]
+
+    def GetNorm_AntiQ(self) -> float:
+        return self.norms()[1]
+
+    def correlators(self) -> dict:
+        out = np.zeros(4, np.complex128)
+        _check(library().nrq_heavy_correlators(self._h, _ptr(out)))
+        return {"O1_1S0": out[0], "O1_3S1": out[1], "O8_1S0": out[2], "O8_3S1": out[3]}
+
+    def GetMesoncorrelator_O1_3S1_ZeroMomentum(self) -> complex:
+        return self.correlators()["O1_3S1"]
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h

@@ -1,0 +1,693 @@
+// nrq_gauge.hip -- gauge configuration in HBM: leapfrog kernels, FP64
+// observables and the gaugeconfiguration_su3 entry points of the C ABI.
+//
+// HBM layout (structure of arrays, one plane per matrix element so a
+// wavefront's 64 consecutive sites read 64 consecutive elements):
+//   links  U[dir][e][site]  complex (R re, R im), e = 3*row + col, dir 0..2
+//   efield E[dir][a][site]  R,                      a = 0..7
+// with site = lattice index - 1 (x fastest).  Two link buffers ping-pong so
+// the fused step can read the old links of a site's neighbours while writing
+// new links.
+#include "nrq_internal.h"
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace nrq {
+
+template <typename R>
+__device__ __forceinline__ mat3<R> load_link(const cplx<R> *__restrict__ u, int64_t vol, int dir, int64_t s)
+{
+    mat3<R> m;
+    const cplx<R> *p = u + (int64_t)dir * 9 * vol + s;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m.m[e] = p[(int64_t)e * vol];
+    return m;
+}
+
+template <typename R>
+__device__ __forceinline__ void store_link(cplx<R> *__restrict__ u, int64_t vol, int dir, int64_t s, const mat3<R> &m)
+{
+    cplx<R> *p = u + (int64_t)dir * 9 * vol + s;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) p[(int64_t)e * vol] = m.m[e];
+}
+
+// Update_Links_Leapfrog at one (site, dir), gaugeconfiguration_su3.f90:
+// 2133-2139: U <- exp(i dt E^a T^a) U.
+template <typename R>
+__device__ __forceinline__ mat3<R> evolve_link(const mat3<R> &u, const R *e, R dt)
+{
+    R al[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) al[a] = e[a] * dt;
+    mat3<R> x = exp_i_hermitian(algebra_matrix(al));
+    return mul(x, u);
+}
+
+// Update_Efield_Leapfrog_atSite_Direction, gaugeconfiguration_su3.f90:
+// 2164-2198: E^a_i(x) -= 2 Im Tr[T^a U_i(x) sum_{k!=i} dt (a0/a_k)^2 (S^U+S^D)]
+// with S^U = U_k(x+i) U_i(x+k)^+ U_k(x)^+ and
+//      S^D = U_k(x+i-k)^+ U_i(x-k)^+ U_k(x-k).
+template <typename R>
+__device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, const geom &g, int i, int64_t s,
+                                              const mat3<R> &ui, R *e, R dt)
+{
+    mat3<R> sum;
+    set_zero(sum);
+    // k runs in increasing order, as in the reference's staple sum
+    for (int k = 0; k < 3; ++k) {
+        if (k == i) continue;
+        const R coef = dt * R((g.a[0] / g.a[k + 1]) * (g.a[0] / g.a[k + 1]));
+        const int64_t xi = neib_up(g, s, i);
+        const int64_t xk = neib_up(g, s, k);
+        const int64_t xmk = neib_dn(g, s, k);
+        const int64_t ximk = neib_up(g, xmk, i);
+        mat3<R> su = mul_nd(load_link(u, g.vol, k, xi), load_link(u, g.vol, i, xk));
+        su = mul_nd(su, load_link(u, g.vol, k, s));
+        mat3<R> sd = mul_dd(load_link(u, g.vol, k, ximk), load_link(u, g.vol, i, xmk));
+        sd = mul(sd, load_link(u, g.vol, k, xmk));
+#pragma unroll
+        for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * (su.m[m] + sd.m[m]);
+    }
+    mat3<R> w = mul(ui, sum);
+    R t[8];
+    two_im_trace_gen(w, t);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) e[a] -= t[a];
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_links(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
+                                               const R *__restrict__ ef, int64_t vol, R dt)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (s >= vol) return;
+    R e[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * vol + s];
+    mat3<R> u = load_link(uin, vol, i, s);
+    store_link(uout, vol, i, s, evolve_link(u, e, dt));
+}
+
+template <typename R>
+__global__ __launch_bounds__(256) void k_efield(const cplx<R> *__restrict__ u, R *__restrict__ ef, geom g, R dt)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (s >= g.vol) return;
+    R e[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * g.vol + s];
+    mat3<R> ui = load_link(u, g.vol, i, s);
+    evolve_efield(u, g, i, s, ui, e, dt);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) ef[((int64_t)i * 8 + a) * g.vol + s] = e[a];
+}
+
+// One fused pass: the E update of a leapfrog step followed by the link update
+// of the next step at the same (site, dir).  Both halves are exactly the
+// per-(site, dir) operations of the reference; fusing them only removes one
+// HBM round trip of E and U.  Neighbour links are read from uin, new links
+// go to uout (ping-pong), E is updated in place (each thread owns its E).
+template <typename R>
+__global__ __launch_bounds__(256) void k_fused(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
+                                               R *__restrict__ ef, geom g, R dt)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (s >= g.vol) return;
+    R e[8];
+#pragma unroll
+    for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * g.vol + s];
+    mat3<R> ui = load_link(uin, g.vol, i, s);
+    evolve_efield(uin, g, i, s, ui, e, dt);
+#pragma unroll
+    for (int a = 0; a < 8; ++a) ef[((int64_t)i * 8 + a) * g.vol + s] = e[a];
+    store_link(uout, g.vol, i, s, evolve_link(ui, e, dt));
+}
+
+template <typename R> __global__ void k_cold(cplx<R> *u, R *ef, int64_t vol)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= vol) return;
+    for (int i = 0; i < 3; ++i) {
+        for (int e = 0; e < 9; ++e) u[((int64_t)i * 9 + e) * vol + s] = {R((e % 4) == 0 ? 1 : 0), R(0)};
+        for (int a = 0; a < 8; ++a) ef[((int64_t)i * 8 + a) * vol + s] = R(0);
+    }
+}
+
+// Block reduction over a 64-wide wavefront then across the block's waves.
+__device__ __forceinline__ double wave_sum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+__device__ __forceinline__ double wave_max(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v = fmax(v, __shfl_down(v, off, 64));
+    return v;
+}
+
+constexpr int NOBS = 5; // E-field energy, magnetic energy, sum Re Tr P/3, gauss_ref sum, gauss max
+
+// Per-site observables in FP64 (gaugeconfiguration_su3.f90:1480-1531,
+// 724-813, 1769-1817): one pass computes everything; partials[q][block].
+template <typename R>
+__global__ __launch_bounds__(REDUCE_THREADS) void k_obs(const cplx<R> *__restrict__ u, const R *__restrict__ ef, geom g,
+                                                        double *__restrict__ partials)
+{
+    double acc[NOBS] = {0, 0, 0, 0, 0};
+    const double a0 = g.a[0];
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < g.vol; s += (int64_t)gridDim.x * blockDim.x) {
+        mat3<double> gsum;
+        set_zero(gsum);
+        for (int i = 0; i < 3; ++i) {
+            const double ai = g.a[i + 1];
+            double ed[8], en[8];
+            const int64_t sm = neib_dn(g, s, i);
+            double esq = 0;
+            for (int a = 0; a < 8; ++a) {
+                ed[a] = (double)ef[((int64_t)i * 8 + a) * g.vol + s];
+                en[a] = (double)ef[((int64_t)i * 8 + a) * g.vol + sm];
+                const double v = ed[a] / ai / a0;
+                esq += v * v;
+            }
+            acc[0] += esq / 2.0;
+            mat3<double> um;
+            {
+                mat3<R> t = load_link(u, g.vol, i, sm);
+                for (int m = 0; m < 9; ++m) um.m[m] = {(double)t.m[m].re, (double)t.m[m].im};
+            }
+            // M = E_i(x) - U_i(x-i)^+ E_i(x-i) U_i(x-i)   (algebra matrices)
+            mat3<double> emat = algebra_matrix(ed);
+            mat3<double> t = mul(mul_dn(um, algebra_matrix(en)), um);
+            mat3<double> dm;
+            for (int m = 0; m < 9; ++m) dm.m[m] = emat.m[m] - t.m[m];
+            // reference Gauss deviation: D = M / a_i^2 / a0, sum_a 2|Im Tr(T^a D)|
+            {
+                mat3<double> d;
+                const double sc = 1.0 / (ai * ai) / a0;
+                for (int m = 0; m < 9; ++m) d.m[m] = sc * dm.m[m];
+                double tt[8];
+                two_im_trace_gen(d, tt);
+                for (int a = 0; a < 8; ++a) acc[3] += fabs(tt[a]);
+            }
+            for (int m = 0; m < 9; ++m) gsum.m[m] = gsum.m[m] + (1.0 / ai) * dm.m[m];
+            // magnetic part: plaquettes P_ij, j > i
+            for (int j = i + 1; j < 3; ++j) {
+                const int64_t xi = neib_up(g, s, i), xj = neib_up(g, s, j);
+                mat3<double> l1, l2, l3, l4;
+                {
+                    mat3<R> a1 = load_link(u, g.vol, i, s), a2 = load_link(u, g.vol, j, xi);
+                    mat3<R> a3 = load_link(u, g.vol, i, xj), a4 = load_link(u, g.vol, j, s);
+                    for (int m = 0; m < 9; ++m) {
+                        l1.m[m] = {(double)a1.m[m].re, (double)a1.m[m].im};
+                        l2.m[m] = {(double)a2.m[m].re, (double)a2.m[m].im};
+                        l3.m[m] = {(double)a3.m[m].re, (double)a3.m[m].im};
+                        l4.m[m] = {(double)a4.m[m].re, (double)a4.m[m].im};
+                    }
+                }
+                mat3<double> p = mul_nd(mul_nd(mul(l1, l2), l3), l4);
+                const double tr = trace(p).re;
+                acc[1] += 2.0 * 3.0 * (1.0 - tr / 3.0) / ai / g.a[j + 1];
+                acc[2] += tr / 3.0;
+            }
+        }
+        for (int m = 0; m < 9; ++m) gsum.m[m] = (1.0 / a0) * gsum.m[m];
+        double tt[8];
+        two_re_trace_gen(gsum, tt);
+        for (int a = 0; a < 8; ++a) acc[4] = fmax(acc[4], fabs(tt[a]));
+    }
+    __shared__ double red[NOBS][REDUCE_THREADS / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = 0; q < NOBS; ++q) {
+        double v = q == 4 ? wave_max(acc[q]) : wave_sum(acc[q]);
+        if (lane == 0) red[q][wv] = v;
+    }
+    __syncthreads();
+    if (threadIdx.x < NOBS) {
+        const int q = threadIdx.x;
+        double v = red[q][0];
+        for (int w = 1; w < REDUCE_THREADS / 64; ++w) v = q == 4 ? fmax(v, red[q][w]) : v + red[q][w];
+        partials[(int64_t)q * gridDim.x + blockIdx.x] = v;
+    }
+}
+
+__global__ void k_fold(const double *__restrict__ partials, int nblocks, int nq, int max_q, double *__restrict__ out)
+{
+    const int q = threadIdx.x;
+    if (q >= nq) return;
+    double v = partials[(int64_t)q * nblocks];
+    for (int b = 1; b < nblocks; ++b) {
+        const double w = partials[(int64_t)q * nblocks + b];
+        v = q == max_q ? fmax(v, w) : v + w;
+    }
+    out[q] = v;
+}
+
+// Projects U onto SU(3) in FP64: normalise row 1, orthonormalise row 2
+// against it, row 3 = conj(row1 x row2).
+template <typename R> __global__ void k_reunit(cplx<R> *u, int64_t vol)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int i = blockIdx.y;
+    if (s >= vol) return;
+    mat3<R> m = load_link(u, vol, i, s);
+    cplx<double> r0[3], r1[3];
+    for (int c = 0; c < 3; ++c) {
+        r0[c] = {(double)m.m[c].re, (double)m.m[c].im};
+        r1[c] = {(double)m.m[3 + c].re, (double)m.m[3 + c].im};
+    }
+    double n0 = 0;
+    for (int c = 0; c < 3; ++c) n0 += r0[c].re * r0[c].re + r0[c].im * r0[c].im;
+    n0 = 1.0 / sqrt(n0);
+    for (int c = 0; c < 3; ++c) r0[c] = n0 * r0[c];
+    cplx<double> pr = {0, 0};
+    for (int c = 0; c < 3; ++c) pr = pr + cmul(r0[c], r1[c]);
+    for (int c = 0; c < 3; ++c) r1[c] = r1[c] - pr * r0[c];
+    double n1 = 0;
+    for (int c = 0; c < 3; ++c) n1 += r1[c].re * r1[c].re + r1[c].im * r1[c].im;
+    n1 = 1.0 / sqrt(n1);
+    for (int c = 0; c < 3; ++c) r1[c] = n1 * r1[c];
+    cplx<double> r2[3];
+    r2[0] = conjg(r0[1] * r1[2] - r0[2] * r1[1]);
+    r2[1] = conjg(r0[2] * r1[0] - r0[0] * r1[2]);
+    r2[2] = conjg(r0[0] * r1[1] - r0[1] * r1[0]);
+    for (int c = 0; c < 3; ++c) {
+        m.m[c] = {(R)r0[c].re, (R)r0[c].im};
+        m.m[3 + c] = {(R)r1[c].re, (R)r1[c].im};
+        m.m[6 + c] = {(R)r2[c].re, (R)r2[c].im};
+    }
+    store_link(u, vol, i, s, m);
+}
+
+// Gather one link (possibly daggered / identity) into a 9-complex double slot.
+template <typename R> __global__ void k_get_link(const cplx<R> *u, int64_t vol, int dir, int64_t s, double *out)
+{
+    const int e = threadIdx.x;
+    if (e >= 9) return;
+    if (dir == 0) {
+        out[2 * e] = (e % 4) == 0 ? 1.0 : 0.0;
+        out[2 * e + 1] = 0.0;
+        return;
+    }
+    const int r = e / 3, c = e % 3;
+    const int d = (dir > 0 ? dir : -dir) - 1;
+    cplx<R> v = dir > 0 ? u[((int64_t)d * 9 + 3 * r + c) * vol + s] : conjg(u[((int64_t)d * 9 + 3 * c + r) * vol + s]);
+    // column-major output (Fortran layout)
+    out[2 * (r + 3 * c)] = (double)v.re;
+    out[2 * (r + 3 * c) + 1] = (double)v.im;
+}
+
+template <typename R> __global__ void k_plaq(const cplx<R> *u, geom g, int i, int j, int64_t s, double *out)
+{
+    if (threadIdx.x != 0) return;
+    const int64_t xi = neib_up(g, s, i), xj = neib_up(g, s, j);
+    mat3<double> l[4];
+    mat3<R> a[4] = {load_link(u, g.vol, i, s), load_link(u, g.vol, j, xi), load_link(u, g.vol, i, xj), load_link(u, g.vol, j, s)};
+    for (int k = 0; k < 4; ++k)
+        for (int m = 0; m < 9; ++m) l[k].m[m] = {(double)a[k].m[m].re, (double)a[k].m[m].im};
+    mat3<double> p = mul_nd(mul_nd(mul(l[0], l[1]), l[2]), l[3]);
+    for (int r = 0; r < 3; ++r)
+        for (int c = 0; c < 3; ++c) {
+            out[2 * (r + 3 * c)] = p.m[3 * r + c].re;
+            out[2 * (r + 3 * c) + 1] = p.m[3 * r + c].im;
+        }
+}
+
+// ------------------------------------------------------------------------
+// host side
+// ------------------------------------------------------------------------
+static size_t link_bytes(const nrq_gauge *h)
+{
+    return (size_t)3 * 9 * (size_t)h->vol * (h->precision == 64 ? 16 : 8);
+}
+static size_t efield_bytes(const nrq_gauge *h)
+{
+    return (size_t)3 * 8 * (size_t)h->vol * (h->precision == 64 ? 8 : 4);
+}
+
+static void release(nrq_gauge *h)
+{
+    for (int k = 0; k < 2; ++k)
+        if (h->u[k]) (void)hipFree(h->u[k]);
+    if (h->e) (void)hipFree(h->e);
+    if (h->partials) (void)hipFree(h->partials);
+    if (h->result) (void)hipHostFree(h->result);
+    h->u[0] = h->u[1] = h->e = h->partials = nullptr;
+    h->result = nullptr;
+}
+
+static int allocate(nrq_gauge *h, int precision)
+{
+    release(h);
+    h->precision = precision;
+    h->vol = lat().g.vol;
+    h->cur = 0;
+    h->steps_since_reunit = 0;
+    hipError_t e1 = hipMalloc(&h->u[0], link_bytes(h));
+    hipError_t e2 = hipMalloc(&h->u[1], link_bytes(h));
+    hipError_t e3 = hipMalloc(&h->e, efield_bytes(h));
+    hipError_t e4 = hipMalloc(&h->partials, sizeof(double) * NOBS * REDUCE_BLOCKS + 64 * sizeof(double));
+    hipError_t e5 = hipHostMalloc((void **)&h->result, 64 * sizeof(double), hipHostMallocDefault);
+    if (e1 || e2 || e3 || e4 || e5) {
+        release(h);
+        return fail(NRQ_ERR_ALLOC, "gauge: device allocation of %zu bytes failed", 2 * link_bytes(h) + efield_bytes(h));
+    }
+    return NRQ_OK;
+}
+
+int gauge_obs(const nrq_gauge *h, double out[NOBS])
+{
+    double *part = (double *)h->partials;
+    double *dres = part + NOBS * REDUCE_BLOCKS;
+    const geom g = lat().g;
+    if (h->precision == 64)
+        hipLaunchKernelGGL(k_obs<double>, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, stream(),
+                           (const cplx<double> *)h->u[h->cur], (const double *)h->e, g, part);
+    else
+        hipLaunchKernelGGL(k_obs<float>, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, stream(),
+                           (const cplx<float> *)h->u[h->cur], (const float *)h->e, g, part);
+    NRQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, stream(), part, REDUCE_BLOCKS, NOBS, 4, dres);
+    NRQ_HIP(hipGetLastError());
+    NRQ_HIP(hipMemcpyAsync(h->result, dres, NOBS * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    for (int q = 0; q < NOBS; ++q) out[q] = h->result[q];
+    return NRQ_OK;
+}
+
+template <typename R> static int leapfrog_t(nrq_gauge *h, double sw, int64_t n)
+{
+    const geom g = lat().g;
+    const dim3 blk(256), grd((unsigned)((g.vol + 255) / 256), 3);
+    cplx<R> *u0 = (cplx<R> *)h->u[0], *u1 = (cplx<R> *)h->u[1];
+    R *ef = (R *)h->e;
+    const R dt = (R)sw;
+    auto cur = [&](int c) { return c ? u1 : u0; };
+    if (sw > 0) {
+        hipLaunchKernelGGL(k_links<R>, grd, blk, 0, stream(), cur(h->cur), cur(h->cur), (const R *)ef, g.vol, dt);
+        for (int64_t k = 1; k < n; ++k) {
+            hipLaunchKernelGGL(k_fused<R>, grd, blk, 0, stream(), cur(h->cur), cur(h->cur ^ 1), ef, g, dt);
+            h->cur ^= 1;
+        }
+        hipLaunchKernelGGL(k_efield<R>, grd, blk, 0, stream(), (const cplx<R> *)cur(h->cur), ef, g, dt);
+    } else {
+        for (int64_t k = 0; k < n; ++k) {
+            hipLaunchKernelGGL(k_fused<R>, grd, blk, 0, stream(), cur(h->cur), cur(h->cur ^ 1), ef, g, dt);
+            h->cur ^= 1;
+        }
+    }
+    NRQ_HIP(hipGetLastError());
+    return NRQ_OK;
+}
+
+} // namespace nrq
+
+using namespace nrq;
+
+#define NEED_LATTICE()                                                                  \
+    do {                                                                                \
+        if (!lat().initialised) return fail(NRQ_ERR_STATE, "lattice: not initialised"); \
+    } while (0)
+
+#define NEED_GAUGE(h)                                                                          \
+    do {                                                                                       \
+        if (!(h)) return fail(NRQ_ERR_ARG, "gauge: null handle");                              \
+        if (!(h)->u[0]) return fail(NRQ_ERR_STATE, "gauge: not allocated");                    \
+        if ((h)->vol != lat().g.vol) return fail(NRQ_ERR_STATE, "gauge: lattice changed since allocation"); \
+    } while (0)
+
+extern "C" int nrq_gauge_create(int precision, nrq_gauge **out)
+{
+    NEED_LATTICE();
+    if (!out) return fail(NRQ_ERR_ARG, "gauge: null output handle");
+    if (precision != 64 && precision != 32) return fail(NRQ_ERR_ARG, "gauge: precision must be 64 or 32");
+    if (int r = ensure_device()) return r;
+    nrq_gauge *h = new nrq_gauge();
+    if (int r = allocate(h, precision)) {
+        delete h;
+        return r;
+    }
+    // defined contents: cold configuration
+    *out = h;
+    return nrq_gauge_cold(h);
+}
+
+extern "C" int nrq_gauge_destroy(nrq_gauge *h)
+{
+    if (!h) return NRQ_OK;
+    if (stream()) (void)hipStreamSynchronize(stream());
+    release(h);
+    delete h;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_copy(nrq_gauge *dst, const nrq_gauge *src)
+{
+    NEED_GAUGE(src);
+    if (!dst) return fail(NRQ_ERR_ARG, "gauge: null destination");
+    if (dst == src) return NRQ_OK;
+    if (!dst->u[0] || dst->precision != src->precision || dst->vol != src->vol)
+        if (int r = allocate(dst, src->precision)) return r;
+    NRQ_HIP(hipMemcpyAsync(dst->u[0], src->u[src->cur], link_bytes(src), hipMemcpyDeviceToDevice, stream()));
+    NRQ_HIP(hipMemcpyAsync(dst->e, src->e, efield_bytes(src), hipMemcpyDeviceToDevice, stream()));
+    dst->cur = 0;
+    dst->steps_since_reunit = src->steps_since_reunit;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_cold(nrq_gauge *h)
+{
+    NEED_GAUGE(h);
+    const int64_t v = h->vol;
+    h->cur = 0;
+    if (h->precision == 64)
+        hipLaunchKernelGGL(k_cold<double>, dim3((unsigned)((v + 255) / 256)), dim3(256), 0, stream(),
+                           (cplx<double> *)h->u[0], (double *)h->e, v);
+    else
+        hipLaunchKernelGGL(k_cold<float>, dim3((unsigned)((v + 255) / 256)), dim3(256), 0, stream(),
+                           (cplx<float> *)h->u[0], (float *)h->e, v);
+    NRQ_HIP(hipGetLastError());
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_upload(nrq_gauge *h, const double *links, const double *efield)
+{
+    NEED_GAUGE(h);
+    const int64_t v = h->vol;
+    const bool dp = h->precision == 64;
+    if (links) {
+        std::vector<char> buf(link_bytes(h));
+        for (int64_t s = 0; s < v; ++s)
+            for (int i = 0; i < 3; ++i)
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) {
+                        const double *src = links + 2 * (r + 3 * c + 9 * i + 27 * s);
+                        const size_t at = ((size_t)i * 9 + 3 * r + c) * (size_t)v + (size_t)s;
+                        if (dp) {
+                            ((double *)buf.data())[2 * at] = src[0];
+                            ((double *)buf.data())[2 * at + 1] = src[1];
+                        } else {
+                            ((float *)buf.data())[2 * at] = (float)src[0];
+                            ((float *)buf.data())[2 * at + 1] = (float)src[1];
+                        }
+                    }
+        NRQ_HIP(hipMemcpyAsync(h->u[h->cur], buf.data(), buf.size(), hipMemcpyHostToDevice, stream()));
+        NRQ_HIP(hipStreamSynchronize(stream()));
+    }
+    if (efield) {
+        std::vector<char> buf(efield_bytes(h));
+        for (int64_t s = 0; s < v; ++s)
+            for (int i = 0; i < 3; ++i)
+                for (int a = 0; a < 8; ++a) {
+                    const double x = efield[a + 8 * i + 24 * s];
+                    const size_t at = ((size_t)i * 8 + a) * (size_t)v + (size_t)s;
+                    if (dp) ((double *)buf.data())[at] = x;
+                    else ((float *)buf.data())[at] = (float)x;
+                }
+        NRQ_HIP(hipMemcpyAsync(h->e, buf.data(), buf.size(), hipMemcpyHostToDevice, stream()));
+        NRQ_HIP(hipStreamSynchronize(stream()));
+    }
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_download(const nrq_gauge *h, double *links, double *efield)
+{
+    NEED_GAUGE(h);
+    const int64_t v = h->vol;
+    const bool dp = h->precision == 64;
+    if (links) {
+        std::vector<char> buf(link_bytes(h));
+        NRQ_HIP(hipMemcpyAsync(buf.data(), h->u[h->cur], buf.size(), hipMemcpyDeviceToHost, stream()));
+        NRQ_HIP(hipStreamSynchronize(stream()));
+        for (int64_t s = 0; s < v; ++s)
+            for (int i = 0; i < 3; ++i)
+                for (int r = 0; r < 3; ++r)
+                    for (int c = 0; c < 3; ++c) {
+                        double *dst = links + 2 * (r + 3 * c + 9 * i + 27 * s);
+                        const size_t at = ((size_t)i * 9 + 3 * r + c) * (size_t)v + (size_t)s;
+                        if (dp) {
+                            dst[0] = ((double *)buf.data())[2 * at];
+                            dst[1] = ((double *)buf.data())[2 * at + 1];
+                        } else {
+                            dst[0] = ((float *)buf.data())[2 * at];
+                            dst[1] = ((float *)buf.data())[2 * at + 1];
+                        }
+                    }
+    }
+    if (efield) {
+        std::vector<char> buf(efield_bytes(h));
+        NRQ_HIP(hipMemcpyAsync(buf.data(), h->e, buf.size(), hipMemcpyDeviceToHost, stream()));
+        NRQ_HIP(hipStreamSynchronize(stream()));
+        for (int64_t s = 0; s < v; ++s)
+            for (int i = 0; i < 3; ++i)
+                for (int a = 0; a < 8; ++a) {
+                    const size_t at = ((size_t)i * 8 + a) * (size_t)v + (size_t)s;
+                    efield[a + 8 * i + 24 * s] = dp ? ((double *)buf.data())[at] : ((float *)buf.data())[at];
+                }
+    }
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_get_link(const nrq_gauge *h, int dir, int64_t idx, double link[18])
+{
+    NEED_GAUGE(h);
+    if (!link) return fail(NRQ_ERR_ARG, "null argument");
+    if (dir < -3 || dir > 3) return fail(NRQ_ERR_ARG, "direction %d out of range", dir);
+    if (idx < 1 || idx > h->vol) return fail(NRQ_ERR_ARG, "lattice index out of range");
+    int64_t s = idx - 1;
+    if (dir < 0) {
+        int64_t nb;
+        if (int r = nrq_lattice_neib(dir, idx, &nb)) return r;
+        s = nb - 1;
+    }
+    double *dres = (double *)h->partials + NOBS * REDUCE_BLOCKS;
+    if (h->precision == 64)
+        hipLaunchKernelGGL(k_get_link<double>, dim3(1), dim3(64), 0, stream(), (const cplx<double> *)h->u[h->cur], h->vol, dir, s, dres);
+    else
+        hipLaunchKernelGGL(k_get_link<float>, dim3(1), dim3(64), 0, stream(), (const cplx<float> *)h->u[h->cur], h->vol, dir, s, dres);
+    NRQ_HIP(hipGetLastError());
+    NRQ_HIP(hipMemcpyAsync(h->result, dres, 18 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    memcpy(link, h->result, 18 * sizeof(double));
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_get_efield(const nrq_gauge *h, int dir, int64_t idx, double e[8])
+{
+    NEED_GAUGE(h);
+    if (!e) return fail(NRQ_ERR_ARG, "null argument");
+    if (dir < 1 || dir > 3) return fail(NRQ_ERR_ARG, "direction %d out of range", dir);
+    if (idx < 1 || idx > h->vol) return fail(NRQ_ERR_ARG, "lattice index out of range");
+    const size_t es = h->precision == 64 ? 8 : 4;
+    char tmp[8][8];
+    for (int a = 0; a < 8; ++a)
+        NRQ_HIP(hipMemcpyAsync(tmp[a], (char *)h->e + (((size_t)(dir - 1) * 8 + a) * (size_t)h->vol + (size_t)(idx - 1)) * es, es,
+                               hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    for (int a = 0; a < 8; ++a) e[a] = es == 8 ? *(double *)tmp[a] : (double)*(float *)tmp[a];
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps)
+{
+    NEED_GAUGE(h);
+    if (nsteps < 0 || !std::isfinite(stepwidth)) return fail(NRQ_ERR_ARG, "leapfrog: bad step count or width");
+    if (nsteps == 0) return NRQ_OK;
+    int r = h->precision == 64 ? leapfrog_t<double>(h, stepwidth, nsteps) : leapfrog_t<float>(h, stepwidth, nsteps);
+    h->steps_since_reunit += nsteps;
+    return r;
+}
+
+extern "C" int nrq_gauge_observables(const nrq_gauge *h, double out[4])
+{
+    NEED_GAUGE(h);
+    if (!out) return fail(NRQ_ERR_ARG, "null argument");
+    double o[NOBS];
+    if (int r = gauge_obs(h, o)) return r;
+    out[0] = o[0] + o[1];
+    out[1] = o[3];
+    out[2] = o[4];
+    out[3] = o[2] / (3.0 * (double)h->vol);
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_energy(const nrq_gauge *h, double *v)
+{
+    double o[4];
+    if (!v) return fail(NRQ_ERR_ARG, "null argument");
+    if (int r = nrq_gauge_observables(h, o)) return r;
+    *v = o[0];
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_gauss_ref(const nrq_gauge *h, double *v)
+{
+    double o[4];
+    if (!v) return fail(NRQ_ERR_ARG, "null argument");
+    if (int r = nrq_gauge_observables(h, o)) return r;
+    *v = o[1];
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_gauss_max(const nrq_gauge *h, double *v)
+{
+    double o[4];
+    if (!v) return fail(NRQ_ERR_ARG, "null argument");
+    if (int r = nrq_gauge_observables(h, o)) return r;
+    *v = o[2];
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_mean_plaquette(const nrq_gauge *h, double *v)
+{
+    double o[4];
+    if (!v) return fail(NRQ_ERR_ARG, "null argument");
+    if (int r = nrq_gauge_observables(h, o)) return r;
+    *v = o[3];
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_plaquette(const nrq_gauge *h, int i, int j, int64_t idx, double P[18])
+{
+    NEED_GAUGE(h);
+    if (!P) return fail(NRQ_ERR_ARG, "null argument");
+    if (i < 1 || i > 3 || j < 1 || j > 3) return fail(NRQ_ERR_ARG, "plaquette: spatial directions 1..3 only");
+    if (idx < 1 || idx > h->vol) return fail(NRQ_ERR_ARG, "lattice index out of range");
+    double *dres = (double *)h->partials + NOBS * REDUCE_BLOCKS;
+    const geom g = lat().g;
+    if (h->precision == 64)
+        hipLaunchKernelGGL(k_plaq<double>, dim3(1), dim3(64), 0, stream(), (const cplx<double> *)h->u[h->cur], g, i - 1, j - 1, idx - 1, dres);
+    else
+        hipLaunchKernelGGL(k_plaq<float>, dim3(1), dim3(64), 0, stream(), (const cplx<float> *)h->u[h->cur], g, i - 1, j - 1, idx - 1, dres);
+    NRQ_HIP(hipGetLastError());
+    NRQ_HIP(hipMemcpyAsync(h->result, dres, 18 * sizeof(double), hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    memcpy(P, h->result, 18 * sizeof(double));
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_reunitarise(nrq_gauge *h)
+{
+    NEED_GAUGE(h);
+    const dim3 grd((unsigned)((h->vol + 255) / 256), 3);
+    if (h->precision == 64)
+        hipLaunchKernelGGL(k_reunit<double>, grd, dim3(256), 0, stream(), (cplx<double> *)h->u[h->cur], h->vol);
+    else
+        hipLaunchKernelGGL(k_reunit<float>, grd, dim3(256), 0, stream(), (cplx<float> *)h->u[h->cur], h->vol);
+    NRQ_HIP(hipGetLastError());
+    h->steps_since_reunit = 0;
+    return NRQ_OK;
+}
+
+// Accessors used by the other translation units.
+namespace nrq {
+const void *gauge_links(const nrq_gauge *h) { return h->u[h->cur]; }
+void *gauge_links_mut(nrq_gauge *h) { return h->u[h->cur]; }
+void *gauge_efield(nrq_gauge *h) { return h->e; }
+}

@@ -1,0 +1,67 @@
+// nrq_internal.h -- host-side state shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+#include <stddef.h>
+#include <string>
+#include "nrq_su3.h"
+#include "../../include/nrqcd_amd.h"
+
+namespace nrq {
+
+// Error reporting: every entry point returns an NRQ_* status and leaves a
+// message retrievable with nrq_last_error().
+int fail(int code, const char *fmt, ...);
+int check_hip(hipError_t e, const char *what);
+
+#define NRQ_HIP(call)                                                     \
+    do {                                                                  \
+        hipError_t nrq_e_ = (call);                                       \
+        if (nrq_e_ != hipSuccess) return ::nrq::check_hip(nrq_e_, #call); \
+    } while (0)
+
+struct lattice_state {
+    bool initialised = false;
+    geom g{};
+};
+lattice_state &lat();
+
+// The library's single HIP stream (default priority); every kernel and copy
+// is ordered on it.  Entry points that return host values synchronise it.
+hipStream_t stream();
+int ensure_device();
+
+// Host ranlxd (level 1/2) used for the seeded initial condition.
+int rng_init(int level, int64_t seed);
+void rng_uniform(double *r, int64_t n);
+bool rng_ready();
+
+// Deterministic two-stage reductions: a fixed grid of REDUCE_BLOCKS blocks
+// writes partials, one block folds them in a fixed order.
+constexpr int REDUCE_BLOCKS = 1024;
+constexpr int REDUCE_THREADS = 256;
+
+struct scratch_buf {
+    void *ptr = nullptr;
+    size_t bytes = 0;
+    int reserve(size_t n);
+    ~scratch_buf();
+};
+
+} // namespace nrq
+
+struct nrq_gauge {
+    int precision = 64;       // 64: double storage, 32: float storage
+    int64_t vol = 0;
+    void *u[2] = {nullptr, nullptr}; // ping-pong link buffers [dir][elem][site] complex
+    int cur = 0;              // which u[] holds the current links
+    void *e = nullptr;        // E field [dir][a][site] real
+    void *partials = nullptr; // reduction partials
+    double *result = nullptr; // pinned host result slots
+    int64_t steps_since_reunit = 0;
+};
+
+struct nrq_heavy {
+    int64_t vol = 0;
+    void *psi = nullptr;      // 24 colour-vector fields [field][colour][site] complex double
+};

@@ -1,0 +1,276 @@
+// nrq_su3.h -- device-side complex and SU(3) arithmetic for gfx950.
+//
+// Matrices live in registers as 9 complex numbers, row-major (m[3*r + c]).
+// Everything is templated on the storage/compute real type R (double or
+// float).  The generator convention is the reference's (su3.f90:97-131): the
+// Fortran reshape fills column-major, so T^2, T^5 and T^7 are the complex
+// conjugates of the textbook Gell-Mann matrices.  Written out, with
+// s = 1/(2 sqrt 3):
+//   alpha^a T^a = [[ a3/2 + a8 s,   (a1 + i a2)/2,  (a4 + i a5)/2 ],
+//                  [ (a1 - i a2)/2, -a3/2 + a8 s,   (a6 + i a7)/2 ],
+//                  [ (a4 - i a5)/2, (a6 - i a7)/2,  -2 a8 s       ]]
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace nrq {
+
+template <typename R> struct cplx {
+    R re, im;
+};
+
+template <typename R> __device__ __forceinline__ cplx<R> mk(R a, R b) { return {a, b}; }
+template <typename R> __device__ __forceinline__ cplx<R> operator+(cplx<R> a, cplx<R> b) { return {a.re + b.re, a.im + b.im}; }
+template <typename R> __device__ __forceinline__ cplx<R> operator-(cplx<R> a, cplx<R> b) { return {a.re - b.re, a.im - b.im}; }
+template <typename R> __device__ __forceinline__ cplx<R> operator*(cplx<R> a, cplx<R> b)
+{
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+template <typename R> __device__ __forceinline__ cplx<R> operator*(R s, cplx<R> a) { return {s * a.re, s * a.im}; }
+template <typename R> __device__ __forceinline__ cplx<R> conjg(cplx<R> a) { return {a.re, -a.im}; }
+// a * conj(b)
+template <typename R> __device__ __forceinline__ cplx<R> mulc(cplx<R> a, cplx<R> b)
+{
+    return {a.re * b.re + a.im * b.im, a.im * b.re - a.re * b.im};
+}
+// conj(a) * b
+template <typename R> __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R> b)
+{
+    return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
+}
+
+template <typename R> struct mat3 {
+    cplx<R> m[9];
+};
+
+template <typename R> __device__ __forceinline__ void set_zero(mat3<R> &a)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.m[k] = {R(0), R(0)};
+}
+
+// C = A * B
+template <typename R> __device__ __forceinline__ mat3<R> mul(const mat3<R> &a, const mat3<R> &b)
+{
+    mat3<R> c;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cplx<R> s = a.m[3 * r] * b.m[k];
+            s = s + a.m[3 * r + 1] * b.m[3 + k];
+            s = s + a.m[3 * r + 2] * b.m[6 + k];
+            c.m[3 * r + k] = s;
+        }
+    return c;
+}
+
+// C = A * B^dagger
+template <typename R> __device__ __forceinline__ mat3<R> mul_nd(const mat3<R> &a, const mat3<R> &b)
+{
+    mat3<R> c;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cplx<R> s = mulc(a.m[3 * r], b.m[3 * k]);
+            s = s + mulc(a.m[3 * r + 1], b.m[3 * k + 1]);
+            s = s + mulc(a.m[3 * r + 2], b.m[3 * k + 2]);
+            c.m[3 * r + k] = s;
+        }
+    return c;
+}
+
+// C = A^dagger * B
+template <typename R> __device__ __forceinline__ mat3<R> mul_dn(const mat3<R> &a, const mat3<R> &b)
+{
+    mat3<R> c;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            cplx<R> s = cmul(a.m[r], b.m[k]);
+            s = s + cmul(a.m[3 + r], b.m[3 + k]);
+            s = s + cmul(a.m[6 + r], b.m[6 + k]);
+            c.m[3 * r + k] = s;
+        }
+    return c;
+}
+
+// C = A^dagger * B^dagger = (B A)^dagger
+template <typename R> __device__ __forceinline__ mat3<R> mul_dd(const mat3<R> &a, const mat3<R> &b)
+{
+    mat3<R> c;
+#pragma unroll
+    for (int r = 0; r < 3; ++r)
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            // sum_j conj(a[j][r]) conj(b[k][j])
+            cplx<R> s = conjg(a.m[r] * b.m[3 * k]);
+            s = s + conjg(a.m[3 + r] * b.m[3 * k + 1]);
+            s = s + conjg(a.m[6 + r] * b.m[3 * k + 2]);
+            c.m[3 * r + k] = s;
+        }
+    return c;
+}
+
+template <typename R> __device__ __forceinline__ void add_to(mat3<R> &a, const mat3<R> &b)
+{
+#pragma unroll
+    for (int k = 0; k < 9; ++k) a.m[k] = a.m[k] + b.m[k];
+}
+
+template <typename R> __device__ __forceinline__ cplx<R> trace(const mat3<R> &a) { return a.m[0] + a.m[4] + a.m[8]; }
+
+// Hermitian algebra matrix alpha^a T^a (su3.f90:287-301 with the generator
+// table written out in the header comment).
+template <typename R> __device__ __forceinline__ mat3<R> algebra_matrix(const R *al)
+{
+    const R h = R(0.5);
+    const R s = R(0.28867513459481288225457439025098); // 1/(2 sqrt 3)
+    mat3<R> q;
+    q.m[0] = {h * al[2] + s * al[7], R(0)};
+    q.m[4] = {-h * al[2] + s * al[7], R(0)};
+    q.m[8] = {R(-2) * s * al[7], R(0)};
+    q.m[1] = {h * al[0], h * al[1]};
+    q.m[3] = {h * al[0], -h * al[1]};
+    q.m[2] = {h * al[3], h * al[4]};
+    q.m[6] = {h * al[3], -h * al[4]};
+    q.m[5] = {h * al[5], h * al[6]};
+    q.m[7] = {h * al[5], -h * al[6]};
+    return q;
+}
+
+// out[a] = 2 Im Tr(T^a W) for a = 1..8 (su3.f90:369-381 with the generator
+// table written out in the header comment).
+template <typename R> __device__ __forceinline__ void two_im_trace_gen(const mat3<R> &w, R *out)
+{
+    const R s3 = R(0.57735026918962576450914878050196); // 1/sqrt 3
+    out[0] = w.m[1].im + w.m[3].im;
+    out[1] = w.m[3].re - w.m[1].re;
+    out[2] = w.m[0].im - w.m[4].im;
+    out[3] = w.m[2].im + w.m[6].im;
+    out[4] = w.m[6].re - w.m[2].re;
+    out[5] = w.m[5].im + w.m[7].im;
+    out[6] = w.m[7].re - w.m[5].re;
+    out[7] = s3 * (w.m[0].im + w.m[4].im - R(2) * w.m[8].im);
+}
+
+// Tr(T^a G) for a Hermitian G is real; out[a] = 2 Tr(T^a G) = 2 Re Tr(T^a G)
+template <typename R> __device__ __forceinline__ void two_re_trace_gen(const mat3<R> &w, R *out)
+{
+    const R s3 = R(0.57735026918962576450914878050196);
+    out[0] = w.m[1].re + w.m[3].re;
+    out[1] = w.m[1].im - w.m[3].im;
+    out[2] = w.m[0].re - w.m[4].re;
+    out[3] = w.m[2].re + w.m[6].re;
+    out[4] = w.m[2].im - w.m[6].im;
+    out[5] = w.m[5].re + w.m[7].re;
+    out[6] = w.m[5].im - w.m[7].im;
+    out[7] = s3 * (w.m[0].re + w.m[4].re - R(2) * w.m[8].re);
+}
+
+template <typename R> struct exp_eps;
+template <> struct exp_eps<double> { static constexpr double v = 1e-17; };
+template <> struct exp_eps<float> { static constexpr float v = 1e-8f; };
+
+// exp(i Q) for a traceless Hermitian Q by Cayley-Hamilton: Q^3 = c1 Q + c0 I
+// with c1 = tr(Q^2)/2, c0 = det Q, so exp(iQ) = f0 I + f1 Q + f2 Q^2 where the
+// f's are the power series sum_n i^n (a_n, b_n, c_n)/n! of the recurrence
+// (a,b,c)_{n+1} = (c0 c_n, a_n + c1 c_n, b_n).  Q is first scaled by 2^-s so
+// that its spectral radius is <= 1 (|lambda|_max^2 <= 4 c1 / 3 for traceless
+// Hermitian 3x3), and the result is squared s times.
+template <typename R> __device__ __forceinline__ mat3<R> exp_i_hermitian(mat3<R> q)
+{
+    // Q^2 (Hermitian): compute all entries from rows of Q and columns of Q
+    mat3<R> q2 = mul(q, q);
+    R c1 = R(0.5) * (q2.m[0].re + q2.m[4].re + q2.m[8].re);
+    int s = 0;
+    R rho2 = R(4) / R(3) * c1;
+    R scale = R(1);
+    while (rho2 > R(1) && s < 64) { // bounded: a non-finite input cannot loop forever
+        rho2 *= R(0.25);
+        scale *= R(0.5);
+        ++s;
+    }
+    if (s) {
+#pragma unroll
+        for (int k = 0; k < 9; ++k) q.m[k] = scale * q.m[k];
+#pragma unroll
+        for (int k = 0; k < 9; ++k) q2.m[k] = (scale * scale) * q2.m[k];
+        c1 *= scale * scale;
+    }
+    // det Q (real for Hermitian Q)
+    cplx<R> d = q.m[0] * (q.m[4] * q.m[8] - q.m[5] * q.m[7]) - q.m[1] * (q.m[3] * q.m[8] - q.m[5] * q.m[6]) +
+                q.m[2] * (q.m[3] * q.m[7] - q.m[4] * q.m[6]);
+    R c0 = d.re;
+    // series
+    R a = 1, b = 0, c = 0;            // (a,b,c)_n / n!
+    R f0r = 1, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
+    for (int n = 0; n < 40; ++n) {
+        R na = c0 * c, nb = a + c1 * c, nc = b;
+        R inv = R(1) / R(n + 1);
+        a = na * inv;
+        b = nb * inv;
+        c = nc * inv;
+        // multiply by i^(n+1)
+        switch ((n + 1) & 3) {
+        case 0: f0r += a; f1r += b; f2r += c; break;
+        case 1: f0i += a; f1i += b; f2i += c; break;
+        case 2: f0r -= a; f1r -= b; f2r -= c; break;
+        default: f0i -= a; f1i -= b; f2i -= c; break;
+        }
+        R mag = fabs(a) + fabs(b) + fabs(c);
+        if (mag < exp_eps<R>::v && n >= 2) break;
+    }
+    cplx<R> f0 = {f0r, f0i}, f1 = {f1r, f1i}, f2 = {f2r, f2i};
+    mat3<R> x;
+#pragma unroll
+    for (int k = 0; k < 9; ++k) x.m[k] = f1 * q.m[k] + f2 * q2.m[k];
+    x.m[0] = x.m[0] + f0;
+    x.m[4] = x.m[4] + f0;
+    x.m[8] = x.m[8] + f0;
+    for (int k = 0; k < s; ++k) x = mul(x, x);
+    return x;
+}
+
+// Host-visible lattice geometry, passed by value to kernels.
+struct geom {
+    int32_t n0, n1, n2;   // extents in x, y, z
+    int64_t vol;          // n0 n1 n2
+    double a[4];          // a0 (time), a1, a2, a3
+};
+
+// Periodic neighbour of site s (0-based, x fastest) in direction d (0,1,2)
+// and sign +1/-1 (lattice.f90:846-874 on 0-based indices).
+__device__ __forceinline__ int64_t neib_up(const geom &g, int64_t s, int d)
+{
+    if (d == 0) {
+        int64_t x = s % g.n0;
+        return x == g.n0 - 1 ? s - (g.n0 - 1) : s + 1;
+    } else if (d == 1) {
+        int64_t y = (s / g.n0) % g.n1;
+        return y == g.n1 - 1 ? s - (int64_t)(g.n1 - 1) * g.n0 : s + g.n0;
+    } else {
+        int64_t pl = (int64_t)g.n0 * g.n1;
+        int64_t z = s / pl;
+        return z == g.n2 - 1 ? s - (int64_t)(g.n2 - 1) * pl : s + pl;
+    }
+}
+
+__device__ __forceinline__ int64_t neib_dn(const geom &g, int64_t s, int d)
+{
+    if (d == 0) {
+        int64_t x = s % g.n0;
+        return x == 0 ? s + (g.n0 - 1) : s - 1;
+    } else if (d == 1) {
+        int64_t y = (s / g.n0) % g.n1;
+        return y == 0 ? s + (int64_t)(g.n1 - 1) * g.n0 : s - g.n0;
+    } else {
+        int64_t pl = (int64_t)g.n0 * g.n1;
+        int64_t z = s / pl;
+        return z == 0 ? s + (int64_t)(g.n2 - 1) * pl : s - pl;
+    }
+}
+
+} // namespace nrq

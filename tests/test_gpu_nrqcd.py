@@ -1,0 +1,133 @@
+"""GPU parity of the NRQCD Crank-Nicolson half of the hot path.
+
+Both sides solve to an FP64 true relative residual of 1e-12; the linear
+solution is unique, so the propagators must agree to ~cond * 1e-12 (the
+system is within 0.3% of the identity here): tolerance 1e-10 relative.
+"""
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+SP = (0.05, 1.0, 1.0, 1.0)
+C1 = [1.0, 0.0, 0.0, 0.0]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+@pytest.fixture()
+def cn(nrq, lattice):
+    """Factory: lattice::InitModule, then nrqcd::InitModule (the reference's
+    order, nrqcd.f90:167-192 checks the lattice module first)."""
+    state = {"on": False}
+
+    def init(ext, spacing, tol=1e-12):
+        lattice(ext, spacing)
+        nrq.nrqcd_init(2.0, C1, 1.0, tol)
+        state["on"] = True
+
+    yield init
+    if state["on"]:
+        nrq.nrqcd_finalize()
+
+
+def test_cn_step_matches_oracle(nrq, oracle, cn):
+    ext = (8, 8, 8)
+    _say("CN 3 steps 8^3")
+    cn(ext, SP)
+    oracle.lattice_init(ext, SP)
+    U, E = oracle.box_init(1, 0.5, 1.0, 1.0)
+    g = nrq.GaugeConfiguration()
+    g.upload(U, E)
+    h = nrq.NRQCDField()
+    h.InitSinglePoint(1, 1)
+    Q = oracle.point_prop(1)
+    A = oracle.point_prop(1)
+    for step in range(3):
+        st = h.Update(g, 2.0, C1)
+        rc, ost = oracle.cn_update(U, Q, A, 2.0, C1, 1.0, 1e-12)
+        assert rc == 0
+        assert st.solves == 12
+        assert st.max_true_relres <= 1e-12
+    Qg, Ag = h.download()
+    scale = np.max(np.abs(Q))
+    assert np.max(np.abs(Qg - Q)) < 1e-10 * scale
+    assert np.max(np.abs(Ag - A)) < 1e-10 * scale
+    nq, na = h.norms()
+    assert nq == pytest.approx(oracle.prop_norm(Q), rel=1e-11)
+    assert na == pytest.approx(oracle.prop_norm(A), rel=1e-11)
+    corr = h.correlators()
+    ocorr = oracle.correlators(Q, A)
+    for k in ocorr:
+        assert abs(corr[k] - ocorr[k]) < 1e-10 * max(1.0, abs(ocorr[k])), k
+
+
+def test_cn_general_propagator_and_anisotropy(nrq, oracle, cn):
+    ext = (4, 6, 5)
+    sp = (0.1, 1.0, 0.7, 1.2)
+    _say("CN random propagator 4x6x5")
+    cn(ext, sp)
+    oracle.lattice_init(ext, sp)
+    U, E = oracle.box_init(9, 0.7, 1.0, 1.0)
+    rng = np.random.default_rng(3)
+    v = oracle.vol
+    Q = np.asfortranarray(rng.normal(size=(6, 6, v)) + 1j * rng.normal(size=(6, 6, v)))
+    A = np.asfortranarray(rng.normal(size=(6, 6, v)) + 1j * rng.normal(size=(6, 6, v)))
+    c = [0.8 + 0.1j, 0, 0, 0]
+    g = nrq.GaugeConfiguration()
+    g.upload(U, E)
+    h = nrq.NRQCDField()
+    h.upload(Q, A)
+    Qd, Ad = h.download()
+    assert np.array_equal(Qd, Q) and np.array_equal(Ad, A)
+    h.Update(g, 1.3, c, 0.5)
+    rc, _ = oracle.cn_update(U, Q, A, 1.3, c, 0.5, 1e-12)
+    assert rc == 0
+    Qg, Ag = h.download()
+    assert np.max(np.abs(Qg - Q)) < 1e-10 * np.max(np.abs(Q))
+    assert np.max(np.abs(Ag - A)) < 1e-10 * np.max(np.abs(A))
+
+
+def test_cn_rejects_higher_wilson_terms(nrq, cn):
+    cn((4, 4, 4), SP)
+    g = nrq.GaugeConfiguration()
+    h = nrq.NRQCDField()
+    h.InitSinglePoint(1, 1)
+    with pytest.raises(nrq.NRQError) as e:
+        h.Update(g, 2.0, [1, 0.5, 0, 0])
+    assert e.value.code == -4
+
+
+def test_cn_norm_conservation_and_reversibility(nrq, cn):
+    ext = (8, 8, 8)
+    _say("CN reversibility 8^3")
+    cn(ext, SP)
+    nrq.random_init(11)
+    g = nrq.GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    h = nrq.NRQCDField()
+    h.InitSinglePoint(1, 1)
+    Q0, A0 = h.download()
+    for _ in range(5):
+        h.Update(g, 2.0, C1)
+    nq, na = h.norms()
+    assert abs(nq - 1.0) < 1e-10 and abs(na - 1.0) < 1e-10
+    for _ in range(5):
+        h.Update(g, 2.0, C1, -1.0)
+    Q1, A1 = h.download()
+    assert np.max(np.abs(Q1 - Q0)) < 1e-10
+    assert np.max(np.abs(A1 - A0)) < 1e-10
+
+
+def test_init_single_dof_matches_reference_indexing(nrq, lattice):
+    lattice((4, 4, 4), SP)
+    h = nrq.NRQCDField()
+    h.InitSinglePointSingleDoF(2, 3, 5, 1, 2, 7)
+    Q, A = h.download()
+    # nrqcd.f90:437-444: index = colour + 3 (spin - 1)
+    iq = 3 + 3 * (2 - 1)
+    ia = 2 + 3 * (1 - 1)
+    assert Q[iq - 1, iq - 1, 5 - 1] == 1.0 and np.count_nonzero(Q) == 1
+    assert A[ia - 1, ia - 1, 7 - 1] == 1.0 and np.count_nonzero(A) == 1
