@@ -34,9 +34,9 @@ HIPCC_FLAGS = [
     "-I", str(ROOT / "include"),
 ]
 FORTRAN_MODULES = [
-    "mpiinterface.f90",
     "precision.f90",
     "nrq_capi.f90",
+    "mpiinterface.f90",
     "lattice.f90",
     "random.f90",
     "gaugeconfiguration_su3.f90",

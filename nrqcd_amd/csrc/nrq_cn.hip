@@ -421,98 +421,99 @@ static cn_workspace g_ws;
 
 static int upload_active(const std::vector<int> &act)
 {
-    NRQ_HIP(hipMemcpyAsync(g_ws.active, act.data(), sizeof(int) * NFIELDS, hipMemcpyHostToDevice, stream()));
+    NRQ_HIP(hipMemcpyAsync(g_ws.active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, stream()));
     return NRQ_OK;
 }
 
-// Applies y_r = x_r + beta_r K x_r for the active fields.
-static int apply_op(const nrq_gauge *gf, const cplx<double> *in, cplx<double> *out)
+// Applies y_r = x_r + beta_r K x_r for the active fields r < nf.
+static int apply_op(const nrq_gauge *gf, const cplx<double> *in, cplx<double> *out, int nf)
 {
     const geom g = lat().g;
-    const dim3 grd((unsigned)((g.vol + 255) / 256), (NFIELDS + RCHUNK - 1) / RCHUNK);
+    const dim3 grd((unsigned)((g.vol + 255) / 256), (nf + RCHUNK - 1) / RCHUNK);
     if (gf->precision == 64)
         hipLaunchKernelGGL(k_stencil<double>, grd, dim3(256), 0, stream(), (const cplx<double> *)gauge_links(gf), in, out,
-                           g_ws.beta, g_ws.active, NFIELDS, g);
+                           g_ws.beta, g_ws.active, nf, g);
     else
         hipLaunchKernelGGL(k_stencil<float>, grd, dim3(256), 0, stream(), (const cplx<float> *)gauge_links(gf), in, out,
-                           g_ws.beta, g_ws.active, NFIELDS, g);
+                           g_ws.beta, g_ws.active, nf, g);
     NRQ_HIP(hipGetLastError());
     return NRQ_OK;
 }
 
-// Squared norms of the active fields of v into host array out[NFIELDS].
-static int field_norms2(const cplx<double> *v, double *out)
+// Squared norms of the active fields r < nf of v into host array out[nf].
+static int field_norms2(const cplx<double> *v, double *out, int nf)
 {
     const int64_t n = 3 * g_ws.vol;
-    hipLaunchKernelGGL(k_norm2, dim3(VBLOCKS, NFIELDS), dim3(VTHREADS), 0, stream(), v, n, g_ws.active, g_ws.rpart);
-    hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, NFIELDS, g_ws.active, g_ws.nrm);
+    hipLaunchKernelGGL(k_norm2, dim3(VBLOCKS, nf), dim3(VTHREADS), 0, stream(), v, n, g_ws.active, g_ws.rpart);
+    hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, nf, g_ws.active, g_ws.nrm);
     NRQ_HIP(hipGetLastError());
-    NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * NFIELDS, hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * nf, hipMemcpyDeviceToHost, stream()));
     NRQ_HIP(hipStreamSynchronize(stream()));
-    memcpy(out, g_ws.nbuf, sizeof(double) * NFIELDS);
+    memcpy(out, g_ws.nbuf, sizeof(double) * nf);
     return NRQ_OK;
 }
 
 // Batched restarted GMRES(m), classical Gram-Schmidt, zero initial guess,
 // on the fields flagged in `act`: solves (1 + conj(alpha_r) K) x_r = b_r to
 // ||b_r - A x_r|| <= tol ||b_r|| (FP64 true residual).
-static int gmres_batched(const nrq_gauge *gf, std::vector<int> act, const double *bnorm2, nrq_solve_stats *st)
+static int gmres_batched(const nrq_gauge *gf, int R, const double *bnorm2, nrq_solve_stats *st)
 {
     const int m = g_ws.m;
     const int64_t n = 3 * g_ws.vol;
     const size_t fbytes = sizeof(cplx<double>) * (size_t)n;
     typedef std::complex<double> zc;
-    std::vector<double> bn(NFIELDS), rn(NFIELDS);
-    std::vector<int64_t> its(NFIELDS, 0);
-    for (int r = 0; r < NFIELDS; ++r) bn[r] = std::sqrt(bnorm2[r]);
-    NRQ_HIP(hipMemsetAsync(g_ws.x, 0, fbytes * NFIELDS, stream()));
+    std::vector<int> act(R, 1);
+    std::vector<double> bn(R), rn(R);
+    std::vector<int64_t> its(R, 0);
+    for (int r = 0; r < R; ++r) bn[r] = std::sqrt(bnorm2[r]);
+    NRQ_HIP(hipMemsetAsync(g_ws.x, 0, fbytes * R, stream()));
     // r0 = b
-    NRQ_HIP(hipMemcpyAsync(g_ws.res, g_ws.b, fbytes * NFIELDS, hipMemcpyDeviceToDevice, stream()));
-    for (int r = 0; r < NFIELDS; ++r) rn[r] = bn[r];
-    std::vector<std::vector<zc>> H(NFIELDS, std::vector<zc>((size_t)(m + 1) * m)), G(NFIELDS, std::vector<zc>(m + 1)),
-        SN(NFIELDS, std::vector<zc>(m)), Y(NFIELDS, std::vector<zc>(m));
-    std::vector<std::vector<double>> CS(NFIELDS, std::vector<double>(m));
-    std::vector<int> jused(NFIELDS, 0);
+    NRQ_HIP(hipMemcpyAsync(g_ws.res, g_ws.b, fbytes * R, hipMemcpyDeviceToDevice, stream()));
+    for (int r = 0; r < R; ++r) rn[r] = bn[r];
+    std::vector<std::vector<zc>> H(R, std::vector<zc>((size_t)(m + 1) * m)), G(R, std::vector<zc>(m + 1)),
+        SN(R, std::vector<zc>(m)), Y(R, std::vector<zc>(m));
+    std::vector<std::vector<double>> CS(R, std::vector<double>(m));
+    std::vector<int> jused(R, 0);
     int64_t total_its = 0, restarts = 0;
     double worst = 0;
     int status = NRQ_OK;
-    std::vector<int> cycle(NFIELDS);
+    std::vector<int> cycle(R);
     while (true) {
         bool any = false;
-        for (int r = 0; r < NFIELDS; ++r) any = any || act[r];
+        for (int r = 0; r < R; ++r) any = any || act[r];
         if (!any) break;
         ++restarts;
         // v0 = res / ||res||
-        for (int r = 0; r < NFIELDS; ++r) {
+        for (int r = 0; r < R; ++r) {
             g_ws.nbuf[r] = act[r] ? rn[r] * rn[r] : 0.0;
             std::fill(G[r].begin(), G[r].end(), zc(0));
             G[r][0] = rn[r];
             jused[r] = 0;
         }
-        NRQ_HIP(hipMemcpyAsync(g_ws.nrm, g_ws.nbuf, sizeof(double) * NFIELDS, hipMemcpyHostToDevice, stream()));
+        NRQ_HIP(hipMemcpyAsync(g_ws.nrm, g_ws.nbuf, sizeof(double) * R, hipMemcpyHostToDevice, stream()));
         if (int e = upload_active(act)) return e;
-        hipLaunchKernelGGL(k_scale, dim3(VBLOCKS, NFIELDS), dim3(VTHREADS), 0, stream(), g_ws.res, g_ws.basis, n, g_ws.nrm, g_ws.active);
+        hipLaunchKernelGGL(k_scale, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.res, g_ws.basis, n, g_ws.nrm, g_ws.active);
         cycle = act;
         for (int j = 0; j < m; ++j) {
             bool busy = false;
-            for (int r = 0; r < NFIELDS; ++r) busy = busy || cycle[r];
+            for (int r = 0; r < R; ++r) busy = busy || cycle[r];
             if (!busy) break;
             if (int e = upload_active(cycle)) return e;
-            cplx<double> *vj = g_ws.basis + (size_t)j * NFIELDS * n;
-            cplx<double> *w = g_ws.basis + (size_t)(j + 1) * NFIELDS * n;
-            if (int e = apply_op(gf, vj, w)) return e;
+            cplx<double> *vj = g_ws.basis + (size_t)j * R * n;
+            cplx<double> *w = g_ws.basis + (size_t)(j + 1) * R * n;
+            if (int e = apply_op(gf, vj, w, R)) return e;
             const int nv = j + 1;
-            hipLaunchKernelGGL(k_mdot, dim3(VBLOCKS, NFIELDS), dim3(VTHREADS), 0, stream(), g_ws.basis, nv, w, n, NFIELDS, g_ws.active, g_ws.cpart);
-            hipLaunchKernelGGL(k_fold_c, dim3((nv * NFIELDS + 63) / 64), dim3(64), 0, stream(), g_ws.cpart, VBLOCKS, nv, NFIELDS, g_ws.active, g_ws.hdev);
-            hipLaunchKernelGGL(k_cgs_update, dim3(VBLOCKS, NFIELDS), dim3(VTHREADS), 0, stream(), g_ws.basis, nv, w, n, NFIELDS, g_ws.active, g_ws.hdev, nv, g_ws.rpart);
-            hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, NFIELDS, g_ws.active, g_ws.nrm);
-            hipLaunchKernelGGL(k_scale, dim3(VBLOCKS, NFIELDS), dim3(VTHREADS), 0, stream(), w, w, n, g_ws.nrm, g_ws.active);
+            hipLaunchKernelGGL(k_mdot, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.basis, nv, w, n, R, g_ws.active, g_ws.cpart);
+            hipLaunchKernelGGL(k_fold_c, dim3((nv * R + 63) / 64), dim3(64), 0, stream(), g_ws.cpart, VBLOCKS, nv, R, g_ws.active, g_ws.hdev);
+            hipLaunchKernelGGL(k_cgs_update, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.basis, nv, w, n, R, g_ws.active, g_ws.hdev, nv, g_ws.rpart);
+            hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm);
+            hipLaunchKernelGGL(k_scale, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), w, w, n, g_ws.nrm, g_ws.active);
             NRQ_HIP(hipGetLastError());
-            NRQ_HIP(hipMemcpyAsync(g_ws.hbuf, g_ws.hdev, sizeof(cplx<double>) * nv * NFIELDS, hipMemcpyDeviceToHost, stream()));
-            NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * NFIELDS, hipMemcpyDeviceToHost, stream()));
+            NRQ_HIP(hipMemcpyAsync(g_ws.hbuf, g_ws.hdev, sizeof(cplx<double>) * nv * R, hipMemcpyDeviceToHost, stream()));
+            NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToHost, stream()));
             NRQ_HIP(hipStreamSynchronize(stream()));
             ++total_its;
-            for (int r = 0; r < NFIELDS; ++r) {
+            for (int r = 0; r < R; ++r) {
                 if (!cycle[r]) continue;
                 ++its[r];
                 zc *Hc = H[r].data() + (size_t)(m + 1) * j;
@@ -543,8 +544,8 @@ static int gmres_batched(const nrq_gauge *gf, std::vector<int> act, const double
             }
         }
         // y = H^-1 g per field, x += V y
-        std::vector<int> nu(NFIELDS, 0);
-        for (int r = 0; r < NFIELDS; ++r) {
+        std::vector<int> nu(R, 0);
+        for (int r = 0; r < R; ++r) {
             if (!act[r]) continue;
             const int jj = jused[r];
             nu[r] = jj;
@@ -556,17 +557,17 @@ static int gmres_batched(const nrq_gauge *gf, std::vector<int> act, const double
             for (int i = 0; i < jj; ++i) g_ws.hbuf[r * (m + 1) + i] = {Y[r][i].real(), Y[r][i].imag()};
         }
         if (int e = upload_active(act)) return e;
-        NRQ_HIP(hipMemcpyAsync(g_ws.ydev, g_ws.hbuf, sizeof(cplx<double>) * NFIELDS * (m + 1), hipMemcpyHostToDevice, stream()));
-        NRQ_HIP(hipMemcpyAsync(g_ws.nused, nu.data(), sizeof(int) * NFIELDS, hipMemcpyHostToDevice, stream()));
-        hipLaunchKernelGGL(k_xupdate, dim3(VBLOCKS, NFIELDS), dim3(VTHREADS), 0, stream(), g_ws.x, g_ws.basis, NFIELDS, n, g_ws.ydev, m + 1, g_ws.nused, g_ws.active);
+        NRQ_HIP(hipMemcpyAsync(g_ws.ydev, g_ws.hbuf, sizeof(cplx<double>) * R * (m + 1), hipMemcpyHostToDevice, stream()));
+        NRQ_HIP(hipMemcpyAsync(g_ws.nused, nu.data(), sizeof(int) * R, hipMemcpyHostToDevice, stream()));
+        hipLaunchKernelGGL(k_xupdate, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.x, g_ws.basis, R, n, g_ws.ydev, m + 1, g_ws.nused, g_ws.active);
         // true residual
-        if (int e = apply_op(gf, g_ws.x, g_ws.t)) return e;
-        hipLaunchKernelGGL(k_residual, dim3(VBLOCKS, NFIELDS), dim3(VTHREADS), 0, stream(), g_ws.b, g_ws.t, g_ws.res, n, g_ws.active, g_ws.rpart);
-        hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, NFIELDS, g_ws.active, g_ws.nrm);
+        if (int e = apply_op(gf, g_ws.x, g_ws.t, R)) return e;
+        hipLaunchKernelGGL(k_residual, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.b, g_ws.t, g_ws.res, n, g_ws.active, g_ws.rpart);
+        hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm);
         NRQ_HIP(hipGetLastError());
-        NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * NFIELDS, hipMemcpyDeviceToHost, stream()));
+        NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToHost, stream()));
         NRQ_HIP(hipStreamSynchronize(stream()));
-        for (int r = 0; r < NFIELDS; ++r) {
+        for (int r = 0; r < R; ++r) {
             if (!act[r]) continue;
             rn[r] = std::sqrt(std::max(g_ws.nbuf[r], 0.0));
             const double rel = rn[r] / bn[r];
@@ -583,7 +584,7 @@ static int gmres_batched(const nrq_gauge *gf, std::vector<int> act, const double
     }
     if (st) {
         int64_t s = 0;
-        for (int r = 0; r < NFIELDS; ++r) s += its[r];
+        for (int r = 0; r < R; ++r) s += its[r];
         st->iterations += s;
         st->restarts += restarts;
         st->max_true_relres = std::max(st->max_true_relres, worst);
@@ -785,35 +786,45 @@ extern "C" int nrq_heavy_update(nrq_heavy *h, const nrq_gauge *gf, double mass, 
     const std::complex<double> alpha_a = mi * std::conj(c1) / (-mass);
     std::vector<cplx<double>> beta(NFIELDS);
     std::vector<int> all(NFIELDS, 1);
-    // rhs = S psi (SystMat_herm before the in-place Hermitian transpose)
+    // rhs = S psi (SystMat_herm before the in-place Hermitian transpose) for
+    // all 24 fields, into the scratch slot array t
     for (int r = 0; r < NFIELDS; ++r) {
         const std::complex<double> a = r < 12 ? alpha_q : alpha_a;
         beta[r] = {a.real(), a.imag()};
     }
     NRQ_HIP(hipMemcpyAsync(g_ws.beta, beta.data(), sizeof(cplx<double>) * NFIELDS, hipMemcpyHostToDevice, stream()));
     if (int e = upload_active(all)) return e;
-    if (int e = apply_op(gf, (const cplx<double> *)h->psi, g_ws.b)) return e;
-    double bn2[NFIELDS];
-    if (int e = field_norms2(g_ws.b, bn2)) return e;
-    std::vector<int> act(NFIELDS, 0);
-    for (int r = 0; r < NFIELDS; ++r) {
-        act[r] = bn2[r] > 0.0;
-        st.solves += act[r];
-    }
-    // operator S^+ = 1 + conj(alpha) K
-    for (int r = 0; r < NFIELDS; ++r) {
-        const std::complex<double> a = std::conj(r < 12 ? alpha_q : alpha_a);
-        beta[r] = {a.real(), a.imag()};
-    }
-    NRQ_HIP(hipMemcpyAsync(g_ws.beta, beta.data(), sizeof(cplx<double>) * NFIELDS, hipMemcpyHostToDevice, stream()));
-    int status = gmres_batched(gf, act, bn2, &st);
-    // columns with a zero right-hand side stay zero; copy the solutions back
-    const size_t fb = (size_t)3 * (size_t)h->vol * sizeof(cplx<double>);
+    if (int e = apply_op(gf, (const cplx<double> *)h->psi, g_ws.t, NFIELDS)) return e;
+    double bn2_all[NFIELDS];
+    if (int e = field_norms2(g_ws.t, bn2_all, NFIELDS)) return e;
+    // compact the non-zero right-hand sides into slots 0..R-1 (a zero
+    // right-hand side has the zero solution, as KSPSolve returns for b = 0)
+    std::vector<int> field;
     for (int r = 0; r < NFIELDS; ++r)
-        if (bn2[r] > 0.0)
-            NRQ_HIP(hipMemcpyAsync((char *)h->psi + fb * r, (char *)g_ws.x + fb * r, fb, hipMemcpyDeviceToDevice, stream()));
-        else
-            NRQ_HIP(hipMemsetAsync((char *)h->psi + fb * r, 0, fb, stream()));
+        if (bn2_all[r] > 0.0) field.push_back(r);
+    const int R = (int)field.size();
+    st.solves = R;
+    const size_t fb = (size_t)3 * (size_t)h->vol * sizeof(cplx<double>);
+    double bn2[NFIELDS];
+    std::vector<cplx<double>> sbeta(R > 0 ? R : 1);
+    for (int q = 0; q < R; ++q) {
+        const int r = field[q];
+        NRQ_HIP(hipMemcpyAsync((char *)g_ws.b + fb * q, (char *)g_ws.t + fb * r, fb, hipMemcpyDeviceToDevice, stream()));
+        bn2[q] = bn2_all[r];
+        // operator S^+ = 1 + conj(alpha) K
+        const std::complex<double> a = std::conj(r < 12 ? alpha_q : alpha_a);
+        sbeta[q] = {a.real(), a.imag()};
+    }
+    int status = NRQ_OK;
+    if (R > 0) {
+        NRQ_HIP(hipMemcpyAsync(g_ws.beta, sbeta.data(), sizeof(cplx<double>) * R, hipMemcpyHostToDevice, stream()));
+        status = gmres_batched(gf, R, bn2, &st);
+    }
+    // scatter the solutions back; fields with a zero right-hand side are zero
+    for (int r = 0; r < NFIELDS; ++r)
+        if (!(bn2_all[r] > 0.0)) NRQ_HIP(hipMemsetAsync((char *)h->psi + fb * r, 0, fb, stream()));
+    for (int q = 0; q < R; ++q)
+        NRQ_HIP(hipMemcpyAsync((char *)h->psi + fb * field[q], (char *)g_ws.x + fb * q, fb, hipMemcpyDeviceToDevice, stream()));
     NRQ_HIP(hipStreamSynchronize(stream()));
     if (stats) *stats = st;
     return status;
@@ -827,7 +838,7 @@ extern "C" int nrq_heavy_norms(const nrq_heavy *h, double *nq, double *na)
     std::vector<int> all(NFIELDS, 1);
     if (int e = upload_active(all)) return e;
     double n2[NFIELDS];
-    if (int e = field_norms2((const cplx<double> *)h->psi, n2)) return e;
+    if (int e = field_norms2((const cplx<double> *)h->psi, n2, NFIELDS)) return e;
     double q = 0, a = 0;
     for (int r = 0; r < 12; ++r) q += n2[r];
     for (int r = 12; r < 24; ++r) a += n2[r];

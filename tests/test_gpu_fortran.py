@@ -1,0 +1,78 @@
+"""The Fortran drop-in (nrqcd_amd/fortran: modules lattice, random,
+gaugeconfiguration_su3, nrqcd and the nrqcd_run driver with the reference's
+positional command line) run end to end on the GPU and checked against the
+oracle."""
+import subprocess
+from pathlib import Path
+
+import numpy as np
+import pytest
+
+pytestmark = pytest.mark.gpu
+
+ROOT = Path(__file__).resolve().parents[1]
+EXE = ROOT / "nrqcd_amd" / "fortran" / "nrqcd_run"
+
+
+def _run(args, cwd):
+    print("nrqcd_run", " ".join(map(str, args)), flush=True)
+    if not EXE.exists():
+        pytest.fail("nrqcd_run not built (python -m nrqcd_amd.build)")
+    r = subprocess.run([str(EXE), *map(str, args)], cwd=cwd, capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stdout[-2000:] + r.stderr[-2000:]
+    return r
+
+
+def test_mode2_energy_and_gauss(oracle, tmp_path):
+    ext, sp, seed = (6, 6, 6), (0.05, 1.0, 1.0, 1.0), 3
+    # mode Nx Ny Nz a0 ax ay az CoMTime TimeRange Seed Qs Amp g
+    _run([2, *ext, *sp, 0.0, 0.5, seed, 0.5, 1.0, 1.0], tmp_path)
+    rows = np.loadtxt(tmp_path / "energy_gauss_constraint_plaquette.txt")
+    short = np.loadtxt(tmp_path / "energy_gauss.txt")
+    assert rows.shape == (11, 5) and short.shape == (11, 3)
+    oracle.lattice_init(ext, sp)
+    U, E = oracle.box_init(seed, 0.5, 1.0, 1.0)
+    for k in range(rows.shape[0]):
+        assert rows[k, 0] == pytest.approx(k * sp[0], abs=1e-15)
+        assert rows[k, 1] == pytest.approx(oracle.energy(U, E), rel=1e-12)
+        assert rows[k, 3] == pytest.approx(oracle.gauss_max(U, E), rel=1e-10)
+        assert rows[k, 4] == pytest.approx(oracle.mean_plaquette(U), rel=1e-12)
+        assert abs(rows[k, 2]) < 1e-8 * rows[k, 1]
+        assert short[k, 1] == pytest.approx(rows[k, 1], rel=1e-5)
+        oracle.leapfrog(U, E, 1.0, 1)
+
+
+def test_mode3_correlator_grid(oracle, tmp_path):
+    ext, seed, ds, smax = (6, 6, 6), 2, 0.1, 0.2
+    a0 = ds / 2
+    mass, tol = 2.0, 1e-12
+    # mode Nx Ny Nz ds ax ay az tmin tmax smax Seed Qs Amp g kspTol M c1..c4 (re, im) files
+    _run([3, *ext, ds, 1.0, 1.0, 1.0, 0.0, 0.0, smax, seed, 0.5, 1.0, 1.0, tol, mass,
+          1.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, 0.0, "corr.txt", "norm.txt"], tmp_path)
+    corr = np.loadtxt(tmp_path / "corr.txt", skiprows=1)
+    norm = np.loadtxt(tmp_path / "norm.txt", skiprows=1)
+    assert corr.shape == (3, 3)
+    assert np.allclose(corr[:, 0], [0.0, ds, 2 * ds])
+    assert corr[0, 1] == 0.0 and corr[0, 2] == 1.0
+    # oracle replay of main.f90:2275-2364 for tmin = tmax = 0, smax = 2 ds:
+    # t1 = -2 a0 ... 0; (s=2ds) from t1 = -2 a0 after 4 CN+leapfrog steps,
+    # (s=ds) from t1 = -a0 after 2 steps
+    sp = (a0, 1.0, 1.0, 1.0)
+    oracle.lattice_init(ext, sp)
+    U, E = oracle.box_init(seed, 0.5, 1.0, 1.0)
+    oracle.leapfrog(U, E, -1.0, 2)
+    want = {}
+    for it1, nsteps, srow in ((-2, 4, 2), (-1, 2, 1)):
+        U2, E2 = U.copy(order="F"), E.copy(order="F")
+        Q, A = oracle.point_prop(1), oracle.point_prop(1)
+        for _ in range(nsteps):
+            rc, _ = oracle.cn_update(U2, Q, A, mass, [1, 0, 0, 0], 1.0, tol)
+            assert rc == 0
+            oracle.leapfrog(U2, E2, 1.0, 1)
+        want[srow] = (oracle.correlators(Q, A)["O1_3S1"], oracle.prop_norm(Q), oracle.prop_norm(A))
+        oracle.leapfrog(U, E, 1.0, 1)
+    for srow, (c, nq, na) in want.items():
+        assert corr[srow, 1] == pytest.approx(c.real, rel=1e-8, abs=1e-9)
+        assert corr[srow, 2] == pytest.approx(c.imag, rel=1e-8, abs=1e-9)
+        assert norm[srow, 1] == pytest.approx(nq, rel=1e-8)
+        assert norm[srow, 2] == pytest.approx(na, rel=1e-8)
