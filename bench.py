@@ -29,14 +29,18 @@ METRIC = "link-updates/s & CN steps/s at FP64 rel.res 1e-10: learned+mixed vs pl
 HBM_PEAK_GBS = 8000.0
 
 
-def _cpu_baseline(size: int, steps: int) -> dict:
-    """Oracle leapfrog (OpenMP FP64 C restatement) on a bounded sample."""
+def _cpu_baseline(size: int, seconds: float) -> dict:
+    """Oracle leapfrog (OpenMP FP64 C restatement) on a bounded sample of the
+    same workload: the step count is chosen to fill about `seconds`."""
     from oracle.pyoracle import Oracle
 
     orc = Oracle()
     orc.lattice_init((size, size, size), (0.05, 1.0, 1.0, 1.0))
     U, E = orc.box_init(2, 0.5, 1.0, 1.0)
-    orc.leapfrog(U, E, 1.0, 1)  # warm-up
+    t0 = time.perf_counter()
+    orc.leapfrog(U, E, 1.0, 1)  # warm-up and calibration
+    one = time.perf_counter() - t0
+    steps = max(1, int(seconds / max(one, 1e-6)))
     t0 = time.perf_counter()
     orc.leapfrog(U, E, 1.0, steps)
     dt = time.perf_counter() - t0
@@ -57,7 +61,7 @@ def main() -> None:
     ap.add_argument("--size", type=int, default=64)
     ap.add_argument("--cn-steps", type=int, default=20)
     ap.add_argument("--no-cpu", action="store_true")
-    ap.add_argument("--cpu-steps", type=int, default=4)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the lattice is not "
@@ -84,6 +88,7 @@ def main() -> None:
     ms = nrq.event_elapsed_ms(0, 1)
     wall = time.perf_counter() - t0
     obs1 = g.observables()
+    shape = g.launch_shape()
     lu_per_s = 3 * V * args.steps / (ms * 1e-3)
     ms_step = ms / args.steps
     # dominant kernel: the fused E+link pass (steps-1 launches, plus one
@@ -93,27 +98,32 @@ def main() -> None:
 
     # ---- CN ----------------------------------------------------------------
     nrq.nrqcd_init(2.0, [1, 0, 0, 0], 1.0, 1e-10)
-    nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_GMRES_FP64, 30, 10000)
-    h = nrq.NRQCDField()
-    h.InitSinglePoint(1, 1)
-    h.Update(g, 2.0, [1, 0, 0, 0])  # warm-up (workspace allocation)
-    nrq.synchronize()
-    its = 0
-    worst = 0.0
-    nrq.event_record(2)
-    for _ in range(args.cn_steps):
-        st = h.Update(g, 2.0, [1, 0, 0, 0])
-        its += st.iterations
-        worst = max(worst, st.max_true_relres)
-    nrq.event_record(3)
-    cn_ms = nrq.event_elapsed_ms(2, 3)
-    cn_per_s = args.cn_steps / (cn_ms * 1e-3)
-    nq, na = h.norms()
+    cn = {}
+    for name, kind in (("plain_fp64", nrq.NRQ_SOLVER_GMRES_FP64), ("mixed", nrq.NRQ_SOLVER_GMRES_MIXED)):
+        nrq.nrqcd_set_solver(kind, 30, 10000)
+        h = nrq.NRQCDField()
+        h.InitSinglePoint(1, 1)
+        h.Update(g, 2.0, [1, 0, 0, 0])  # warm-up (workspace allocation)
+        nrq.synchronize()
+        its = 0
+        worst = 0.0
+        nrq.event_record(2)
+        for _ in range(args.cn_steps):
+            st = h.Update(g, 2.0, [1, 0, 0, 0])
+            its += st.iterations
+            worst = max(worst, st.max_true_relres)
+        nrq.event_record(3)
+        cn_ms = nrq.event_elapsed_ms(2, 3)
+        nq, na = h.norms()
+        cn[name] = {"cn_steps_per_s": args.cn_steps / (cn_ms * 1e-3), "ms_per_step": cn_ms / args.cn_steps,
+                    "krylov_iterations_per_step": its / args.cn_steps, "worst_true_relres": worst, "norms": [nq, na]}
+        del h
     nrq.nrqcd_finalize()
+    copy_gbs = nrq.bandwidth_probe(1 << 30, 20)
 
     cpu = None
     if not args.no_cpu:
-        cpu = _cpu_baseline(min(L, 32), args.cpu_steps)
+        cpu = _cpu_baseline(L, args.cpu_seconds)
 
     out = {
         "metric": METRIC,
@@ -135,14 +145,13 @@ def main() -> None:
             "cn_solver": "GMRES(30) FP64, true rel.res 1e-10",
             "mass": 2.0,
         },
-        "cn_steps_per_s": cn_per_s,
-        "cn_ms_per_step": cn_ms / args.cn_steps,
-        "cn_krylov_iterations_per_step": its / args.cn_steps,
-        "cn_worst_true_relres": worst,
-        "cn_norms": [nq, na],
+        "cn_steps_per_s": cn["plain_fp64"]["cn_steps_per_s"],
+        "cn": cn,
+        "cn_speedup_mixed_vs_plain": cn["mixed"]["cn_steps_per_s"] / cn["plain_fp64"]["cn_steps_per_s"],
         "energy_drift_rel": (obs1["energy"] - obs0["energy"]) / obs0["energy"],
         "gauss_max_drift": obs1["gauss_max"] - obs0["gauss_max"],
         "wall_s_leapfrog": wall,
+        "hbm_copy_probe_gbs": copy_gbs,
         "roofline": {
             "bound": "hbm",
             "achieved": achieved,
@@ -151,6 +160,7 @@ def main() -> None:
             "frac": achieved / HBM_PEAK_GBS,
             "traffic": None,
             "kernel": "k_fused<double> (E update + next link update per (site, dir))",
+            "launch_shape": shape,
             "bytes_per_launch": bytes_per_site * V,
         },
         "cpu_baseline": cpu,

@@ -57,6 +57,11 @@ const char *nrq_version(void);
 int nrq_event_record(int slot);
 int nrq_event_elapsed(int start, int stop, double *ms);
 int nrq_synchronize(void);
+/* Runs a plain streaming-copy kernel (read + write `bytes`) `reps` times and
+ * returns the achieved read+write bandwidth in GB/s: the practical HBM
+ * ceiling of this device and a known-size reference for hardware byte
+ * counters.  No reference counterpart. */
+int nrq_bandwidth_probe(int64_t bytes, int reps, double *gbs);
 
 /* ---- lattice (lattice.f90) ----------------------------------------- */
 
@@ -145,6 +150,13 @@ int nrq_gauge_plaquette(const nrq_gauge *h, int i, int j, int64_t idx, double P[
  * row 3 = conj(row1 x row2)).  No reference counterpart: the reference keeps
  * FP64 links and never reunitarises; used by the FP32 storage mode. */
 int nrq_gauge_reunitarise(nrq_gauge *h);
+/* Launch shape the leapfrog autotuner chose for this configuration's
+ * precision and lattice size: threads per block, XCD remap factor (1 = none)
+ * and tiling (0: consecutive sites, 1: z-sweep tiles of full x-rows).  The
+ * first leapfrog call per (precision, size) times every shape on scratch
+ * copies; NRQ_AUTOTUNE=0 in the environment disables it.  All shapes compute
+ * identical results.  No reference counterpart. */
+int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling);
 
 /* ---- nrqcd ------------------------------------------------------------ */
 
@@ -157,7 +169,10 @@ typedef struct {
     double max_true_relres;  /* worst FP64 true relative residual ||b-Ax||/||b|| */
 } nrq_solve_stats;
 
-#define NRQ_SOLVER_GMRES_FP64 0   /* GMRES(m), classical Gram-Schmidt, FP64 */
+#define NRQ_SOLVER_GMRES_FP64 0   /* GMRES(m), classical Gram-Schmidt, FP64, zero guess */
+#define NRQ_SOLVER_GMRES_MIXED 1  /* FP32 GMRES(m) corrections inside FP64 iterative
+                                     refinement on the true residual, first-order
+                                     initial guess */
 
 /* nrqcd::InitModule(Mass, WilsonCoeffs(4), StepWidth, kspTol),
  * nrqcd.f90:167-192 with KSPSetTolerances(rtol=kspTol), nrqcd.f90:803-804.

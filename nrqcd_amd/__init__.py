@@ -34,6 +34,7 @@ NRQ_OK = 0
 NRQ_PRECISION_FP64 = 64
 NRQ_PRECISION_FP32 = 32
 NRQ_SOLVER_GMRES_FP64 = 0
+NRQ_SOLVER_GMRES_MIXED = 1
 
 _ERRORS = {
     -1: "NRQ_ERR_ARG",
@@ -90,6 +91,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_event_record": (I, [I]),
         "nrq_event_elapsed": (I, [I, I, P]),
         "nrq_synchronize": (I, []),
+        "nrq_bandwidth_probe": (I, [I64, I, P]),
         "nrq_lattice_init": (I, [P, P]),
         "nrq_lattice_finalize": (I, []),
         "nrq_lattice_size": (I, [P]),
@@ -118,6 +120,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_gauge_observables": (I, [P, P]),
         "nrq_gauge_plaquette": (I, [P, I, I, I64, P]),
         "nrq_gauge_reunitarise": (I, [P]),
+        "nrq_gauge_launch_shape": (I, [P, P, P, P]),
         "nrq_cn_init": (I, [D, P, D, D]),
         "nrq_cn_finalize": (I, []),
         "nrq_cn_set_solver": (I, [I, I, I64]),
@@ -162,6 +165,13 @@ def event_elapsed_ms(start: int, stop: int) -> float:
 
 def synchronize() -> None:
     _check(library().nrq_synchronize())
+
+
+def bandwidth_probe(nbytes: int = 1 << 30, reps: int = 20) -> float:
+    """Read+write GB/s of a plain streaming-copy kernel over ``nbytes``."""
+    out = ctypes.c_double()
+    _check(library().nrq_bandwidth_probe(nbytes, reps, ctypes.byref(out)))
+    return out.value
 
 
 # ---------------------------------------------------------------------------
@@ -307,6 +317,13 @@ class GaugeConfiguration:
 
     def Reunitarise(self) -> None:
         _check(library().nrq_gauge_reunitarise(self._h))
+
+    def launch_shape(self) -> dict:
+        """Launch shape chosen by the leapfrog autotuner."""
+        bs, xcd, tiling = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
+        _check(library().nrq_gauge_launch_shape(self._h, ctypes.byref(bs), ctypes.byref(xcd), ctypes.byref(tiling)))
+        return {"threads_per_block": bs.value, "xcd_remap": xcd.value,
+                "tiling": "z-sweep tiles" if tiling.value else "consecutive sites"}
 
     # -- observables -----------------------------------------------------
     def observables(self) -> dict:

@@ -348,3 +348,53 @@ extern "C" int nrq_synchronize(void)
     NRQ_HIP(hipStreamSynchronize(stream()));
     return NRQ_OK;
 }
+
+// ---------------------------------------------------------------------------
+// Streaming copy probe: the achievable HBM bandwidth of a plain read+write
+// kernel on this device, and a known-size reference for hardware counters.
+// ---------------------------------------------------------------------------
+typedef double v2d __attribute__((ext_vector_type(2)));
+
+__global__ __launch_bounds__(256) void k_stream_copy(const v2d *__restrict__ src, v2d *__restrict__ dst, int64_t n)
+{
+    // each thread keeps four 16-byte loads in flight
+    const int64_t base = (int64_t)blockIdx.x * 1024 + threadIdx.x;
+    v2d v[4];
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n) v[k] = __builtin_nontemporal_load(src + base + 256 * k);
+#pragma unroll
+    for (int k = 0; k < 4; ++k)
+        if (base + 256 * k < n) __builtin_nontemporal_store(v[k], dst + base + 256 * k);
+}
+
+extern "C" int nrq_bandwidth_probe(int64_t bytes, int reps, double *gbs)
+{
+    if (bytes < 4096 || reps < 1 || !gbs) return fail(NRQ_ERR_ARG, "bandwidth probe: bad arguments");
+    if (int r = ensure_device()) return r;
+    const int64_t n = bytes / (int64_t)sizeof(double2);
+    double2 *a = nullptr, *b = nullptr;
+    if (hipMalloc(&a, n * sizeof(double2)) != hipSuccess || hipMalloc(&b, n * sizeof(double2)) != hipSuccess) {
+        if (a) (void)hipFree(a);
+        return fail(NRQ_ERR_ALLOC, "bandwidth probe: allocation of %lld bytes failed", (long long)bytes);
+    }
+    NRQ_HIP(hipMemsetAsync(a, 0, n * sizeof(double2), stream()));
+    hipEvent_t e0, e1;
+    NRQ_HIP(hipEventCreate(&e0));
+    NRQ_HIP(hipEventCreate(&e1));
+    const unsigned blocks = (unsigned)((n + 1023) / 1024);
+    hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, stream(), (const v2d *)a, (v2d *)b, n); // warm-up
+    NRQ_HIP(hipEventRecord(e0, stream()));
+    for (int r = 0; r < reps; ++r)
+        hipLaunchKernelGGL(k_stream_copy, dim3(blocks), dim3(256), 0, stream(), (const v2d *)a, (v2d *)b, n);
+    NRQ_HIP(hipEventRecord(e1, stream()));
+    NRQ_HIP(hipEventSynchronize(e1));
+    float ms = 0;
+    NRQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(a);
+    (void)hipFree(b);
+    *gbs = 2.0 * (double)(n * sizeof(double2)) * reps / (ms * 1e-3) / 1e9;
+    return NRQ_OK;
+}

@@ -9,7 +9,10 @@
 // the fused step can read the old links of a site's neighbours while writing
 // new links.
 #include "nrq_internal.h"
+#include "nrq_map.h"
 #include <cmath>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 #include <vector>
 
@@ -77,13 +80,14 @@ __device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, con
     for (int a = 0; a < 8; ++a) e[a] -= t[a];
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_links(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
-                                               const R *__restrict__ ef, int64_t vol, R dt)
+template <typename R, int BS>
+__global__ __launch_bounds__(BS) void k_links(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
+                                              const R *__restrict__ ef, geom g, R dt, lf_map m)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    if (s >= vol) return;
+    int64_t s;
+    int i;
+    if (!lf_site(m, g, s, i)) return;
+    const int64_t vol = g.vol;
     R e[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * vol + s];
@@ -91,12 +95,13 @@ __global__ __launch_bounds__(256) void k_links(const cplx<R> *__restrict__ uin, 
     store_link(uout, vol, i, s, evolve_link(u, e, dt));
 }
 
-template <typename R>
-__global__ __launch_bounds__(256) void k_efield(const cplx<R> *__restrict__ u, R *__restrict__ ef, geom g, R dt)
+template <typename R, int BS>
+__global__ __launch_bounds__(BS) void k_efield(const cplx<R> *__restrict__ u, R *__restrict__ ef, geom g, R dt,
+                                               lf_map m)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    if (s >= g.vol) return;
+    int64_t s;
+    int i;
+    if (!lf_site(m, g, s, i)) return;
     R e[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * g.vol + s];
@@ -111,13 +116,13 @@ __global__ __launch_bounds__(256) void k_efield(const cplx<R> *__restrict__ u, R
 // per-(site, dir) operations of the reference; fusing them only removes one
 // HBM round trip of E and U.  Neighbour links are read from uin, new links
 // go to uout (ping-pong), E is updated in place (each thread owns its E).
-template <typename R>
-__global__ __launch_bounds__(256) void k_fused(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
-                                               R *__restrict__ ef, geom g, R dt)
+template <typename R, int BS>
+__global__ __launch_bounds__(BS) void k_fused(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
+                                              R *__restrict__ ef, geom g, R dt, lf_map m)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    const int i = blockIdx.y;
-    if (s >= g.vol) return;
+    int64_t s;
+    int i;
+    if (!lf_site(m, g, s, i)) return;
     R e[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * g.vol + s];
@@ -381,28 +386,174 @@ int gauge_obs(const nrq_gauge *h, double out[NOBS])
     return NRQ_OK;
 }
 
+// Launch shapes the autotuner chooses from (block size, XCD remap).  All
+// variants perform identical arithmetic per (site, dir), so the choice never
+// changes results, only speed.
+struct lf_variant {
+    int bs;
+    int xcd;
+    int mode;
+};
+static const lf_variant LF_VARIANTS[] = {{256, 8, 1}, {128, 8, 1}, {64, 8, 1}, {256, 8, 0},
+                                         {128, 8, 0}, {64, 8, 0},  {256, 1, 0}};
+constexpr int LF_NVARIANTS = sizeof(LF_VARIANTS) / sizeof(LF_VARIANTS[0]);
+
+struct lf_choice {
+    int precision;
+    int64_t vol;
+    int variant;
+};
+static std::vector<lf_choice> g_lf_choice;
+static int g_autotune = -1; // -1: unset (env NRQ_AUTOTUNE, default on)
+
+static bool variant_ok(int variant, const geom &g)
+{
+    const lf_variant v = LF_VARIANTS[variant];
+    return v.mode == 0 || g.n0 <= v.bs;
+}
+
+static lf_map make_map(int variant, const geom &g)
+{
+    const lf_variant v = LF_VARIANTS[variant];
+    return make_lf_map(g, v.bs, v.xcd, v.mode, 3);
+}
+
+template <typename R>
+static void launch_links(int variant, const cplx<R> *uin, cplx<R> *uout, const R *ef, const geom &g, R dt)
+{
+    const lf_map m = make_map(variant, g);
+    const dim3 grd((unsigned)(m.ngroup * m.nsb));
+    switch (m.bs) {
+    case 64: hipLaunchKernelGGL((k_links<R, 64>), grd, dim3(64), 0, stream(), uin, uout, ef, g, dt, m); break;
+    case 128: hipLaunchKernelGGL((k_links<R, 128>), grd, dim3(128), 0, stream(), uin, uout, ef, g, dt, m); break;
+    default: hipLaunchKernelGGL((k_links<R, 256>), grd, dim3(256), 0, stream(), uin, uout, ef, g, dt, m); break;
+    }
+}
+
+template <typename R>
+static void launch_efield(int variant, const cplx<R> *u, R *ef, const geom &g, R dt)
+{
+    const lf_map m = make_map(variant, g);
+    const dim3 grd((unsigned)(m.ngroup * m.nsb));
+    switch (m.bs) {
+    case 64: hipLaunchKernelGGL((k_efield<R, 64>), grd, dim3(64), 0, stream(), u, ef, g, dt, m); break;
+    case 128: hipLaunchKernelGGL((k_efield<R, 128>), grd, dim3(128), 0, stream(), u, ef, g, dt, m); break;
+    default: hipLaunchKernelGGL((k_efield<R, 256>), grd, dim3(256), 0, stream(), u, ef, g, dt, m); break;
+    }
+}
+
+template <typename R>
+static void launch_fused(int variant, const cplx<R> *uin, cplx<R> *uout, R *ef, const geom &g, R dt)
+{
+    const lf_map m = make_map(variant, g);
+    const dim3 grd((unsigned)(m.ngroup * m.nsb));
+    switch (m.bs) {
+    case 64: hipLaunchKernelGGL((k_fused<R, 64>), grd, dim3(64), 0, stream(), uin, uout, ef, g, dt, m); break;
+    case 128: hipLaunchKernelGGL((k_fused<R, 128>), grd, dim3(128), 0, stream(), uin, uout, ef, g, dt, m); break;
+    default: hipLaunchKernelGGL((k_fused<R, 256>), grd, dim3(256), 0, stream(), uin, uout, ef, g, dt, m); break;
+    }
+}
+
+// Times every launch shape on scratch copies of the configuration (the
+// handle's own state is not touched) and returns the fastest.
+template <typename R> static int tune_leapfrog(const nrq_gauge *h, R dt, int *best)
+{
+    const geom g = lat().g;
+    const size_t lb = (size_t)3 * 9 * (size_t)g.vol * sizeof(cplx<R>), eb = (size_t)3 * 8 * (size_t)g.vol * sizeof(R);
+    cplx<R> *u0 = nullptr, *u1 = nullptr;
+    R *ef = nullptr;
+    *best = 0;
+    if (hipMalloc(&u0, lb) != hipSuccess || hipMalloc(&u1, lb) != hipSuccess || hipMalloc(&ef, eb) != hipSuccess) {
+        if (u0) (void)hipFree(u0);
+        if (u1) (void)hipFree(u1);
+        (void)hipGetLastError();
+        return NRQ_OK; // no room to tune: keep the default shape
+    }
+    NRQ_HIP(hipMemcpyAsync(u0, h->u[h->cur], lb, hipMemcpyDeviceToDevice, stream()));
+    NRQ_HIP(hipMemcpyAsync(ef, h->e, eb, hipMemcpyDeviceToDevice, stream()));
+    hipEvent_t e0, e1;
+    NRQ_HIP(hipEventCreate(&e0));
+    NRQ_HIP(hipEventCreate(&e1));
+    float best_ms = 1e30f;
+    const int reps = g.vol >= (1 << 18) ? 6 : 20;
+    *best = variant_ok(0, g) ? 0 : 3;
+    for (int v = 0; v < LF_NVARIANTS; ++v) {
+        if (!variant_ok(v, g)) continue;
+        launch_fused<R>(v, u0, u1, ef, g, dt); // warm-up
+        NRQ_HIP(hipEventRecord(e0, stream()));
+        for (int k = 0; k < reps; ++k) launch_fused<R>(v, (k & 1) ? u1 : u0, (k & 1) ? u0 : u1, ef, g, dt);
+        NRQ_HIP(hipEventRecord(e1, stream()));
+        NRQ_HIP(hipEventSynchronize(e1));
+        float ms = 0;
+        NRQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+        if (getenv("NRQ_AUTOTUNE_VERBOSE"))
+            fprintf(stderr, "nrqcd_amd autotune: precision %d V=%lld bs=%d xcd=%d tiling=%d: %.2f us/step\n", (int)(8 * sizeof(R)),
+                    (long long)g.vol, LF_VARIANTS[v].bs, LF_VARIANTS[v].xcd, LF_VARIANTS[v].mode, 1e3 * ms / reps);
+        if (ms < best_ms) {
+            best_ms = ms;
+            *best = v;
+        }
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    (void)hipFree(u0);
+    (void)hipFree(u1);
+    (void)hipFree(ef);
+    return NRQ_OK;
+}
+
+template <typename R> static int leapfrog_variant(const nrq_gauge *h, R dt, int *variant)
+{
+    if (g_autotune < 0) {
+        const char *env = getenv("NRQ_AUTOTUNE");
+        g_autotune = (env && env[0] == '0') ? 0 : 1;
+    }
+    *variant = variant_ok(0, lat().g) ? 0 : 3;
+    if (!g_autotune) return NRQ_OK;
+    for (const lf_choice &c : g_lf_choice)
+        if (c.precision == h->precision && c.vol == h->vol) {
+            *variant = c.variant;
+            return NRQ_OK;
+        }
+    if (int r = tune_leapfrog<R>(h, dt, variant)) return r;
+    g_lf_choice.push_back({h->precision, h->vol, *variant});
+    return NRQ_OK;
+}
+
 template <typename R> static int leapfrog_t(nrq_gauge *h, double sw, int64_t n)
 {
     const geom g = lat().g;
-    const dim3 blk(256), grd((unsigned)((g.vol + 255) / 256), 3);
+    int v = 0;
+    if (int r = leapfrog_variant<R>(h, (R)sw, &v)) return r;
     cplx<R> *u0 = (cplx<R> *)h->u[0], *u1 = (cplx<R> *)h->u[1];
     R *ef = (R *)h->e;
     const R dt = (R)sw;
     auto cur = [&](int c) { return c ? u1 : u0; };
     if (sw > 0) {
-        hipLaunchKernelGGL(k_links<R>, grd, blk, 0, stream(), cur(h->cur), cur(h->cur), (const R *)ef, g.vol, dt);
+        launch_links<R>(v, cur(h->cur), cur(h->cur), (const R *)ef, g, dt);
         for (int64_t k = 1; k < n; ++k) {
-            hipLaunchKernelGGL(k_fused<R>, grd, blk, 0, stream(), cur(h->cur), cur(h->cur ^ 1), ef, g, dt);
+            launch_fused<R>(v, cur(h->cur), cur(h->cur ^ 1), ef, g, dt);
             h->cur ^= 1;
         }
-        hipLaunchKernelGGL(k_efield<R>, grd, blk, 0, stream(), (const cplx<R> *)cur(h->cur), ef, g, dt);
+        launch_efield<R>(v, (const cplx<R> *)cur(h->cur), ef, g, dt);
     } else {
         for (int64_t k = 0; k < n; ++k) {
-            hipLaunchKernelGGL(k_fused<R>, grd, blk, 0, stream(), cur(h->cur), cur(h->cur ^ 1), ef, g, dt);
+            launch_fused<R>(v, cur(h->cur), cur(h->cur ^ 1), ef, g, dt);
             h->cur ^= 1;
         }
     }
     NRQ_HIP(hipGetLastError());
+    return NRQ_OK;
+}
+
+int leapfrog_choice(const nrq_gauge *h, int *bs, int *xcd, int *mode)
+{
+    int v = 0;
+    if (int r = h->precision == 64 ? leapfrog_variant<double>(h, 1.0, &v) : leapfrog_variant<float>(h, 1.0f, &v))
+        return r;
+    *bs = LF_VARIANTS[v].bs;
+    *xcd = LF_VARIANTS[v].xcd;
+    *mode = LF_VARIANTS[v].mode;
     return NRQ_OK;
 }
 
@@ -690,4 +841,11 @@ namespace nrq {
 const void *gauge_links(const nrq_gauge *h) { return h->u[h->cur]; }
 void *gauge_links_mut(nrq_gauge *h) { return h->u[h->cur]; }
 void *gauge_efield(nrq_gauge *h) { return h->e; }
+}
+
+extern "C" int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling)
+{
+    NEED_GAUGE(h);
+    if (!block_sites || !xcd_remap || !tiling) return fail(NRQ_ERR_ARG, "null argument");
+    return leapfrog_choice(h, block_sites, xcd_remap, tiling);
 }

@@ -1,0 +1,67 @@
+// nrq_map.h -- block-to-work mapping shared by the lattice kernels.
+//
+// A launch is a 1-D grid of ngroup * nsb blocks: nsb site blocks times
+// `ngroup` work items per site block (the three link directions of the
+// leapfrog, the field chunks of the CN stencil), the items of one site block
+// adjacent.  Blocks are dealt round-robin to the XCDs, each with its own L2;
+// with xcd > 1 the hardware block id is remapped so that each XCD walks one
+// contiguous range of site blocks, and the neighbour data a block needs was
+// fetched by the same XCD shortly before.
+//
+// Site blocks are either `bs` consecutive sites (mode 0) or z-sweep tiles
+// (mode 1): tile t = yband * n2 + z holds ty full x-rows at one z, so the
+// z +- 1 neighbours of a tile are the tiles just before and after it.
+#pragma once
+#include "nrq_su3.h"
+
+namespace nrq {
+
+struct lf_map {
+    int64_t nsb; // site blocks (mode 0) or tiles (mode 1)
+    int bs;      // threads per block
+    int xcd;     // XCD remap factor (1: none)
+    int mode;    // 0: bs consecutive sites; 1: z-sweep tiles
+    int ty;      // mode 1: y rows per tile
+    int ngroup;  // work items per site block
+};
+
+__device__ __forceinline__ int64_t lf_logical_block(const lf_map &m)
+{
+    const int64_t b = blockIdx.x;
+    if (m.xcd <= 1) return b;
+    const int64_t total = (int64_t)m.ngroup * m.nsb;
+    const int64_t q = total / m.xcd, r = total % m.xcd;
+    const int64_t x = b % m.xcd, k = b / m.xcd;
+    return x * q + (x < r ? x : r) + k;
+}
+
+// Site of this thread and the work item (direction / chunk) of its block.
+__device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, int64_t &s, int &item)
+{
+    const int64_t lb = lf_logical_block(m);
+    item = (int)(lb % m.ngroup);
+    const int64_t t = lb / m.ngroup;
+    if (m.mode == 0) {
+        s = t * m.bs + threadIdx.x;
+        return s < g.vol;
+    }
+    const int64_t z = t % g.n2, yb = t / g.n2;
+    const int x = (int)threadIdx.x % g.n0, yy = (int)threadIdx.x / g.n0;
+    const int64_t y = yb * m.ty + yy;
+    if (yy >= m.ty || y >= g.n1) return false;
+    s = x + (int64_t)g.n0 * (y + (int64_t)g.n1 * z);
+    return true;
+}
+
+// Host: map for `bs` threads per block; tile mode needs n0 <= bs.
+inline lf_map make_lf_map(const geom &g, int bs, int xcd, int mode, int ngroup)
+{
+    if (mode == 1 && g.n0 <= bs) {
+        const int ty = bs / g.n0 < g.n1 ? bs / g.n0 : g.n1;
+        const int64_t nyb = (g.n1 + ty - 1) / ty;
+        return lf_map{nyb * g.n2, bs, xcd, 1, ty, ngroup};
+    }
+    return lf_map{(g.vol + bs - 1) / bs, bs, xcd, 0, 0, ngroup};
+}
+
+} // namespace nrq

@@ -131,3 +131,54 @@ def test_init_single_dof_matches_reference_indexing(nrq, lattice):
     ia = 2 + 3 * (1 - 1)
     assert Q[iq - 1, iq - 1, 5 - 1] == 1.0 and np.count_nonzero(Q) == 1
     assert A[ia - 1, ia - 1, 7 - 1] == 1.0 and np.count_nonzero(A) == 1
+
+
+@pytest.mark.parametrize("tol", [1e-12, 1e-10])
+def test_mixed_precision_solver_matches_oracle(nrq, oracle, cn, tol):
+    """FP32 corrections inside FP64 iterative refinement: the FP64 true
+    residual is the judge, so the result agrees with the FP64 oracle to the
+    same bar as the all-FP64 solver."""
+    ext = (8, 6, 4)
+    sp = (0.05, 1.0, 0.9, 1.1)
+    _say(f"CN mixed precision tol {tol}")
+    cn(ext, sp, tol)
+    nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_GMRES_MIXED, 30, 10000)
+    oracle.lattice_init(ext, sp)
+    U, E = oracle.box_init(3, 0.5, 1.0, 1.0)
+    g = nrq.GaugeConfiguration()
+    g.upload(U, E)
+    h = nrq.NRQCDField()
+    h.InitSinglePoint(1, 7)
+    Q, A = oracle.point_prop(1), oracle.point_prop(7)
+    for _ in range(3):
+        st = h.Update(g, 2.0, C1)
+        assert st.max_true_relres <= tol
+        assert st.solves == 12
+        oracle.cn_update(U, Q, A, 2.0, C1, 1.0, 1e-13)
+    Qg, Ag = h.download()
+    bar = 200 * tol
+    assert np.max(np.abs(Qg - Q)) < bar * np.max(np.abs(Q))
+    assert np.max(np.abs(Ag - A)) < bar * np.max(np.abs(A))
+
+
+def test_mixed_solver_on_fp32_gauge_storage(nrq, oracle, cn):
+    """FP32 link storage: the FP64 outer residual uses the stored links, so
+    the solve is exact for that operator; compare with the oracle on the
+    FP32-rounded links."""
+    ext = (6, 6, 6)
+    _say("CN mixed on FP32 links")
+    cn(ext, SP, 1e-11)
+    nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_GMRES_MIXED, 30, 10000)
+    oracle.lattice_init(ext, SP)
+    U, E = oracle.box_init(4, 0.5, 1.0, 1.0)
+    U32 = np.asfortranarray(U.astype(np.complex64).astype(np.complex128))
+    g = nrq.GaugeConfiguration(nrq.NRQ_PRECISION_FP32)
+    g.upload(U, E)
+    h = nrq.NRQCDField()
+    h.InitSinglePoint(3, 3)
+    st = h.Update(g, 2.0, C1)
+    assert st.max_true_relres <= 1e-11
+    Q, A = oracle.point_prop(3), oracle.point_prop(3)
+    oracle.cn_update(U32, Q, A, 2.0, C1, 1.0, 1e-13)
+    Qg, Ag = h.download()
+    assert np.max(np.abs(Qg - Q)) < 5e-9 * np.max(np.abs(Q))
