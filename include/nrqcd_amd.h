@@ -173,6 +173,11 @@ typedef struct {
 #define NRQ_SOLVER_GMRES_MIXED 1  /* FP32 GMRES(m) corrections inside FP64 iterative
                                      refinement on the true residual, first-order
                                      initial guess */
+#define NRQ_SOLVER_CHEBYSHEV_FP64 2  /* Chebyshev iteration on the known spectral
+                                        segment of 1 + i dt/2 H (no inner products),
+                                        FP64, first-order guess, GMRES fallback */
+#define NRQ_SOLVER_CHEBYSHEV_MIXED 3 /* FP32 Chebyshev corrections inside FP64
+                                        iterative refinement */
 
 /* nrqcd::InitModule(Mass, WilsonCoeffs(4), StepWidth, kspTol),
  * nrqcd.f90:167-192 with KSPSetTolerances(rtol=kspTol), nrqcd.f90:803-804.

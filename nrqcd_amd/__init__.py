@@ -35,6 +35,8 @@ NRQ_PRECISION_FP64 = 64
 NRQ_PRECISION_FP32 = 32
 NRQ_SOLVER_GMRES_FP64 = 0
 NRQ_SOLVER_GMRES_MIXED = 1
+NRQ_SOLVER_CHEBYSHEV_FP64 = 2
+NRQ_SOLVER_CHEBYSHEV_MIXED = 3
 
 _ERRORS = {
     -1: "NRQ_ERR_ARG",

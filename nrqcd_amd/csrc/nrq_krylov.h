@@ -17,6 +17,8 @@ constexpr int NSLOTS = 24; // colour-vector fields per NRQCDField (and max batch
 enum krylov_kind {
     KRYLOV_GMRES_FP64 = 0,  // GMRES(m), classical Gram-Schmidt, FP64, zero guess
     KRYLOV_GMRES_MIXED = 1, // FP32 GMRES(m) corrections inside FP64 iterative refinement
+    KRYLOV_CHEB_FP64 = 2,   // Chebyshev iteration on the known spectral segment, FP64
+    KRYLOV_CHEB_MIXED = 3,  // FP32 Chebyshev corrections inside FP64 iterative refinement
 };
 
 struct krylov_params {

@@ -25,6 +25,7 @@ namespace nrq {
 const void *gauge_links(const nrq_gauge *h);
 
 constexpr int RCHUNK = 4;     // slots per thread in the stencil (link reuse)
+constexpr int CHEB_MAX = 64;  // Chebyshev steps per coefficient table
 constexpr int VBLOCKS = 256;  // blocks per slot in the vector kernels
 constexpr int VTHREADS = 256;
 
@@ -35,10 +36,13 @@ constexpr int VTHREADS = 256;
 // out_r = in_r + beta_r K in_r for the active slots of one chunk; links of
 // type RL are converted to the compute type T on load and reused across the
 // RCHUNK slots of the chunk.
-template <typename RL, typename T>
+// RESID: out_r = b_r - (in_r + beta_r K in_r) instead.  map (may be null)
+// gives the input field of slot r (in + map[r] * 3V).
+template <typename RL, typename T, bool RESID>
 __global__ __launch_bounds__(256) void k_stencil(const cplx<RL> *__restrict__ u, const cplx<T> *__restrict__ in,
                                                  cplx<T> *__restrict__ out, const cplx<double> *__restrict__ beta,
-                                                 const int *__restrict__ active, int nf, geom g, lf_map mp)
+                                                 const int *__restrict__ active, int nf, geom g, lf_map mp,
+                                                 const int *__restrict__ map, const cplx<T> *__restrict__ b)
 {
     int64_t s;
     int chunk;
@@ -67,7 +71,7 @@ __global__ __launch_bounds__(256) void k_stencil(const cplx<RL> *__restrict__ u,
 #pragma unroll
         for (int q = 0; q < RCHUNK; ++q) {
             if (!on[q]) continue;
-            const cplx<T> *f = in + (int64_t)(r0 + q) * fs;
+            const cplx<T> *f = in + (int64_t)(map ? map[r0 + q] : r0 + q) * fs;
             cplx<T> vp[3], vm[3], v0[3];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
@@ -88,11 +92,88 @@ __global__ __launch_bounds__(256) void k_stencil(const cplx<RL> *__restrict__ u,
     for (int q = 0; q < RCHUNK; ++q) {
         if (!on[q]) continue;
         const int r = r0 + q;
-        const cplx<T> b = {(T)beta[r].re, (T)beta[r].im};
-        const cplx<T> *f = in + (int64_t)r * fs;
+        const cplx<T> bt = {(T)beta[r].re, (T)beta[r].im};
+        const cplx<T> *f = in + (int64_t)(map ? map[r] : r) * fs;
         cplx<T> *o = out + (int64_t)r * fs;
 #pragma unroll
-        for (int c = 0; c < 3; ++c) o[(int64_t)c * g.vol + s] = f[(int64_t)c * g.vol + s] + b * acc[q][c];
+        for (int c = 0; c < 3; ++c) {
+            const cplx<T> ax = f[(int64_t)c * g.vol + s] + bt * acc[q][c];
+            o[(int64_t)c * g.vol + s] = RESID ? b[(int64_t)r * fs + (int64_t)c * g.vol + s] - ax : ax;
+        }
+    }
+}
+
+// One Chebyshev step on the segment [1, 1 + beta_r kmax] of the spectrum of
+// A_r = 1 + beta_r K, fused with the operator application:
+//   x_{k+1} = rho_r (x_k + (b_r - A_r x_k) / c_r) + (1 - rho_r) x_{k-1}
+// c_r = 1 + beta_r kmax / 2 is the segment centre; x_{k-1} is read and
+// x_{k+1} written at the same site of `xio` (first step: rho = 1, no read).
+template <typename RL, typename T>
+__global__ __launch_bounds__(256) void k_cheb(const cplx<RL> *__restrict__ u, const cplx<T> *__restrict__ xk,
+                                              cplx<T> *__restrict__ xio, const cplx<T> *__restrict__ b,
+                                              const cplx<double> *__restrict__ beta, const cplx<double> *__restrict__ coef,
+                                              const int *__restrict__ active, int nf, geom g, lf_map mp, int first)
+{
+    int64_t s;
+    int chunk;
+    if (!lf_site(mp, g, s, chunk)) return;
+    const int r0 = chunk * RCHUNK;
+    const int64_t fs = 3 * g.vol;
+    cplx<T> acc[RCHUNK][3];
+    bool on[RCHUNK];
+#pragma unroll
+    for (int q = 0; q < RCHUNK; ++q) {
+        on[q] = (r0 + q < nf) && active[r0 + q];
+#pragma unroll
+        for (int c = 0; c < 3; ++c) acc[q][c] = {T(0), T(0)};
+    }
+    for (int i = 0; i < 3; ++i) {
+        const T ia2 = T(1.0 / (g.a[i + 1] * g.a[i + 1]));
+        const int64_t sp = neib_up(g, s, i), sm = neib_dn(g, s, i);
+        cplx<T> up[9], um[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            const cplx<RL> a = u[((int64_t)i * 9 + e) * g.vol + s];
+            const cplx<RL> bb = u[((int64_t)i * 9 + e) * g.vol + sm];
+            up[e] = {(T)a.re, (T)a.im};
+            um[e] = {(T)bb.re, (T)bb.im};
+        }
+#pragma unroll
+        for (int q = 0; q < RCHUNK; ++q) {
+            if (!on[q]) continue;
+            const cplx<T> *f = xk + (int64_t)(r0 + q) * fs;
+            cplx<T> vp[3], vm[3], v0[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                vp[c] = f[(int64_t)c * g.vol + sp];
+                vm[c] = f[(int64_t)c * g.vol + sm];
+                v0[c] = f[(int64_t)c * g.vol + s];
+            }
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                cplx<T> hop = up[3 * c] * vp[0] + up[3 * c + 1] * vp[1] + up[3 * c + 2] * vp[2];
+                hop = hop + cmul(um[c], vm[0]) + cmul(um[3 + c], vm[1]) + cmul(um[6 + c], vm[2]);
+                acc[q][c] = acc[q][c] + ia2 * (v0[c] - T(0.5) * hop);
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < RCHUNK; ++q) {
+        if (!on[q]) continue;
+        const int r = r0 + q;
+        const cplx<T> bt = {(T)beta[r].re, (T)beta[r].im};
+        const cplx<T> ic = {(T)coef[2 * r].re, (T)coef[2 * r].im};    // 1 / c_r
+        const cplx<T> rho = {(T)coef[2 * r + 1].re, (T)coef[2 * r + 1].im};
+        const cplx<T> one_m_rho = {T(1) - rho.re, -rho.im};
+        const int64_t base = (int64_t)r * fs + s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int64_t at = base + (int64_t)c * g.vol;
+            const cplx<T> x0 = xk[at];
+            const cplx<T> res = b[at] - (x0 + bt * acc[q][c]);
+            const cplx<T> nx = rho * (x0 + ic * res);
+            xio[at] = first ? nx : nx + one_m_rho * xio[at];
+        }
     }
 }
 
@@ -314,6 +395,7 @@ struct workspace {
     cplx<float> *y32 = nullptr, *x32 = nullptr, *t32 = nullptr, *res32 = nullptr;
     cplx<float> *links32 = nullptr;    // FP32 copy of the links for the inner solves
     cplx<double> *beta = nullptr, *coef = nullptr, *hdev = nullptr, *ydev = nullptr, *cpart = nullptr;
+    cplx<double> *cheb = nullptr;      // Chebyshev coefficients [iteration][slot][2]
     double *rpart = nullptr, *nrm = nullptr;
     int *active = nullptr, *nused = nullptr;
     cplx<double> *hbuf = nullptr;      // pinned host
@@ -321,7 +403,7 @@ struct workspace {
 
     void release()
     {
-        void *dev[] = {basis, y, x, t, res, scr, y32, x32, t32, res32, links32, beta, coef, hdev, ydev, cpart, rpart, nrm, active, nused};
+        void *dev[] = {basis, y, x, t, res, scr, y32, x32, t32, res32, links32, beta, coef, hdev, ydev, cpart, rpart, nrm, active, nused, cheb};
         for (void *q : dev)
             if (q) (void)hipFree(q);
         if (hbuf) (void)hipHostFree(hbuf);
@@ -341,6 +423,7 @@ struct workspace {
                   dm((void **)&x32, f32) && dm((void **)&t32, f32) && dm((void **)&res32, f32) &&
                   dm((void **)&links32, (size_t)27 * (size_t)v * sizeof(cplx<float>)) &&
                   dm((void **)&beta, sizeof(cplx<double>) * NSLOTS) && dm((void **)&coef, sizeof(cplx<double>) * NSLOTS) &&
+                  dm((void **)&cheb, sizeof(cplx<double>) * 2 * NSLOTS * CHEB_MAX) &&
                   dm((void **)&hdev, sizeof(cplx<double>) * NSLOTS * (mm + 1)) &&
                   dm((void **)&ydev, sizeof(cplx<double>) * NSLOTS * (mm + 1)) &&
                   dm((void **)&cpart, sizeof(cplx<double>) * NSLOTS * (mm + 1) * VBLOCKS) &&
@@ -381,12 +464,17 @@ static int upload_beta(const cplx<double> *beta, int nf)
 }
 
 template <typename RL, typename T>
-static int apply_t(const cplx<RL> *u, const cplx<T> *in, cplx<T> *out, int nf)
+static int apply_t(const cplx<RL> *u, const cplx<T> *in, cplx<T> *out, int nf, const int *map = nullptr,
+                   const cplx<T> *resid_b = nullptr)
 {
     const geom g = lat().g;
     const lf_map mp = make_lf_map(g, 256, 8, 1, (nf + RCHUNK - 1) / RCHUNK);
-    hipLaunchKernelGGL((k_stencil<RL, T>), dim3((unsigned)(mp.ngroup * mp.nsb)), dim3(256), 0, stream(), u, in, out,
-                       g_ws.beta, g_ws.active, nf, g, mp);
+    if (resid_b)
+        hipLaunchKernelGGL((k_stencil<RL, T, true>), dim3((unsigned)(mp.ngroup * mp.nsb)), dim3(256), 0, stream(), u, in,
+                           out, g_ws.beta, g_ws.active, nf, g, mp, map, resid_b);
+    else
+        hipLaunchKernelGGL((k_stencil<RL, T, false>), dim3((unsigned)(mp.ngroup * mp.nsb)), dim3(256), 0, stream(), u, in,
+                           out, g_ws.beta, g_ws.active, nf, g, mp, map, (const cplx<T> *)nullptr);
     NRQ_HIP(hipGetLastError());
     return NRQ_OK;
 }
@@ -567,6 +655,156 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
     return NRQ_OK;
 }
 
+// ---------------------------------------------------------------------------
+// Chebyshev iteration.  A_r = 1 + beta_r K is normal and K has its spectrum in
+// [0, kmax], kmax = sum_i 2/a_i^2 (K = sum_i (1 - (T_i + T_i^+)/2)/a_i^2 with
+// unitary hops T_i), so the spectrum of A_r lies on the segment from 1 to
+// 1 + beta_r kmax.  With centre c and half-width d of that segment and
+// sigma = c/d, the residual after k steps is bounded by ||r0|| / |T_k(sigma)|.
+// ---------------------------------------------------------------------------
+static double cheb_kmax()
+{
+    const geom &g = lat().g;
+    return 2.0 / (g.a[1] * g.a[1]) + 2.0 / (g.a[2] * g.a[2]) + 2.0 / (g.a[3] * g.a[3]);
+}
+
+// Smallest k <= CHEB_MAX with max_r 1/|T_k(sigma_r)| <= target.
+static int cheb_steps(const std::vector<std::complex<double>> &sig, double target)
+{
+    for (int k = 1; k < CHEB_MAX; ++k) {
+        double worst = 0;
+        for (const auto &sg : sig) {
+            std::complex<double> t0 = 1.0, t1 = sg;
+            for (int j = 1; j < k; ++j) {
+                const std::complex<double> t2 = 2.0 * sg * t1 - t0;
+                t0 = t1;
+                t1 = t2;
+            }
+            worst = std::max(worst, 1.0 / std::abs(t1));
+        }
+        if (worst <= target) return k;
+    }
+    return CHEB_MAX;
+}
+
+// k Chebyshev steps on A x = b from x (result back in x); xalt is scratch.
+template <typename RL, typename T>
+static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, cplx<T> *xalt,
+                    const std::vector<std::complex<double>> &beta, int k, double kmax, const std::vector<int> &act)
+{
+    std::vector<cplx<double>> tab((size_t)2 * NSLOTS * k);
+    for (int r = 0; r < R; ++r) {
+        const std::complex<double> c = 1.0 + beta[r] * (kmax / 2), d = beta[r] * (kmax / 2);
+        const std::complex<double> sg = c / d, ic = 1.0 / c;
+        std::complex<double> rho = 1.0;
+        for (int j = 0; j < k; ++j) {
+            if (j == 1) rho = 1.0 / (1.0 - 1.0 / (2.0 * sg * sg));
+            else if (j > 1) rho = 1.0 / (1.0 - rho / (4.0 * sg * sg));
+            tab[(size_t)2 * NSLOTS * j + 2 * r] = {ic.real(), ic.imag()};
+            tab[(size_t)2 * NSLOTS * j + 2 * r + 1] = {rho.real(), rho.imag()};
+        }
+    }
+    NRQ_HIP(hipMemcpyAsync(g_ws.cheb, tab.data(), sizeof(cplx<double>) * tab.size(), hipMemcpyHostToDevice, stream()));
+    const geom g = lat().g;
+    const lf_map mp = make_lf_map(g, 256, 8, 1, (R + RCHUNK - 1) / RCHUNK);
+    cplx<T> *cur = x, *other = xalt;
+    for (int j = 0; j < k; ++j) {
+        hipLaunchKernelGGL((k_cheb<RL, T>), dim3((unsigned)(mp.ngroup * mp.nsb)), dim3(256), 0, stream(), links,
+                           (const cplx<T> *)cur, other, b, g_ws.beta, g_ws.cheb + (size_t)2 * NSLOTS * j, g_ws.active, R, g,
+                           mp, j == 0 ? 1 : 0);
+        std::swap(cur, other);
+    }
+    NRQ_HIP(hipGetLastError());
+    if (cur != x) {
+        // odd step count: the result is in xalt; only active slots were iterated
+        const size_t fb = sizeof(cplx<T>) * (size_t)3 * g.vol;
+        for (int r = 0; r < R; ++r)
+            if (act[r]) NRQ_HIP(hipMemcpyAsync((char *)x + fb * r, (const char *)cur + fb * r, fb, hipMemcpyDeviceToDevice, stream()));
+    }
+    return NRQ_OK;
+}
+
+// FP64 true residual res = y - A x for the active slots; ||res||^2 to host.
+static int residual64(const nrq_gauge *gf, int R, double *rn2)
+{
+    if (gf->precision == 64) {
+        if (int e = apply_t<double, double>((const cplx<double> *)gauge_links(gf), g_ws.x, g_ws.res, R, nullptr, g_ws.y)) return e;
+    } else {
+        if (int e = apply_t<float, double>((const cplx<float> *)gauge_links(gf), g_ws.x, g_ws.res, R, nullptr, g_ws.y)) return e;
+    }
+    return norms2_t<double>(g_ws.res, R, rn2);
+}
+
+static int cheb_solve(const nrq_gauge *gf, int R, const std::vector<std::complex<double>> &beta, const double *bn,
+                      const krylov_params &p, std::vector<int> &act, std::vector<double> &rel, std::vector<int64_t> &its,
+                      int64_t &cycles)
+{
+    const int64_t n = 3 * g_ws.vol;
+    const double kmax = cheb_kmax();
+    std::vector<std::complex<double>> sig(R);
+    for (int r = 0; r < R; ++r) {
+        const std::complex<double> d = beta[r] * (kmax / 2);
+        sig[r] = std::abs(d) > 0 ? (1.0 + d) / d : std::complex<double>(1e30, 0.0);
+    }
+    const bool mixed = p.kind == KRYLOV_CHEB_MIXED;
+    const cplx<float> *l32 = nullptr;
+    if (mixed) {
+        if (gf->precision == 64) {
+            hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)gauge_links(gf),
+                               g_ws.links32, 27 * g_ws.vol);
+            l32 = g_ws.links32;
+        } else {
+            l32 = (const cplx<float> *)gauge_links(gf);
+        }
+    }
+    double rn2[NSLOTS];
+    for (int outer = 0; outer < 8; ++outer) {
+        if (int e = upload_active(act)) return e;
+        if (int e = residual64(gf, R, rn2)) return e;
+        double need = 1.0;
+        bool any = false;
+        for (int r = 0; r < R; ++r) {
+            if (!act[r]) continue;
+            const double rr = std::sqrt(std::max(rn2[r], 0.0));
+            rel[r] = rr / bn[r];
+            if (rr <= p.tol * bn[r]) {
+                act[r] = 0;
+                continue;
+            }
+            any = true;
+            need = std::min(need, 0.3 * p.tol * bn[r] / rr);
+        }
+        if (!any) return NRQ_OK;
+        ++cycles;
+        if (int e = upload_active(act)) return e;
+        if (!mixed) {
+            const int k = cheb_steps(sig, need);
+            if (gf->precision == 64) {
+                if (int e = cheb_run<double, double>((const cplx<double> *)gauge_links(gf), R, g_ws.y, g_ws.x, g_ws.t, beta, k, kmax, act))
+                    return e;
+            } else {
+                if (int e = cheb_run<float, double>((const cplx<float> *)gauge_links(gf), R, g_ws.y, g_ws.x, g_ws.t, beta, k, kmax, act))
+                    return e;
+            }
+            for (int r = 0; r < R; ++r) its[r] += act[r] ? k : 0;
+        } else {
+            // correction A d = r / ||r|| in FP32 to the accuracy FP32 allows
+            const int k = cheb_steps(sig, std::max(need, 3e-7));
+            double *rnorm2 = g_ws.nrm + 32;
+            NRQ_HIP(hipMemcpyAsync(rnorm2, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToDevice, stream()));
+            hipLaunchKernelGGL(k_demote, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<double> *)g_ws.res, g_ws.y32, n,
+                               (const double *)rnorm2, g_ws.active);
+            NRQ_HIP(hipMemsetAsync(g_ws.x32, 0, sizeof(cplx<float>) * (size_t)n * R, stream()));
+            if (int e = cheb_run<float, float>(l32, R, g_ws.y32, g_ws.x32, g_ws.t32, beta, k, kmax, act)) return e;
+            hipLaunchKernelGGL(k_promote_add, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.x, (const cplx<float> *)g_ws.x32, n,
+                               (const double *)rnorm2, g_ws.active);
+            NRQ_HIP(hipGetLastError());
+            for (int r = 0; r < R; ++r) its[r] += act[r] ? k : 0;
+        }
+    }
+    return NRQ_OK; // slots still active fall back to GMRES in the caller
+}
+
 int krylov_solve(const nrq_gauge *gf, int R, const cplx<double> *beta, const cplx<double> *alpha,
                  const cplx<double> *prev, const double *y_norm2, const krylov_params &p, nrq_solve_stats *st)
 {
@@ -590,6 +828,37 @@ int krylov_solve(const nrq_gauge *gf, int R, const cplx<double> *beta, const cpl
                                                (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, true, bn.data(), act, p.tol,
                                                p.restart, p.maxit, its, rel.data(), cycles);
         if (status) return status;
+    } else if (p.kind == KRYLOV_CHEB_FP64 || p.kind == KRYLOV_CHEB_MIXED) {
+        std::vector<std::complex<double>> bz(R);
+        for (int r = 0; r < R; ++r) bz[r] = std::complex<double>(beta[r].re, beta[r].im);
+        if (int e = upload_active(act)) return e;
+        if (p.extrapolate && prev && alpha) {
+            std::vector<cplx<double>> coef(R);
+            for (int r = 0; r < R; ++r) {
+                const std::complex<double> a(alpha[r].re, alpha[r].im);
+                const std::complex<double> c = std::abs(a) > 0 ? (a - bz[r]) / a : std::complex<double>(0.0);
+                coef[r] = {c.real(), c.imag()};
+            }
+            NRQ_HIP(hipMemcpyAsync(g_ws.coef, coef.data(), sizeof(cplx<double>) * R, hipMemcpyHostToDevice, stream()));
+            hipLaunchKernelGGL(k_guess, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.x, prev, g_ws.y, g_ws.coef, n, g_ws.active);
+        } else {
+            NRQ_HIP(hipMemsetAsync(g_ws.x, 0, sizeof(cplx<double>) * (size_t)n * R, stream()));
+        }
+        if (int e = cheb_solve(gf, R, bz, bn.data(), p, act, rel, its, cycles)) return e;
+        bool left = false;
+        for (int r = 0; r < R; ++r) left = left || act[r];
+        if (left) {
+            // spectrum bound violated (e.g. far from unitary links): GMRES from here
+            if (gf->precision == 64)
+                status = gmres_core<double, double>((const cplx<double> *)gauge_links(gf), R, g_ws.y, g_ws.x,
+                                                    (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, false, bn.data(), act,
+                                                    p.tol, p.restart, p.maxit, its, rel.data(), cycles);
+            else
+                status = gmres_core<double, float>((const cplx<float> *)gauge_links(gf), R, g_ws.y, g_ws.x,
+                                                   (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, false, bn.data(), act,
+                                                   p.tol, p.restart, p.maxit, its, rel.data(), cycles);
+            if (status) return status;
+        }
     } else {
         // FP32 links for the corrections
         const cplx<float> *l32;

@@ -133,16 +133,20 @@ def test_init_single_dof_matches_reference_indexing(nrq, lattice):
     assert A[ia - 1, ia - 1, 7 - 1] == 1.0 and np.count_nonzero(A) == 1
 
 
+SOLVERS = ["GMRES_MIXED", "CHEBYSHEV_FP64", "CHEBYSHEV_MIXED"]
+
+
+@pytest.mark.parametrize("kind", SOLVERS)
 @pytest.mark.parametrize("tol", [1e-12, 1e-10])
-def test_mixed_precision_solver_matches_oracle(nrq, oracle, cn, tol):
-    """FP32 corrections inside FP64 iterative refinement: the FP64 true
-    residual is the judge, so the result agrees with the FP64 oracle to the
-    same bar as the all-FP64 solver."""
+def test_fast_solvers_match_oracle(nrq, oracle, cn, tol, kind):
+    """Mixed-precision GMRES and the Chebyshev solvers: the FP64 true
+    residual is the judge, so each result agrees with the FP64 oracle to the
+    same bar as the all-FP64 GMRES."""
     ext = (8, 6, 4)
     sp = (0.05, 1.0, 0.9, 1.1)
-    _say(f"CN mixed precision tol {tol}")
+    _say(f"CN {kind} tol {tol}")
     cn(ext, sp, tol)
-    nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_GMRES_MIXED, 30, 10000)
+    nrq.nrqcd_set_solver(getattr(nrq, "NRQ_SOLVER_" + kind), 30, 10000)
     oracle.lattice_init(ext, sp)
     U, E = oracle.box_init(3, 0.5, 1.0, 1.0)
     g = nrq.GaugeConfiguration()
@@ -161,14 +165,15 @@ def test_mixed_precision_solver_matches_oracle(nrq, oracle, cn, tol):
     assert np.max(np.abs(Ag - A)) < bar * np.max(np.abs(A))
 
 
-def test_mixed_solver_on_fp32_gauge_storage(nrq, oracle, cn):
+@pytest.mark.parametrize("kind", SOLVERS)
+def test_fast_solvers_on_fp32_gauge_storage(nrq, oracle, cn, kind):
     """FP32 link storage: the FP64 outer residual uses the stored links, so
     the solve is exact for that operator; compare with the oracle on the
     FP32-rounded links."""
     ext = (6, 6, 6)
-    _say("CN mixed on FP32 links")
+    _say(f"CN {kind} on FP32 links")
     cn(ext, SP, 1e-11)
-    nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_GMRES_MIXED, 30, 10000)
+    nrq.nrqcd_set_solver(getattr(nrq, "NRQ_SOLVER_" + kind), 30, 10000)
     oracle.lattice_init(ext, SP)
     U, E = oracle.box_init(4, 0.5, 1.0, 1.0)
     U32 = np.asfortranarray(U.astype(np.complex64).astype(np.complex128))
@@ -182,3 +187,32 @@ def test_mixed_solver_on_fp32_gauge_storage(nrq, oracle, cn):
     oracle.cn_update(U32, Q, A, 2.0, C1, 1.0, 1e-13)
     Qg, Ag = h.download()
     assert np.max(np.abs(Qg - Q)) < 5e-9 * np.max(np.abs(Q))
+
+
+def test_chebyshev_complex_coefficient_and_anisotropy(nrq, oracle, cn):
+    """Chebyshev segment for complex c1 and anisotropic spacings, and the
+    reverse step (StepWidth < 0 flips the segment)."""
+    ext = (4, 6, 5)
+    sp = (0.1, 1.0, 0.7, 1.2)
+    _say("CN Chebyshev complex c1")
+    cn(ext, sp, 1e-12)
+    nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_CHEBYSHEV_FP64, 30, 10000)
+    oracle.lattice_init(ext, sp)
+    U, E = oracle.box_init(9, 0.7, 1.0, 1.0)
+    rng = np.random.default_rng(5)
+    v = oracle.vol
+    Q = np.asfortranarray(rng.normal(size=(6, 6, v)) + 1j * rng.normal(size=(6, 6, v)))
+    A = np.asfortranarray(rng.normal(size=(6, 6, v)) + 1j * rng.normal(size=(6, 6, v)))
+    c = [0.8 + 0.3j, 0, 0, 0]
+    g = nrq.GaugeConfiguration()
+    g.upload(U, E)
+    h = nrq.NRQCDField()
+    h.upload(Q, A)
+    for sw in (0.5, -1.0):
+        st = h.Update(g, 1.3, c, sw)
+        assert st.max_true_relres <= 1e-12
+        rc, _ = oracle.cn_update(U, Q, A, 1.3, c, sw, 1e-13)
+        assert rc == 0
+    Qg, Ag = h.download()
+    assert np.max(np.abs(Qg - Q)) < 2e-10 * np.max(np.abs(Q))
+    assert np.max(np.abs(Ag - A)) < 2e-10 * np.max(np.abs(A))
