@@ -233,6 +233,7 @@ extern "C" int nrq_heavy_create(nrq_heavy **out)
         return fail(NRQ_ERR_ALLOC, "heavy: device allocation failed");
     }
     (void)hipMemsetAsync(h->psi, 0, heavy_bytes(h->vol), stream());
+    h->live = 0;
     *out = h;
     return NRQ_OK;
 }
@@ -258,6 +259,7 @@ extern "C" int nrq_heavy_copy(nrq_heavy *dst, const nrq_heavy *src)
         if (hipMalloc(&dst->psi, heavy_bytes(dst->vol)) != hipSuccess) return fail(NRQ_ERR_ALLOC, "heavy: allocation failed");
     }
     NRQ_HIP(hipMemcpyAsync(dst->psi, src->psi, heavy_bytes(src->vol), hipMemcpyDeviceToDevice, stream()));
+    dst->live = src->live;
     return NRQ_OK;
 }
 
@@ -270,6 +272,10 @@ extern "C" int nrq_heavy_init_point(nrq_heavy *h, int64_t iq, int64_t ia)
     hipLaunchKernelGGL(k_set_point, dim3(1), dim3(64), 0, stream(), (cplx<double> *)h->psi, h->vol, 0, iq - 1, v);
     hipLaunchKernelGGL(k_set_point, dim3(1), dim3(64), 0, stream(), (cplx<double> *)h->psi, h->vol, 12, ia - 1, v);
     NRQ_HIP(hipGetLastError());
+    // fields (j, spin) with j - spin even carry the source, for both propagators
+    h->live = 0;
+    for (int t = 0; t < 12; ++t)
+        if ((t / 2 - t % 2) % 2 == 0) h->live |= (1u << t) | (1u << (12 + t));
     return NRQ_OK;
 }
 
@@ -286,10 +292,12 @@ extern "C" int nrq_heavy_init_point_dof(nrq_heavy *h, int spin_q, int colour_q, 
     // 2(c-1) + s, that dof index D (0-based) is colour D/2, spin D%2.
     const int dofs[2] = {colour_q + 3 * (spin_q - 1) - 1, colour_a + 3 * (spin_a - 1) - 1};
     const int64_t sites[2] = {iq - 1, ia - 1};
+    h->live = 0;
     for (int p = 0; p < 2; ++p) {
         const int D = dofs[p];
         const int col = D, c = D / 2, sp = D % 2;
         const int field = 12 * p + 2 * col + sp;
+        h->live |= 1u << field;
         const cplx<double> one = {1.0, 0.0};
         NRQ_HIP(hipMemcpyAsync((cplx<double> *)h->psi + ((int64_t)field * 3 + c) * h->vol + sites[p], &one, sizeof(one),
                                hipMemcpyHostToDevice, stream()));
@@ -306,13 +314,16 @@ extern "C" int nrq_heavy_upload(nrq_heavy *h, const double *quark, const double 
     const double *src[2] = {quark, antiq};
     for (int p = 0; p < 2; ++p) {
         if (!src[p]) continue;
+        unsigned live = 0;
         for (int64_t s = 0; s < v; ++s)
             for (int j = 0; j < 6; ++j)
                 for (int d = 0; d < 6; ++d) {
                     const int c = d / 2, sp = d % 2;
                     const double *z = src[p] + 2 * (d + 6 * j + 36 * s);
                     buf[((size_t)(2 * j + sp) * 3 + c) * (size_t)v + (size_t)s] = {z[0], z[1]};
+                    if (z[0] != 0.0 || z[1] != 0.0) live |= 1u << (12 * p + 2 * j + sp);
                 }
+        h->live = (h->live & ~(0xfffu << (12 * p))) | live;
         NRQ_HIP(hipMemcpyAsync((cplx<double> *)h->psi + (size_t)12 * p * 3 * (size_t)v, buf.data(), buf.size() * sizeof(cplx<double>),
                                hipMemcpyHostToDevice, stream()));
         NRQ_HIP(hipStreamSynchronize(stream()));
@@ -363,48 +374,24 @@ extern "C" int nrq_heavy_update(nrq_heavy *h, const nrq_gauge *gf, double mass, 
     const std::complex<double> mi(0.0, -(stepwidth / 2.0 * a0));
     const std::complex<double> alpha_q = mi * c1 / mass;
     const std::complex<double> alpha_a = mi * std::conj(c1) / (-mass);
-    std::vector<cplx<double>> beta(NFIELDS);
-    // rhs = S psi (SystMat_herm before the in-place Hermitian transpose) for
-    // all 24 fields, into the scratch slots
-    for (int r = 0; r < NFIELDS; ++r) {
-        const std::complex<double> a = r < 12 ? alpha_q : alpha_a;
-        beta[r] = {a.real(), a.imag()};
+    // The live fields are the right-hand sides (a field that is exactly zero
+    // stays zero, as KSPSolve returns x = 0 for b = 0).  rhs = S psi with
+    // S = 1 + alpha K (SystMat_herm before the in-place Hermitian
+    // transpose), then S^+ psi' = rhs with S^+ = 1 + conj(alpha) K.
+    int fields[NFIELDS];
+    cplx<double> alpha[NFIELDS], beta[NFIELDS];
+    int R = 0;
+    for (int f = 0; f < NFIELDS; ++f) {
+        if (!(h->live >> f & 1u)) continue;
+        const std::complex<double> a = f < 12 ? alpha_q : alpha_a;
+        fields[R] = f;
+        alpha[R] = {a.real(), a.imag()};
+        beta[R] = {a.real(), -a.imag()};
+        ++R;
     }
-    cplx<double> *scr = krylov_scratch();
-    if (int e = krylov_apply(gf, (const cplx<double> *)h->psi, scr, NFIELDS, beta.data())) return e;
-    double bn2_all[NFIELDS];
-    if (int e = krylov_norms2(scr, NFIELDS, bn2_all)) return e;
-    // compact the non-zero right-hand sides into slots 0..R-1 (a zero
-    // right-hand side has the zero solution, as KSPSolve returns for b = 0)
-    std::vector<int> field;
-    for (int r = 0; r < NFIELDS; ++r)
-        if (bn2_all[r] > 0.0) field.push_back(r);
-    const int R = (int)field.size();
     st.solves = R;
-    const size_t fb = (size_t)3 * (size_t)h->vol * sizeof(cplx<double>);
-    double bn2[NFIELDS];
-    std::vector<cplx<double>> sbeta(NFIELDS), salpha(NFIELDS);
-    cplx<double> *y = krylov_rhs();
-    for (int q = 0; q < R; ++q) {
-        const int r = field[q];
-        NRQ_HIP(hipMemcpyAsync((char *)y + fb * q, (char *)scr + fb * r, fb, hipMemcpyDeviceToDevice, stream()));
-        bn2[q] = bn2_all[r];
-        // operator S^+ = 1 + conj(alpha) K
-        const std::complex<double> a = r < 12 ? alpha_q : alpha_a;
-        salpha[q] = {a.real(), a.imag()};
-        sbeta[q] = {a.real(), -a.imag()};
-    }
-    // previous fields in slot order (for the extrapolated initial guess)
-    for (int q = 0; q < R; ++q)
-        NRQ_HIP(hipMemcpyAsync((char *)scr + fb * q, (char *)h->psi + fb * field[q], fb, hipMemcpyDeviceToDevice, stream()));
-    int status = NRQ_OK;
-    if (R > 0) status = krylov_solve(gf, R, sbeta.data(), salpha.data(), scr, bn2, g_cn.solver, &st);
-    // scatter the solutions back; fields with a zero right-hand side are zero
-    const cplx<double> *x = krylov_solution();
-    for (int r = 0; r < NFIELDS; ++r)
-        if (!(bn2_all[r] > 0.0)) NRQ_HIP(hipMemsetAsync((char *)h->psi + fb * r, 0, fb, stream()));
-    for (int q = 0; q < R; ++q)
-        NRQ_HIP(hipMemcpyAsync((char *)h->psi + fb * field[q], (const char *)x + fb * q, fb, hipMemcpyDeviceToDevice, stream()));
+    const int status = krylov_cn_step(gf, (cplx<double> *)h->psi, fields, R, alpha, beta, g_cn.solver, &st);
+    if (status != NRQ_OK && status != NRQ_ERR_NOCONV) return status;
     NRQ_HIP(hipStreamSynchronize(stream()));
     if (stats) *stats = st;
     return status;

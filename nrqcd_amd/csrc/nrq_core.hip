@@ -175,6 +175,10 @@ extern "C" int nrq_lattice_init(const int64_t ext[3], const double spacing[4])
     for (int i = 0; i < 4; ++i)
         if (!(spacing[i] > 0) || !std::isfinite(spacing[i]))
             return fail(NRQ_ERR_ARG, "lattice: spacing a%d = %g must be positive", i, spacing[i]);
+    // kernels index sites and grids with 32-bit arithmetic
+    if (ext[0] * ext[1] * ext[2] > (int64_t)INT32_MAX / 32)
+        return fail(NRQ_ERR_ARG, "lattice: volume %lld exceeds %d sites", (long long)(ext[0] * ext[1] * ext[2]),
+                    INT32_MAX / 32);
     geom &g = lat().g;
     g.n0 = (int32_t)ext[0];
     g.n1 = (int32_t)ext[1];

@@ -53,7 +53,7 @@ __device__ __forceinline__ mat3<R> evolve_link(const mat3<R> &u, const R *e, R d
 // with S^U = U_k(x+i) U_i(x+k)^+ U_k(x)^+ and
 //      S^D = U_k(x+i-k)^+ U_i(x-k)^+ U_k(x-k).
 template <typename R>
-__device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, const geom &g, int i, int64_t s,
+__device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, const geom &g, int i, const lsite &p,
                                               const mat3<R> &ui, R *e, R dt)
 {
     mat3<R> sum;
@@ -62,12 +62,13 @@ __device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, con
     for (int k = 0; k < 3; ++k) {
         if (k == i) continue;
         const R coef = dt * R((g.a[0] / g.a[k + 1]) * (g.a[0] / g.a[k + 1]));
-        const int64_t xi = neib_up(g, s, i);
-        const int64_t xk = neib_up(g, s, k);
-        const int64_t xmk = neib_dn(g, s, k);
-        const int64_t ximk = neib_up(g, xmk, i);
+        const lsite pmk = site_dn(g, p, k);
+        const int64_t xi = site_up(g, p, i).s;
+        const int64_t xk = site_up(g, p, k).s;
+        const int64_t xmk = pmk.s;
+        const int64_t ximk = site_up(g, pmk, i).s;
         mat3<R> su = mul_nd(load_link(u, g.vol, k, xi), load_link(u, g.vol, i, xk));
-        su = mul_nd(su, load_link(u, g.vol, k, s));
+        su = mul_nd(su, load_link(u, g.vol, k, p.s));
         mat3<R> sd = mul_dd(load_link(u, g.vol, k, ximk), load_link(u, g.vol, i, xmk));
         sd = mul(sd, load_link(u, g.vol, k, xmk));
 #pragma unroll
@@ -84,9 +85,10 @@ template <typename R, int BS>
 __global__ __launch_bounds__(BS) void k_links(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
                                               const R *__restrict__ ef, geom g, R dt, lf_map m)
 {
-    int64_t s;
+    lsite p;
     int i;
-    if (!lf_site(m, g, s, i)) return;
+    if (!lf_site(m, g, p, i)) return;
+    const int64_t s = p.s;
     const int64_t vol = g.vol;
     R e[8];
 #pragma unroll
@@ -99,14 +101,15 @@ template <typename R, int BS>
 __global__ __launch_bounds__(BS) void k_efield(const cplx<R> *__restrict__ u, R *__restrict__ ef, geom g, R dt,
                                                lf_map m)
 {
-    int64_t s;
+    lsite p;
     int i;
-    if (!lf_site(m, g, s, i)) return;
+    if (!lf_site(m, g, p, i)) return;
+    const int64_t s = p.s;
     R e[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * g.vol + s];
     mat3<R> ui = load_link(u, g.vol, i, s);
-    evolve_efield(u, g, i, s, ui, e, dt);
+    evolve_efield(u, g, i, p, ui, e, dt);
 #pragma unroll
     for (int a = 0; a < 8; ++a) ef[((int64_t)i * 8 + a) * g.vol + s] = e[a];
 }
@@ -120,14 +123,15 @@ template <typename R, int BS>
 __global__ __launch_bounds__(BS) void k_fused(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
                                               R *__restrict__ ef, geom g, R dt, lf_map m)
 {
-    int64_t s;
+    lsite p;
     int i;
-    if (!lf_site(m, g, s, i)) return;
+    if (!lf_site(m, g, p, i)) return;
+    const int64_t s = p.s;
     R e[8];
 #pragma unroll
     for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * g.vol + s];
     mat3<R> ui = load_link(uin, g.vol, i, s);
-    evolve_efield(uin, g, i, s, ui, e, dt);
+    evolve_efield(uin, g, i, p, ui, e, dt);
 #pragma unroll
     for (int a = 0; a < 8; ++a) ef[((int64_t)i * 8 + a) * g.vol + s] = e[a];
     store_link(uout, g.vol, i, s, evolve_link(ui, e, dt));
@@ -168,12 +172,13 @@ __global__ __launch_bounds__(REDUCE_THREADS) void k_obs(const cplx<R> *__restric
     double acc[NOBS] = {0, 0, 0, 0, 0};
     const double a0 = g.a[0];
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < g.vol; s += (int64_t)gridDim.x * blockDim.x) {
+        const lsite p = make_site(g, s);
         mat3<double> gsum;
         set_zero(gsum);
         for (int i = 0; i < 3; ++i) {
             const double ai = g.a[i + 1];
             double ed[8], en[8];
-            const int64_t sm = neib_dn(g, s, i);
+            const int64_t sm = site_dn(g, p, i).s;
             double esq = 0;
             for (int a = 0; a < 8; ++a) {
                 ed[a] = (double)ef[((int64_t)i * 8 + a) * g.vol + s];
@@ -204,7 +209,7 @@ __global__ __launch_bounds__(REDUCE_THREADS) void k_obs(const cplx<R> *__restric
             for (int m = 0; m < 9; ++m) gsum.m[m] = gsum.m[m] + (1.0 / ai) * dm.m[m];
             // magnetic part: plaquettes P_ij, j > i
             for (int j = i + 1; j < 3; ++j) {
-                const int64_t xi = neib_up(g, s, i), xj = neib_up(g, s, j);
+                const int64_t xi = site_up(g, p, i).s, xj = site_up(g, p, j).s;
                 mat3<double> l1, l2, l3, l4;
                 {
                     mat3<R> a1 = load_link(u, g.vol, i, s), a2 = load_link(u, g.vol, j, xi);
@@ -311,7 +316,8 @@ template <typename R> __global__ void k_get_link(const cplx<R> *u, int64_t vol, 
 template <typename R> __global__ void k_plaq(const cplx<R> *u, geom g, int i, int j, int64_t s, double *out)
 {
     if (threadIdx.x != 0) return;
-    const int64_t xi = neib_up(g, s, i), xj = neib_up(g, s, j);
+    const lsite ps = make_site(g, s);
+    const int64_t xi = site_up(g, ps, i).s, xj = site_up(g, ps, j).s;
     mat3<double> l[4];
     mat3<R> a[4] = {load_link(u, g.vol, i, s), load_link(u, g.vol, j, xi), load_link(u, g.vol, i, xj), load_link(u, g.vol, j, s)};
     for (int k = 0; k < 4; ++k)

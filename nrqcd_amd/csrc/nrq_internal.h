@@ -64,4 +64,5 @@ struct nrq_gauge {
 struct nrq_heavy {
     int64_t vol = 0;
     void *psi = nullptr;      // 24 colour-vector fields [field][colour][site] complex double
+    unsigned live = 0;        // bit f set: field f may be non-zero (clear bits are exactly zero)
 };

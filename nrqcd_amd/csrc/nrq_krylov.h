@@ -1,10 +1,10 @@
-// nrq_krylov.h -- batched Krylov solver for the CN systems (1 + b_r K) x_r = y_r
-// of R right-hand-side slots, replacing PETSc KSP (nrqcd.f90:786-804).
+// nrq_krylov.h -- batched solvers for the CN systems of the NRQCD heavy
+// fields, replacing PETSc KSP (nrqcd.f90:786-804, 1097-1161).
 //
 // K psi(x) = sum_i [psi(x) - (U_i(x) psi(x+i) + U_i(x-i)^+ psi(x-i))/2] / a_i^2
 // is the matrix-free covariant Laplacian of the kinetic NRQCD Hamiltonian;
-// each slot is a colour-vector field psi[slot][colour][site] and has its own
-// complex coefficient b_r (quark and antiquark, or different masses, share one
+// each slot is a colour-vector field psi[slot][colour][site] with its own
+// complex coefficients (quark and antiquark, or different masses, share one
 // batched solve and one link stream).
 #pragma once
 #include "nrq_internal.h"
@@ -26,27 +26,20 @@ struct krylov_params {
     int restart = 30;          // GMRES restart length m (PETSc default 30)
     int64_t maxit = 10000;     // cap on Krylov iterations per slot
     double tol = 1e-10;        // FP64 true relative residual
-    double inner_tol = 1e-5;   // relative residual of each FP32 correction solve
+    double inner_tol = 1e-5;   // relative residual of each FP32 GMRES correction
     bool extrapolate = false;  // first-order initial guess instead of zero
 };
 
-// Solves (1 + beta_r K) x_r = y_r for r < R.  y (slot-major, 3V complex
-// doubles each) must be in krylov_rhs(); the solution is left in
-// krylov_solution().  `prev` (may be null) holds the fields whose CN update
-// produced y (used by the extrapolated initial guess with coefficient
-// alpha_r: y = (1 + alpha_r K) prev).
-int krylov_solve(const nrq_gauge *gf, int R, const cplx<double> *beta, const cplx<double> *alpha,
-                 const cplx<double> *prev_slots, const double *y_norm2, const krylov_params &p, nrq_solve_stats *st);
+// One CN step of R fields of the 24-field storage psi ([field][colour][site]
+// complex doubles): for q < R and f = fields[q],
+//   y_q = (1 + alpha_q K) psi_f,  solve (1 + beta_q K) x_q = y_q,  psi_f = x_q
+// to an FP64 true relative residual <= p.tol.  Statistics accumulate into st.
+int krylov_cn_step(const nrq_gauge *gf, cplx<double> *psi, const int *fields, int R, const cplx<double> *alpha,
+                   const cplx<double> *beta, const krylov_params &p, nrq_solve_stats *st);
 
 int krylov_reserve(int64_t vol, int m);
 void krylov_release();
-cplx<double> *krylov_rhs();
-cplx<double> *krylov_solution();
-cplx<double> *krylov_scratch(); // NSLOTS x 3V complex doubles
-
-// y_r = x_r + beta_r K x_r for r < nf (FP64 fields); beta on the host.
-int krylov_apply(const nrq_gauge *gf, const cplx<double> *x, cplx<double> *y, int nf, const cplx<double> *beta);
-// squared norms of nf FP64 fields into out[nf] (host)
+// squared norms of nf consecutive FP64 fields into out[nf] (host)
 int krylov_norms2(const cplx<double> *x, int nf, double *out);
 // partial sums for the correlator kernel: device scratch of >= 8 x 256 doubles
 double *krylov_reduce_scratch();

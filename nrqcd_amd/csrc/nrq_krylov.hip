@@ -1,181 +1,43 @@
-// nrq_krylov.hip -- batched Krylov solver for the NRQCD Crank-Nicolson systems
+// nrq_krylov.hip -- batched solvers for the NRQCD Crank-Nicolson systems
 // (replaces PETSc KSP, nrqcd.f90:786-804, 1103-1105, 1123, 1161).
 //
-// Two drivers share one GMRES(m) core with classical Gram-Schmidt:
-//  * KRYLOV_GMRES_FP64: GMRES(m) in FP64 from a zero guess (PETSc's defaults,
-//    judged on the FP64 true residual).
-//  * KRYLOV_GMRES_MIXED: iterative refinement.  The FP64 true residual
-//    r = y - A x is formed with the FP64 operator; the correction A d = r is
-//    solved by GMRES in FP32 (FP32 copy of the links, FP32 vectors) to
-//    inner_tol; x += d; repeat until ||r|| <= tol ||y||.  The FP64 residual
-//    is the judge, so FP32 changes the iteration count, never the answer.
-//
-// All slots advance together: one stencil launch per iteration streams the
-// links once per chunk of RCHUNK slots; the host keeps the small Hessenberg
-// systems (one Givens-rotated least-squares problem per slot).
+// A CN step of R colour-vector fields psi_r solves
+//   (1 + beta_r K) x_r = y_r,   y_r = (1 + alpha_r K) psi_r
+// for all r at once; every stencil launch streams the links once per chunk of
+// NC slots.  Solvers:
+//  * KRYLOV_GMRES_FP64: GMRES(m), classical Gram-Schmidt, FP64, zero guess
+//    (PETSc's defaults).
+//  * KRYLOV_GMRES_MIXED: FP64 iterative refinement around FP32 GMRES(m)
+//    corrections.
+//  * KRYLOV_CHEB_FP64 / KRYLOV_CHEB_MIXED: Chebyshev iteration on the known
+//    spectral segment of A_r, in FP64 or as FP32 corrections inside FP64
+//    iterative refinement.  Step counts are planned from a priori bounds, so
+//    the usual CN step needs one host synchronisation; the FP64 true residual
+//    then confirms convergence (and more steps, or GMRES, follow if not).
+// In every case ||y - A x|| <= tol ||y|| with the FP64 operator decides
+// convergence: precision and solver change the work, never the answer.
 #include "nrq_krylov.h"
 #include "nrq_map.h"
 #include <algorithm>
 #include <cmath>
 #include <complex>
+#include <cstdio>
+#include <cstdlib>
 #include <cstring>
 
 namespace nrq {
 
 const void *gauge_links(const nrq_gauge *h);
 
-constexpr int RCHUNK = 4;     // slots per thread in the stencil (link reuse)
 constexpr int CHEB_MAX = 64;  // Chebyshev steps per coefficient table
 constexpr int VBLOCKS = 256;  // blocks per slot in the vector kernels
 constexpr int VTHREADS = 256;
+constexpr int SBS = 256;      // threads per block in the stencil kernels
+constexpr double FP32_CORR_ERR = 1e-6; // relative error assumed for an FP32 correction
 
 // ---------------------------------------------------------------------------
-// kernels
+// device helpers
 // ---------------------------------------------------------------------------
-
-// out_r = in_r + beta_r K in_r for the active slots of one chunk; links of
-// type RL are converted to the compute type T on load and reused across the
-// RCHUNK slots of the chunk.
-// RESID: out_r = b_r - (in_r + beta_r K in_r) instead.  map (may be null)
-// gives the input field of slot r (in + map[r] * 3V).
-template <typename RL, typename T, bool RESID>
-__global__ __launch_bounds__(256) void k_stencil(const cplx<RL> *__restrict__ u, const cplx<T> *__restrict__ in,
-                                                 cplx<T> *__restrict__ out, const cplx<double> *__restrict__ beta,
-                                                 const int *__restrict__ active, int nf, geom g, lf_map mp,
-                                                 const int *__restrict__ map, const cplx<T> *__restrict__ b)
-{
-    int64_t s;
-    int chunk;
-    if (!lf_site(mp, g, s, chunk)) return;
-    const int r0 = chunk * RCHUNK;
-    const int64_t fs = 3 * g.vol;
-    cplx<T> acc[RCHUNK][3];
-    bool on[RCHUNK];
-#pragma unroll
-    for (int q = 0; q < RCHUNK; ++q) {
-        on[q] = (r0 + q < nf) && active[r0 + q];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[q][c] = {T(0), T(0)};
-    }
-    for (int i = 0; i < 3; ++i) {
-        const T ia2 = T(1.0 / (g.a[i + 1] * g.a[i + 1]));
-        const int64_t sp = neib_up(g, s, i), sm = neib_dn(g, s, i);
-        cplx<T> up[9], um[9];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) {
-            const cplx<RL> a = u[((int64_t)i * 9 + e) * g.vol + s];
-            const cplx<RL> b = u[((int64_t)i * 9 + e) * g.vol + sm];
-            up[e] = {(T)a.re, (T)a.im};
-            um[e] = {(T)b.re, (T)b.im};
-        }
-#pragma unroll
-        for (int q = 0; q < RCHUNK; ++q) {
-            if (!on[q]) continue;
-            const cplx<T> *f = in + (int64_t)(map ? map[r0 + q] : r0 + q) * fs;
-            cplx<T> vp[3], vm[3], v0[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                vp[c] = f[(int64_t)c * g.vol + sp];
-                vm[c] = f[(int64_t)c * g.vol + sm];
-                v0[c] = f[(int64_t)c * g.vol + s];
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                // U_i(x) psi(x+i) + U_i(x-i)^+ psi(x-i)
-                cplx<T> hop = up[3 * c] * vp[0] + up[3 * c + 1] * vp[1] + up[3 * c + 2] * vp[2];
-                hop = hop + cmul(um[c], vm[0]) + cmul(um[3 + c], vm[1]) + cmul(um[6 + c], vm[2]);
-                acc[q][c] = acc[q][c] + ia2 * (v0[c] - T(0.5) * hop);
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < RCHUNK; ++q) {
-        if (!on[q]) continue;
-        const int r = r0 + q;
-        const cplx<T> bt = {(T)beta[r].re, (T)beta[r].im};
-        const cplx<T> *f = in + (int64_t)(map ? map[r] : r) * fs;
-        cplx<T> *o = out + (int64_t)r * fs;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const cplx<T> ax = f[(int64_t)c * g.vol + s] + bt * acc[q][c];
-            o[(int64_t)c * g.vol + s] = RESID ? b[(int64_t)r * fs + (int64_t)c * g.vol + s] - ax : ax;
-        }
-    }
-}
-
-// One Chebyshev step on the segment [1, 1 + beta_r kmax] of the spectrum of
-// A_r = 1 + beta_r K, fused with the operator application:
-//   x_{k+1} = rho_r (x_k + (b_r - A_r x_k) / c_r) + (1 - rho_r) x_{k-1}
-// c_r = 1 + beta_r kmax / 2 is the segment centre; x_{k-1} is read and
-// x_{k+1} written at the same site of `xio` (first step: rho = 1, no read).
-template <typename RL, typename T>
-__global__ __launch_bounds__(256) void k_cheb(const cplx<RL> *__restrict__ u, const cplx<T> *__restrict__ xk,
-                                              cplx<T> *__restrict__ xio, const cplx<T> *__restrict__ b,
-                                              const cplx<double> *__restrict__ beta, const cplx<double> *__restrict__ coef,
-                                              const int *__restrict__ active, int nf, geom g, lf_map mp, int first)
-{
-    int64_t s;
-    int chunk;
-    if (!lf_site(mp, g, s, chunk)) return;
-    const int r0 = chunk * RCHUNK;
-    const int64_t fs = 3 * g.vol;
-    cplx<T> acc[RCHUNK][3];
-    bool on[RCHUNK];
-#pragma unroll
-    for (int q = 0; q < RCHUNK; ++q) {
-        on[q] = (r0 + q < nf) && active[r0 + q];
-#pragma unroll
-        for (int c = 0; c < 3; ++c) acc[q][c] = {T(0), T(0)};
-    }
-    for (int i = 0; i < 3; ++i) {
-        const T ia2 = T(1.0 / (g.a[i + 1] * g.a[i + 1]));
-        const int64_t sp = neib_up(g, s, i), sm = neib_dn(g, s, i);
-        cplx<T> up[9], um[9];
-#pragma unroll
-        for (int e = 0; e < 9; ++e) {
-            const cplx<RL> a = u[((int64_t)i * 9 + e) * g.vol + s];
-            const cplx<RL> bb = u[((int64_t)i * 9 + e) * g.vol + sm];
-            up[e] = {(T)a.re, (T)a.im};
-            um[e] = {(T)bb.re, (T)bb.im};
-        }
-#pragma unroll
-        for (int q = 0; q < RCHUNK; ++q) {
-            if (!on[q]) continue;
-            const cplx<T> *f = xk + (int64_t)(r0 + q) * fs;
-            cplx<T> vp[3], vm[3], v0[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                vp[c] = f[(int64_t)c * g.vol + sp];
-                vm[c] = f[(int64_t)c * g.vol + sm];
-                v0[c] = f[(int64_t)c * g.vol + s];
-            }
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                cplx<T> hop = up[3 * c] * vp[0] + up[3 * c + 1] * vp[1] + up[3 * c + 2] * vp[2];
-                hop = hop + cmul(um[c], vm[0]) + cmul(um[3 + c], vm[1]) + cmul(um[6 + c], vm[2]);
-                acc[q][c] = acc[q][c] + ia2 * (v0[c] - T(0.5) * hop);
-            }
-        }
-    }
-#pragma unroll
-    for (int q = 0; q < RCHUNK; ++q) {
-        if (!on[q]) continue;
-        const int r = r0 + q;
-        const cplx<T> bt = {(T)beta[r].re, (T)beta[r].im};
-        const cplx<T> ic = {(T)coef[2 * r].re, (T)coef[2 * r].im};    // 1 / c_r
-        const cplx<T> rho = {(T)coef[2 * r + 1].re, (T)coef[2 * r + 1].im};
-        const cplx<T> one_m_rho = {T(1) - rho.re, -rho.im};
-        const int64_t base = (int64_t)r * fs + s;
-#pragma unroll
-        for (int c = 0; c < 3; ++c) {
-            const int64_t at = base + (int64_t)c * g.vol;
-            const cplx<T> x0 = xk[at];
-            const cplx<T> res = b[at] - (x0 + bt * acc[q][c]);
-            const cplx<T> nx = rho * (x0 + ic * res);
-            xio[at] = first ? nx : nx + one_m_rho * xio[at];
-        }
-    }
-}
 
 __device__ __forceinline__ double wsum(double v)
 {
@@ -197,6 +59,314 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
         for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
     return t;
 }
+
+template <typename T> __device__ __forceinline__ double nrm2(cplx<T> a) { return (double)a.re * a.re + (double)a.im * a.im; }
+
+template <typename T> __device__ __forceinline__ cplx<T> to(cplx<double> a) { return {(T)a.re, (T)a.im}; }
+
+__device__ __forceinline__ double g_dsum(const geom &g)
+{
+    return 1.0 / (g.a[1] * g.a[1]) + 1.0 / (g.a[2] * g.a[2]) + 1.0 / (g.a[3] * g.a[3]);
+}
+
+// Per-slot fields of one chunk, read in the compute type T.
+template <int NC, typename T, typename S> struct ld_fields {
+    const cplx<S> *f[NC];
+    __device__ __forceinline__ cplx<T> operator()(int q, int64_t at) const
+    {
+        const cplx<S> v = f[q][at];
+        return {(T)v.re, (T)v.im};
+    }
+};
+
+// z = x + d per slot (d[q] null: no correction for that slot).
+template <int NC, typename TD> struct ld_xd {
+    const cplx<double> *x[NC];
+    const cplx<TD> *d[NC];
+    __device__ __forceinline__ cplx<double> operator()(int q, int64_t at) const
+    {
+        cplx<double> v = x[q][at];
+        if (d[q]) {
+            const cplx<TD> e = d[q][at];
+            v = v + cplx<double>{(double)e.re, (double)e.im};
+        }
+        return v;
+    }
+};
+
+// hop_q = sum_i (U_i(x) f_q(x+i) + U_i(x-i)^+ f_q(x-i)) / a_i^2 at x = s for
+// the slots q < NC of one chunk with on[q]; with dsum = sum_i 1/a_i^2 the
+// kinetic operator of nrqcd.f90:1210-1242 (without its coefficient) is
+//   K f(s) = dsum f(s) - hop / 2
+//          = sum_i [f(s) - (U_i(s) f(s+i) + U_i(s-i)^+ f(s-i))/2] / a_i^2.
+// ld(q, at) returns component at = colour * V + site of slot q's field.
+// The links are loaded once for the whole chunk.
+template <int NC, typename T, typename RL, typename LD>
+__device__ __forceinline__ void hop_chunk(const cplx<RL> *__restrict__ u, const LD &ld, const bool (&on)[NC], const lsite &p,
+                                          const geom &g, cplx<T> (&hop)[NC][3])
+{
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) hop[q][c] = {T(0), T(0)};
+    for (int i = 0; i < 3; ++i) {
+        const T ia2 = T(1.0 / (g.a[i + 1] * g.a[i + 1]));
+        const int64_t s = p.s, sp = site_up(g, p, i).s, sm = site_dn(g, p, i).s;
+        cplx<T> up[9];
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            const cplx<RL> a = u[((int64_t)i * 9 + e) * g.vol + s];
+            up[e] = {(T)(ia2 * a.re), (T)(ia2 * a.im)};
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            if (!on[q]) continue;
+            cplx<T> vp[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vp[c] = ld(q, (int64_t)c * g.vol + sp);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                hop[q][c] = hop[q][c] + up[3 * c] * vp[0] + up[3 * c + 1] * vp[1] + up[3 * c + 2] * vp[2];
+        }
+#pragma unroll
+        for (int e = 0; e < 9; ++e) {
+            const cplx<RL> a = u[((int64_t)i * 9 + e) * g.vol + sm];
+            up[e] = {(T)(ia2 * a.re), (T)(ia2 * a.im)};
+        }
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            if (!on[q]) continue;
+            cplx<T> vm[3];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) vm[c] = ld(q, (int64_t)c * g.vol + sm);
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                hop[q][c] = hop[q][c] + cmul(up[c], vm[0]) + cmul(up[3 + c], vm[1]) + cmul(up[6 + c], vm[2]);
+        }
+    }
+}
+
+// Per-slot block sums of v[q]; every thread of the block must call this.
+// Thread q < NC writes partial[(r0 + q) * nsb + t] for slots r0 + q < nf.
+template <int NC>
+__device__ __forceinline__ void chunk_partials(double (&v)[NC], int r0, int nf, int64_t t, int64_t nsb,
+                                               double *__restrict__ partial)
+{
+    __shared__ double sh[NC][SBS / 64];
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        const double x = wsum(v[q]);
+        if (lane == 0) sh[q][w] = x;
+    }
+    __syncthreads();
+    if (threadIdx.x < NC && r0 + (int)threadIdx.x < nf) {
+        const int q = threadIdx.x;
+        double x = 0;
+        for (int k = 0; k < SBS / 64; ++k) x += sh[q][k];
+        partial[(int64_t)(r0 + q) * nsb + t] = x;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// stencil kernels (grid: lf_map with one work item per chunk of NC slots)
+// ---------------------------------------------------------------------------
+
+// out_r = in_r + beta_r K in_r for the active slots.
+template <int NC, typename RL, typename T>
+__global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u, const cplx<T> *__restrict__ in,
+                                                 cplx<T> *__restrict__ out, const cplx<double> *__restrict__ beta,
+                                                 const int *__restrict__ active, int nf, geom g, lf_map mp)
+{
+    lsite p;
+    int chunk;
+    if (!lf_site(mp, g, p, chunk)) return;
+    const int64_t s = p.s;
+    const int r0 = chunk * NC;
+    const int64_t fs = 3 * g.vol;
+    bool on[NC];
+    ld_fields<NC, T, T> ld;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        on[q] = (r0 + q < nf) && active[r0 + q];
+        ld.f[q] = in + (int64_t)(on[q] ? r0 + q : 0) * fs;
+    }
+    cplx<T> hop[NC][3];
+    hop_chunk<NC, T>(u, ld, on, p, g, hop);
+    const T dsum = T(g_dsum(g));
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        if (!on[q]) continue;
+        const cplx<T> bt = to<T>(beta[r0 + q]);
+        cplx<T> *o = out + (int64_t)(r0 + q) * fs + s;
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const cplx<T> v0 = ld(q, (int64_t)c * g.vol + s);
+            o[(int64_t)c * g.vol] = v0 + bt * (dsum * v0 - T(0.5) * hop[q][c]);
+        }
+    }
+}
+
+// Right-hand side and initial guess of a CN step, for slot r and field
+// f = fmap[r] of psi:
+//   y_r = psi_f + alpha_r K psi_f                     (S psi, nrqcd.f90:1112)
+//   x_r = psi_f + g_r K psi_f  (guess != 0)  or  x_r = 0
+// with g_r = alpha_r - beta_r the first-order expansion of
+// (1 + beta K)^-1 (1 + alpha K).  Partial ||y_r||^2 per site block.
+template <int NC, typename RL>
+__global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, const cplx<double> *__restrict__ psi,
+                                                const int *__restrict__ fmap, cplx<double> *__restrict__ y,
+                                                cplx<double> *__restrict__ x, const cplx<double> *__restrict__ alpha,
+                                                const cplx<double> *__restrict__ gco, int guess, int nf, geom g, lf_map mp,
+                                                double *__restrict__ partial)
+{
+    lsite p = {0, 0, 0, 0};
+    int64_t t = 0;
+    int chunk;
+    const bool valid = lf_site(mp, g, p, chunk, t);
+    const int64_t s = p.s;
+    const int r0 = chunk * NC;
+    const int64_t fs = 3 * g.vol;
+    bool on[NC];
+    ld_fields<NC, double, double> ld;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        on[q] = valid && (r0 + q < nf);
+        ld.f[q] = psi + (int64_t)(on[q] ? fmap[r0 + q] : 0) * fs;
+    }
+    double nrm[NC] = {};
+    if (valid) {
+        cplx<double> hop[NC][3];
+        hop_chunk<NC, double>(u, ld, on, p, g, hop);
+        const double dsum = g_dsum(g);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            if (!on[q]) continue;
+            const int r = r0 + q;
+            const cplx<double> a = alpha[r], gc = gco[r];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int64_t at = (int64_t)r * fs + (int64_t)c * g.vol + s;
+                const cplx<double> v0 = ld(q, (int64_t)c * g.vol + s);
+                const cplx<double> kf = dsum * v0 - 0.5 * hop[q][c];
+                const cplx<double> yv = v0 + a * kf;
+                y[at] = yv;
+                x[at] = guess ? v0 + gc * kf : cplx<double>{0.0, 0.0};
+                nrm[q] += nrm2(yv);
+            }
+        }
+    }
+    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial);
+}
+
+// FP64 true residual r_r = y_r - (1 + beta_r K) z_r of z_r = x_r + d_r, where
+// the correction d_r (type TD) is added for slots with dact[r] when HAS_D.
+// With HAS_D, z is stored to xout (every slot, so xout holds the new
+// iterate); with WRITE_RES, r is stored to res as type TR.  Partial ||r_r||^2
+// per site block.
+template <int NC, typename RL, typename TD, bool HAS_D, typename TR, bool WRITE_RES>
+__global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u, const cplx<double> *__restrict__ x,
+                                                  const cplx<TD> *__restrict__ d, const int *__restrict__ dact,
+                                                  cplx<double> *__restrict__ xout, const cplx<double> *__restrict__ y,
+                                                  cplx<TR> *__restrict__ res, const cplx<double> *__restrict__ beta, int nf,
+                                                  geom g, lf_map mp, double *__restrict__ partial)
+{
+    lsite p = {0, 0, 0, 0};
+    int64_t t = 0;
+    int chunk;
+    const bool valid = lf_site(mp, g, p, chunk, t);
+    const int64_t s = p.s;
+    const int r0 = chunk * NC;
+    const int64_t fs = 3 * g.vol;
+    bool on[NC];
+    ld_xd<NC, TD> ld;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        on[q] = valid && (r0 + q < nf);
+        const int64_t off = (int64_t)(on[q] ? r0 + q : 0) * fs;
+        ld.x[q] = x + off;
+        ld.d[q] = (HAS_D && on[q] && dact[r0 + q]) ? d + off : nullptr;
+    }
+    double nrm[NC] = {};
+    if (valid) {
+        cplx<double> hop[NC][3];
+        hop_chunk<NC, double>(u, ld, on, p, g, hop);
+        const double dsum = g_dsum(g);
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            if (!on[q]) continue;
+            const int r = r0 + q;
+            const cplx<double> bt = beta[r];
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const int64_t at = (int64_t)r * fs + (int64_t)c * g.vol + s;
+                const cplx<double> v0 = ld(q, (int64_t)c * g.vol + s);
+                const cplx<double> rr = y[at] - (v0 + bt * (dsum * v0 - 0.5 * hop[q][c]));
+                if (HAS_D) xout[at] = v0;
+                if (WRITE_RES) res[at] = {(TR)rr.re, (TR)rr.im};
+                nrm[q] += nrm2(rr);
+            }
+        }
+    }
+    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial);
+}
+
+// One Chebyshev step on the segment [1, 1 + beta_r kmax] of the spectrum of
+// A_r = 1 + beta_r K, fused with the operator application:
+//   x_{k+1} = rho_r (x_k + (b_r - A_r x_k) / c_r) + (1 - rho_r) x_{k-1}
+// c_r = 1 + beta_r kmax / 2 is the segment centre.  x_{k+1} is written where
+// x_{k-1} was read (`xio`).  mode 0: as above; 1: first step (rho = 1, no
+// x_{k-1}); 2: from a zero start, x_k = b / c implicit (read from b) and
+// x_{k-1} = 0; 3: x_{k-1} = b / c implicit.
+template <int NC, typename RL, typename T>
+__global__ __launch_bounds__(SBS) void k_cheb(const cplx<RL> *__restrict__ u, const cplx<T> *__restrict__ xk,
+                                              cplx<T> *__restrict__ xio, const cplx<T> *__restrict__ b,
+                                              const cplx<double> *__restrict__ beta, const cplx<double> *__restrict__ coef,
+                                              const int *__restrict__ active, int nf, geom g, lf_map mp, int mode)
+{
+    lsite p;
+    int chunk;
+    if (!lf_site(mp, g, p, chunk)) return;
+    const int64_t s = p.s;
+    const int r0 = chunk * NC;
+    const int64_t fs = 3 * g.vol;
+    bool on[NC];
+    ld_fields<NC, T, T> ld;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        on[q] = (r0 + q < nf) && active[r0 + q];
+        ld.f[q] = (mode == 2 ? b : xk) + (int64_t)(on[q] ? r0 + q : 0) * fs;
+    }
+    cplx<T> hop[NC][3];
+    hop_chunk<NC, T>(u, ld, on, p, g, hop);
+    const T dsum = T(g_dsum(g));
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        if (!on[q]) continue;
+        const int r = r0 + q;
+        const cplx<T> bt = to<T>(beta[r]);
+        const cplx<T> ic = to<T>(coef[2 * r]); // 1 / c_r
+        const cplx<T> rho = to<T>(coef[2 * r + 1]);
+        const cplx<T> one_m_rho = {T(1) - rho.re, -rho.im};
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            const int64_t at = (int64_t)r * fs + (int64_t)c * g.vol + s;
+            const cplx<T> v0 = ld(q, (int64_t)c * g.vol + s);
+            const cplx<T> kf = dsum * v0 - T(0.5) * hop[q][c];
+            const cplx<T> bs = mode == 2 ? v0 : b[at];
+            const cplx<T> x0 = mode == 2 ? ic * v0 : v0;
+            const cplx<T> ax = mode == 2 ? ic * kf : kf;
+            const cplx<T> nx = rho * (x0 + ic * (bs - (x0 + bt * ax)));
+            if (mode == 1 || mode == 2) xio[at] = nx;
+            else if (mode == 3) xio[at] = nx + one_m_rho * (ic * bs);
+            else xio[at] = nx + one_m_rho * xio[at];
+        }
+    }
+}
+
+// ---------------------------------------------------------------------------
+// vector kernels (grid: VBLOCKS x slots)
+// ---------------------------------------------------------------------------
 
 // partial[r][i][blk] = this block's share of <V_i,r, w_r>, i < nv (FP64 sums)
 template <typename T>
@@ -234,6 +404,20 @@ __global__ void k_fold_c(const cplx<double> *__restrict__ partial, int nblk, int
     out[t] = s;
 }
 
+// out[r] = sum_b partial[r * nblk + b] for the slot r = blockIdx.x if
+// active[r] (active may be null); fixed summation order.
+__global__ __launch_bounds__(256) void k_fold_slots(const double *__restrict__ partial, int64_t nblk,
+                                                    const int *__restrict__ active, double *__restrict__ out)
+{
+    const int r = blockIdx.x;
+    if (active && !active[r]) return;
+    __shared__ double sh[4];
+    double s = 0;
+    for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[(int64_t)r * nblk + b];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[r] = s;
+}
+
 // CGS update w_r -= sum_i h[r][i] V_i,r and partial ||w_r||^2
 template <typename T>
 __global__ __launch_bounds__(VTHREADS) void k_cgs_update(const cplx<T> *__restrict__ basis, int nv, cplx<T> *__restrict__ w,
@@ -254,30 +438,20 @@ __global__ __launch_bounds__(VTHREADS) void k_cgs_update(const cplx<T> *__restri
         }
         const cplx<T> st = {(T)a.re, (T)a.im};
         wr[k] = st;
-        nrm += (double)st.re * st.re + (double)st.im * st.im;
+        nrm += nrm2(st);
     }
     nrm = block_sum(nrm, sh);
     if (threadIdx.x == 0) partial[(int64_t)r * gridDim.x + blockIdx.x] = nrm;
 }
 
-__global__ void k_fold_r(const double *__restrict__ partial, int nblk, int nf, const int *__restrict__ active,
-                         double *__restrict__ out)
-{
-    const int r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= nf || !active[r]) return;
-    double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)r * nblk + b];
-    out[r] = s;
-}
-
 // dst_r = src_r / sqrt(nrm2[r]) (zero norm leaves zeros)
 template <typename T>
 __global__ void k_scale(const cplx<T> *__restrict__ src, cplx<T> *__restrict__ dst, int64_t n,
-                        const double *__restrict__ nrm2, const int *__restrict__ active)
+                        const double *__restrict__ nrm2v, const int *__restrict__ active)
 {
     const int r = blockIdx.y;
     if (!active[r]) return;
-    const double nn = nrm2[r];
+    const double nn = nrm2v[r];
     const T inv = (T)(nn > 0 ? 1.0 / sqrt(nn) : 0.0);
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
         dst[(int64_t)r * n + k] = inv * src[(int64_t)r * n + k];
@@ -316,7 +490,7 @@ __global__ __launch_bounds__(VTHREADS) void k_residual(const cplx<T> *__restrict
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         const cplx<T> a = b[(int64_t)r * n + k] - t[(int64_t)r * n + k];
         res[(int64_t)r * n + k] = a;
-        nrm += (double)a.re * a.re + (double)a.im * a.im;
+        nrm += nrm2(a);
     }
     nrm = block_sum(nrm, sh);
     if (threadIdx.x == 0) partial[(int64_t)r * gridDim.x + blockIdx.x] = nrm;
@@ -330,21 +504,19 @@ __global__ __launch_bounds__(VTHREADS) void k_norm2(const cplx<T> *__restrict__ 
     if (!active[r]) return;
     __shared__ double sh[8];
     double nrm = 0;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        const cplx<T> a = x[(int64_t)r * n + k];
-        nrm += (double)a.re * a.re + (double)a.im * a.im;
-    }
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        nrm += nrm2(x[(int64_t)r * n + k]);
     nrm = block_sum(nrm, sh);
     if (threadIdx.x == 0) partial[(int64_t)r * gridDim.x + blockIdx.x] = nrm;
 }
 
 // dst32_r = src64_r / sqrt(nrm2[r])  (normalised FP32 right-hand side)
 __global__ void k_demote(const cplx<double> *__restrict__ src, cplx<float> *__restrict__ dst, int64_t n,
-                         const double *__restrict__ nrm2, const int *__restrict__ active)
+                         const double *__restrict__ nrm2v, const int *__restrict__ active)
 {
     const int r = blockIdx.y;
     if (!active[r]) return;
-    const double inv = nrm2[r] > 0 ? 1.0 / sqrt(nrm2[r]) : 0.0;
+    const double inv = nrm2v[r] > 0 ? 1.0 / sqrt(nrm2v[r]) : 0.0;
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         const cplx<double> a = src[(int64_t)r * n + k];
         dst[(int64_t)r * n + k] = {(float)(a.re * inv), (float)(a.im * inv)};
@@ -353,11 +525,11 @@ __global__ void k_demote(const cplx<double> *__restrict__ src, cplx<float> *__re
 
 // x64_r += sqrt(nrm2[r]) d32_r
 __global__ void k_promote_add(cplx<double> *__restrict__ x, const cplx<float> *__restrict__ d, int64_t n,
-                              const double *__restrict__ nrm2, const int *__restrict__ active)
+                              const double *__restrict__ nrm2v, const int *__restrict__ active)
 {
     const int r = blockIdx.y;
     if (!active[r]) return;
-    const double sc = sqrt(nrm2[r]);
+    const double sc = sqrt(nrm2v[r]);
     for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
         const cplx<float> a = d[(int64_t)r * n + k];
         x[(int64_t)r * n + k] = x[(int64_t)r * n + k] + sc * cplx<double>{(double)a.re, (double)a.im};
@@ -370,18 +542,15 @@ __global__ void k_links_to_f32(const cplx<double> *__restrict__ u, cplx<float> *
         v[k] = {(float)u[k].re, (float)u[k].im};
 }
 
-// x_r = prev_r + c_r (y_r - prev_r): the first-order guess
-// prev + (alpha - beta) K prev with K prev = (y - prev)/alpha
-__global__ void k_guess(cplx<double> *__restrict__ x, const cplx<double> *__restrict__ prev, const cplx<double> *__restrict__ y,
-                        const cplx<double> *__restrict__ coef, int64_t n, const int *__restrict__ active)
+// psi_{fmap[r]} = x_r
+__global__ void k_scatter(const cplx<double> *__restrict__ x, const int *__restrict__ fmap, cplx<double> *__restrict__ psi,
+                          int64_t n)
 {
     const int r = blockIdx.y;
-    if (!active[r]) return;
-    const cplx<double> c = coef[r];
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t at = (int64_t)r * n + k;
-        x[at] = prev[at] + c * (y[at] - prev[at]);
-    }
+    cplx<double> *dst = psi + (int64_t)fmap[r] * n;
+    const cplx<double> *src = x + (int64_t)r * n;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        dst[k] = src[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -390,20 +559,26 @@ __global__ void k_guess(cplx<double> *__restrict__ x, const cplx<double> *__rest
 struct workspace {
     int64_t vol = 0;
     int m = 0;
+    int64_t nsb = 0;                   // site blocks of the stencil launches
     void *basis = nullptr;             // (m+1) x NSLOTS x 3V complex double (or float)
-    cplx<double> *y = nullptr, *x = nullptr, *t = nullptr, *res = nullptr, *scr = nullptr;
+    cplx<double> *y = nullptr, *x = nullptr, *t = nullptr, *res = nullptr;
     cplx<float> *y32 = nullptr, *x32 = nullptr, *t32 = nullptr, *res32 = nullptr;
     cplx<float> *links32 = nullptr;    // FP32 copy of the links for the inner solves
-    cplx<double> *beta = nullptr, *coef = nullptr, *hdev = nullptr, *ydev = nullptr, *cpart = nullptr;
+    cplx<double> *beta = nullptr, *cpar = nullptr, *hdev = nullptr, *ydev = nullptr, *cpart = nullptr;
     cplx<double> *cheb = nullptr;      // Chebyshev coefficients [iteration][slot][2]
-    double *rpart = nullptr, *nrm = nullptr;
-    int *active = nullptr, *nused = nullptr;
+    double *rpart = nullptr, *spart = nullptr, *nrm = nullptr;
+    int *active = nullptr, *nused = nullptr, *fmap = nullptr, *dact = nullptr;
     cplx<double> *hbuf = nullptr;      // pinned host
-    double *nbuf = nullptr;            // pinned host (64 doubles)
+    double *nbuf = nullptr;            // pinned host (NRM doubles)
+
+    // nrm layout: [0, 32) working norms, [32, 64) ||y||^2, [64, 96) saved
+    // refinement ||r||^2, [96, 128) GMRES restart norms
+    static constexpr int NRM = 128;
 
     void release()
     {
-        void *dev[] = {basis, y, x, t, res, scr, y32, x32, t32, res32, links32, beta, coef, hdev, ydev, cpart, rpart, nrm, active, nused, cheb};
+        void *dev[] = {basis, y, x, t, res, y32, x32, t32, res32, links32, beta, cpar, hdev, ydev, cpart,
+                       rpart, spart, nrm, active, nused, fmap, dact, cheb};
         for (void *q : dev)
             if (q) (void)hipFree(q);
         if (hbuf) (void)hipHostFree(hbuf);
@@ -417,20 +592,23 @@ struct workspace {
         release();
         const size_t f64 = (size_t)3 * (size_t)v * sizeof(cplx<double>) * NSLOTS;
         const size_t f32 = f64 / 2;
+        const int64_t sb = std::max(make_lf_map(lat().g, SBS, 8, 1, 1).nsb, make_lf_map(lat().g, SBS, 8, 0, 1).nsb);
         auto dm = [](void **p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
         bool ok = dm(&basis, f64 * (size_t)(mm + 1)) && dm((void **)&y, f64) && dm((void **)&x, f64) &&
-                  dm((void **)&t, f64) && dm((void **)&res, f64) && dm((void **)&scr, f64) && dm((void **)&y32, f32) &&
+                  dm((void **)&t, f64) && dm((void **)&res, f64) && dm((void **)&y32, f32) &&
                   dm((void **)&x32, f32) && dm((void **)&t32, f32) && dm((void **)&res32, f32) &&
                   dm((void **)&links32, (size_t)27 * (size_t)v * sizeof(cplx<float>)) &&
-                  dm((void **)&beta, sizeof(cplx<double>) * NSLOTS) && dm((void **)&coef, sizeof(cplx<double>) * NSLOTS) &&
+                  dm((void **)&beta, sizeof(cplx<double>) * NSLOTS) && dm((void **)&cpar, sizeof(cplx<double>) * 2 * NSLOTS) &&
                   dm((void **)&cheb, sizeof(cplx<double>) * 2 * NSLOTS * CHEB_MAX) &&
                   dm((void **)&hdev, sizeof(cplx<double>) * NSLOTS * (mm + 1)) &&
                   dm((void **)&ydev, sizeof(cplx<double>) * NSLOTS * (mm + 1)) &&
                   dm((void **)&cpart, sizeof(cplx<double>) * NSLOTS * (mm + 1) * VBLOCKS) &&
-                  dm((void **)&rpart, sizeof(double) * NSLOTS * VBLOCKS * 8) && dm((void **)&nrm, sizeof(double) * 64) &&
+                  dm((void **)&rpart, sizeof(double) * NSLOTS * VBLOCKS * 8) &&
+                  dm((void **)&spart, sizeof(double) * NSLOTS * (size_t)sb) && dm((void **)&nrm, sizeof(double) * NRM) &&
                   dm((void **)&active, sizeof(int) * NSLOTS) && dm((void **)&nused, sizeof(int) * NSLOTS) &&
+                  dm((void **)&fmap, sizeof(int) * NSLOTS) && dm((void **)&dact, sizeof(int) * NSLOTS) &&
                   hipHostMalloc((void **)&hbuf, sizeof(cplx<double>) * NSLOTS * (mm + 1), hipHostMallocDefault) == hipSuccess &&
-                  hipHostMalloc((void **)&nbuf, sizeof(double) * 64, hipHostMallocDefault) == hipSuccess;
+                  hipHostMalloc((void **)&nbuf, sizeof(double) * NRM, hipHostMallocDefault) == hipSuccess;
         if (!ok) {
             (void)hipGetLastError();
             release();
@@ -438,6 +616,7 @@ struct workspace {
         }
         vol = v;
         m = mm;
+        nsb = sb;
         return NRQ_OK;
     }
 };
@@ -445,77 +624,110 @@ static workspace g_ws;
 
 int krylov_reserve(int64_t vol, int m) { return g_ws.reserve(vol, m); }
 void krylov_release() { g_ws.release(); }
-cplx<double> *krylov_rhs() { return g_ws.y; }
-cplx<double> *krylov_solution() { return g_ws.x; }
-cplx<double> *krylov_scratch() { return g_ws.scr; }
 double *krylov_reduce_scratch() { return g_ws.rpart; }
 double *krylov_host_scratch() { return g_ws.nbuf; }
 
-static int upload_active(const std::vector<int> &act)
+static int upload_ints(int *dev, const std::vector<int> &v)
 {
-    NRQ_HIP(hipMemcpyAsync(g_ws.active, act.data(), sizeof(int) * act.size(), hipMemcpyHostToDevice, stream()));
+    NRQ_HIP(hipMemcpyAsync(dev, v.data(), sizeof(int) * v.size(), hipMemcpyHostToDevice, stream()));
+    return NRQ_OK;
+}
+static int upload_active(const std::vector<int> &act) { return upload_ints(g_ws.active, act); }
+
+// Launch shape of the stencil kernels: slots per thread (NC, sharing each
+// link load) for FP64 and for FP32 arithmetic, and the site-block mapping of
+// make_lf_map.  NRQ_CN_SHAPE="nc64,nc32,mode,xcd" overrides the defaults.
+struct cn_shape {
+    int nc64 = 2, nc32 = 4, mode = 1, xcd = 8;
+};
+static cn_shape read_shape()
+{
+    cn_shape sh;
+    if (const char *e = getenv("NRQ_CN_SHAPE")) {
+        int a = 0, b = 0, c = 0, d = 0;
+        if (sscanf(e, "%d,%d,%d,%d", &a, &b, &c, &d) == 4) {
+            auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
+            if (ok(a)) sh.nc64 = a;
+            if (ok(b)) sh.nc32 = b;
+            if (c == 0 || c == 1) sh.mode = c;
+            if (d == 1 || d == 8) sh.xcd = d;
+        }
+    }
+    return sh;
+}
+static const cn_shape &shape()
+{
+    static const cn_shape sh = read_shape();
+    return sh;
+}
+template <typename T> static int nc_of() { return sizeof(T) == 8 ? shape().nc64 : shape().nc32; }
+
+// stencil launch map for nf slots, nc per thread
+static lf_map cn_map(int nf, int nc)
+{
+    return make_lf_map(lat().g, SBS, shape().xcd, shape().mode, (nf + nc - 1) / nc);
+}
+
+// Runs `body` with the compile-time chunk width NCV equal to nc (1, 2 or 4).
+#define NC_DISPATCH(nc, body)                                                                                            \
+    do {                                                                                                                 \
+        if ((nc) == 1) {                                                                                                 \
+            constexpr int NCV = 1;                                                                                       \
+            body;                                                                                                        \
+        } else if ((nc) == 2) {                                                                                          \
+            constexpr int NCV = 2;                                                                                       \
+            body;                                                                                                        \
+        } else {                                                                                                         \
+            constexpr int NCV = 4;                                                                                       \
+            body;                                                                                                        \
+        }                                                                                                                \
+    } while (0)
+
+static unsigned grid_of(const lf_map &mp) { return (unsigned)(mp.ngroup * mp.nsb); }
+
+// out[r] = sum of the nblk partials of slot r (r < nf; active may be null)
+static int fold_slots(const double *partial, int64_t nblk, int nf, const int *active, double *out)
+{
+    hipLaunchKernelGGL(k_fold_slots, dim3(nf), dim3(256), 0, stream(), partial, nblk, active, out);
+    NRQ_HIP(hipGetLastError());
     return NRQ_OK;
 }
 
-static int upload_beta(const cplx<double> *beta, int nf)
+// g_ws.nrm to the pinned host buffer g_ws.nbuf; waits for the stream
+static int nrm_to_host()
 {
-    NRQ_HIP(hipMemcpyAsync(g_ws.beta, beta, sizeof(cplx<double>) * nf, hipMemcpyHostToDevice, stream()));
+    NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * workspace::NRM, hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
     return NRQ_OK;
 }
 
 template <typename RL, typename T>
-static int apply_t(const cplx<RL> *u, const cplx<T> *in, cplx<T> *out, int nf, const int *map = nullptr,
-                   const cplx<T> *resid_b = nullptr)
+static int apply_t(const cplx<RL> *u, const cplx<T> *in, cplx<T> *out, int nf)
 {
-    const geom g = lat().g;
-    const lf_map mp = make_lf_map(g, 256, 8, 1, (nf + RCHUNK - 1) / RCHUNK);
-    if (resid_b)
-        hipLaunchKernelGGL((k_stencil<RL, T, true>), dim3((unsigned)(mp.ngroup * mp.nsb)), dim3(256), 0, stream(), u, in,
-                           out, g_ws.beta, g_ws.active, nf, g, mp, map, resid_b);
-    else
-        hipLaunchKernelGGL((k_stencil<RL, T, false>), dim3((unsigned)(mp.ngroup * mp.nsb)), dim3(256), 0, stream(), u, in,
-                           out, g_ws.beta, g_ws.active, nf, g, mp, map, (const cplx<T> *)nullptr);
+    const int nc = nc_of<T>();
+    const lf_map mp = cn_map(nf, nc);
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_stencil<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, in, out,
+                                       g_ws.beta, g_ws.active, nf, lat().g, mp));
     NRQ_HIP(hipGetLastError());
     return NRQ_OK;
-}
-
-// FP64 operator with the configuration's own links (FP64 or FP32 storage)
-static int apply64(const nrq_gauge *gf, const cplx<double> *in, cplx<double> *out, int nf)
-{
-    if (gf->precision == 64) return apply_t<double, double>((const cplx<double> *)gauge_links(gf), in, out, nf);
-    return apply_t<float, double>((const cplx<float> *)gauge_links(gf), in, out, nf);
-}
-
-template <typename T> static int norms2_t(const cplx<T> *v, int nf, double *out)
-{
-    const int64_t n = 3 * g_ws.vol;
-    hipLaunchKernelGGL(k_norm2<T>, dim3(VBLOCKS, nf), dim3(VTHREADS), 0, stream(), v, n, g_ws.active, g_ws.rpart);
-    hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, nf, g_ws.active, g_ws.nrm);
-    NRQ_HIP(hipGetLastError());
-    NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * nf, hipMemcpyDeviceToHost, stream()));
-    NRQ_HIP(hipStreamSynchronize(stream()));
-    memcpy(out, g_ws.nbuf, sizeof(double) * nf);
-    return NRQ_OK;
-}
-
-int krylov_apply(const nrq_gauge *gf, const cplx<double> *x, cplx<double> *y, int nf, const cplx<double> *beta)
-{
-    std::vector<int> all(nf, 1);
-    if (int e = upload_active(all)) return e;
-    if (int e = upload_beta(beta, nf)) return e;
-    return apply64(gf, x, y, nf);
 }
 
 int krylov_norms2(const cplx<double> *x, int nf, double *out)
 {
     std::vector<int> all(nf, 1);
     if (int e = upload_active(all)) return e;
-    return norms2_t<double>(x, nf, out);
+    const int64_t n = 3 * g_ws.vol;
+    hipLaunchKernelGGL(k_norm2<double>, dim3(VBLOCKS, nf), dim3(VTHREADS), 0, stream(), x, n, g_ws.active, g_ws.rpart);
+    NRQ_HIP(hipGetLastError());
+    if (int e = fold_slots(g_ws.rpart, VBLOCKS, nf, g_ws.active, g_ws.nrm)) return e;
+    if (int e = nrm_to_host()) return e;
+    memcpy(out, g_ws.nbuf, sizeof(double) * nf);
+    return NRQ_OK;
 }
 
 // ---------------------------------------------------------------------------
 // GMRES(m) core: solves (1 + beta_r K) x_r = b_r for the slots with act[r],
-// from the current x (zero when zero_guess), until the T-precision true
+// from the current x (x = 0 when zero_guess), until the T-precision true
 // residual satisfies ||b - A x|| <= tol ||b|| or its[r] reaches maxit.
 // rel_out[r] receives the last true relative residual.  beta must already be
 // on the device.
@@ -530,16 +742,16 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
     const size_t fbytes = sizeof(cplx<T>) * (size_t)n;
     std::vector<double> rn(R, 0.0);
     if (int e = upload_active(act)) return e;
+    const cplx<T> *r0 = res; // residual the next cycle starts from
     if (zero_guess) {
         NRQ_HIP(hipMemsetAsync(x, 0, fbytes * R, stream()));
-        NRQ_HIP(hipMemcpyAsync(res, b, fbytes * R, hipMemcpyDeviceToDevice, stream()));
+        r0 = b;
         for (int r = 0; r < R; ++r) rn[r] = bnorm[r];
     } else {
         if (int e = apply_t<RL, T>(links, x, t, R)) return e;
         hipLaunchKernelGGL(k_residual<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), b, t, res, n, g_ws.active, g_ws.rpart);
-        hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm);
-        NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToHost, stream()));
-        NRQ_HIP(hipStreamSynchronize(stream()));
+        if (int e = fold_slots(g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm)) return e;
+        if (int e = nrm_to_host()) return e;
         for (int r = 0; r < R; ++r) rn[r] = act[r] ? std::sqrt(std::max(g_ws.nbuf[r], 0.0)) : 0.0;
     }
     for (int r = 0; r < R; ++r) {
@@ -551,20 +763,22 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
         SN(R, std::vector<zc>(m)), Y(R, std::vector<zc>(m));
     std::vector<std::vector<double>> CS(R, std::vector<double>(m));
     std::vector<int> jused(R, 0), cycle(R);
+    double *restart_n2 = g_ws.nrm + 96;
     while (true) {
         bool any = false;
         for (int r = 0; r < R; ++r) any = any || act[r];
         if (!any) break;
         ++cycles;
+        std::vector<double> rn2(R);
         for (int r = 0; r < R; ++r) {
-            g_ws.nbuf[r] = act[r] ? rn[r] * rn[r] : 0.0;
+            rn2[r] = act[r] ? rn[r] * rn[r] : 0.0;
             std::fill(G[r].begin(), G[r].end(), zc(0));
             G[r][0] = rn[r];
             jused[r] = 0;
         }
-        NRQ_HIP(hipMemcpyAsync(g_ws.nrm, g_ws.nbuf, sizeof(double) * R, hipMemcpyHostToDevice, stream()));
+        NRQ_HIP(hipMemcpyAsync(restart_n2, rn2.data(), sizeof(double) * R, hipMemcpyHostToDevice, stream()));
         if (int e = upload_active(act)) return e;
-        hipLaunchKernelGGL(k_scale<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<T> *)res, basis, n, g_ws.nrm,
+        hipLaunchKernelGGL(k_scale<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), r0, basis, n, (const double *)restart_n2,
                            g_ws.active);
         cycle = act;
         for (int j = 0; j < m; ++j) {
@@ -582,13 +796,12 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
                                g_ws.hdev);
             hipLaunchKernelGGL(k_cgs_update<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<T> *)basis, nv, w, n, R,
                                g_ws.active, g_ws.hdev, g_ws.rpart);
-            hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm);
-            hipLaunchKernelGGL(k_scale<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<T> *)w, w, n, g_ws.nrm,
-                               g_ws.active);
+            if (int e = fold_slots(g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm)) return e;
+            hipLaunchKernelGGL(k_scale<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<T> *)w, w, n,
+                               (const double *)g_ws.nrm, g_ws.active);
             NRQ_HIP(hipGetLastError());
             NRQ_HIP(hipMemcpyAsync(g_ws.hbuf, g_ws.hdev, sizeof(cplx<double>) * nv * R, hipMemcpyDeviceToHost, stream()));
-            NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToHost, stream()));
-            NRQ_HIP(hipStreamSynchronize(stream()));
+            if (int e = nrm_to_host()) return e;
             for (int r = 0; r < R; ++r) {
                 if (!cycle[r]) continue;
                 ++its[r];
@@ -634,21 +847,20 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
         }
         if (int e = upload_active(act)) return e;
         NRQ_HIP(hipMemcpyAsync(g_ws.ydev, g_ws.hbuf, sizeof(cplx<double>) * R * (m + 1), hipMemcpyHostToDevice, stream()));
-        NRQ_HIP(hipMemcpyAsync(g_ws.nused, nu.data(), sizeof(int) * R, hipMemcpyHostToDevice, stream()));
+        if (int e = upload_ints(g_ws.nused, nu)) return e;
         hipLaunchKernelGGL(k_xupdate<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), x, (const cplx<T> *)basis, R, n,
                            g_ws.ydev, m + 1, g_ws.nused, g_ws.active);
         // true residual in precision T
         if (int e = apply_t<RL, T>(links, x, t, R)) return e;
         hipLaunchKernelGGL(k_residual<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), b, (const cplx<T> *)t, res, n,
                            g_ws.active, g_ws.rpart);
-        hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm);
-        NRQ_HIP(hipGetLastError());
-        NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToHost, stream()));
-        NRQ_HIP(hipStreamSynchronize(stream()));
+        r0 = res;
+        if (int e = fold_slots(g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm)) return e;
+        if (int e = nrm_to_host()) return e;
         for (int r = 0; r < R; ++r) {
             if (!act[r]) continue;
             rn[r] = std::sqrt(std::max(g_ws.nbuf[r], 0.0));
-            rel_out[r] = rn[r] / bnorm[r];
+            rel_out[r] = bnorm[r] > 0 ? rn[r] / bnorm[r] : 0.0;
             if (rn[r] <= tol * bnorm[r] || its[r] >= maxit) act[r] = 0;
         }
     }
@@ -662,41 +874,54 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
 // 1 + beta_r kmax.  With centre c and half-width d of that segment and
 // sigma = c/d, the residual after k steps is bounded by ||r0|| / |T_k(sigma)|.
 // ---------------------------------------------------------------------------
+typedef std::complex<double> zc;
+
 static double cheb_kmax()
 {
     const geom &g = lat().g;
     return 2.0 / (g.a[1] * g.a[1]) + 2.0 / (g.a[2] * g.a[2]) + 2.0 / (g.a[3] * g.a[3]);
 }
 
-// Smallest k <= CHEB_MAX with max_r 1/|T_k(sigma_r)| <= target.
-static int cheb_steps(const std::vector<std::complex<double>> &sig, double target)
+// 1 / |T_k(sigma)|
+static double cheb_bound(zc sg, int k)
 {
-    for (int k = 1; k < CHEB_MAX; ++k) {
-        double worst = 0;
-        for (const auto &sg : sig) {
-            std::complex<double> t0 = 1.0, t1 = sg;
-            for (int j = 1; j < k; ++j) {
-                const std::complex<double> t2 = 2.0 * sg * t1 - t0;
-                t0 = t1;
-                t1 = t2;
-            }
-            worst = std::max(worst, 1.0 / std::abs(t1));
-        }
-        if (worst <= target) return k;
+    zc t0 = 1.0, t1 = sg;
+    for (int j = 1; j < k; ++j) {
+        const zc t2 = 2.0 * sg * t1 - t0;
+        t0 = t1;
+        t1 = t2;
     }
+    return 1.0 / std::abs(t1);
+}
+
+// Smallest k < CHEB_MAX with 1/|T_k(sigma)| <= target (CHEB_MAX if none).
+static int cheb_steps(zc sg, double target)
+{
+    for (int k = 1; k < CHEB_MAX; ++k)
+        if (cheb_bound(sg, k) <= target) return k;
     return CHEB_MAX;
 }
 
-// k Chebyshev steps on A x = b from x (result back in x); xalt is scratch.
-template <typename RL, typename T>
-static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, cplx<T> *xalt,
-                    const std::vector<std::complex<double>> &beta, int k, double kmax, const std::vector<int> &act)
+// Bound on ||y - A x0|| / ||y|| for the initial guess.  With the first-order
+// guess x0 = psi + (alpha - beta) K psi the residual is
+// -beta (alpha - beta) K^2 psi, and ||y|| >= min_{0<=l<=kmax} |1 + alpha l| ||psi||.
+static double guess_bound(zc a, zc b, bool guess, double kmax)
 {
-    std::vector<cplx<double>> tab((size_t)2 * NSLOTS * k);
+    if (!guess) return 1.0;
+    const double lam = a.real() < 0 ? std::min(kmax, -a.real() / std::norm(a)) : 0.0;
+    const double mn = std::abs(1.0 + a * lam);
+    if (!(mn > 0)) return 1.0;
+    return std::min(1.0, 1.05 * std::abs(b) * std::abs(a - b) * kmax * kmax / mn);
+}
+
+// Coefficient table for k steps; tab[j] = {1/c_r, rho_r of step j}.
+static int cheb_table(int R, const std::vector<zc> &beta, int k, double kmax)
+{
+    std::vector<cplx<double>> tab((size_t)2 * NSLOTS * k, cplx<double>{0.0, 0.0});
     for (int r = 0; r < R; ++r) {
-        const std::complex<double> c = 1.0 + beta[r] * (kmax / 2), d = beta[r] * (kmax / 2);
-        const std::complex<double> sg = c / d, ic = 1.0 / c;
-        std::complex<double> rho = 1.0;
+        const zc c = 1.0 + beta[r] * (kmax / 2), d = beta[r] * (kmax / 2);
+        const zc sg = c / d, ic = 1.0 / c;
+        zc rho = 1.0;
         for (int j = 0; j < k; ++j) {
             if (j == 1) rho = 1.0 / (1.0 - 1.0 / (2.0 * sg * sg));
             else if (j > 1) rho = 1.0 / (1.0 - rho / (4.0 * sg * sg));
@@ -705,245 +930,329 @@ static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, 
         }
     }
     NRQ_HIP(hipMemcpyAsync(g_ws.cheb, tab.data(), sizeof(cplx<double>) * tab.size(), hipMemcpyHostToDevice, stream()));
+    return NRQ_OK;
+}
+
+// k Chebyshev steps on A x = b for the active slots.  zero_start: x starts at
+// 0 and steps 0 and 1 are folded into the first launch (k >= 2); otherwise
+// from the iterate in `x`.  x and xalt alternate; *out receives the buffer
+// holding x_k.
+template <typename RL, typename T>
+static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, cplx<T> *xalt, bool zero_start,
+                    const std::vector<zc> &beta, int k, double kmax, cplx<T> **out)
+{
+    if (zero_start && k < 2) k = 2;
+    if (k > CHEB_MAX) k = CHEB_MAX;
+    if (int e = cheb_table(R, beta, k, kmax)) return e;
     const geom g = lat().g;
-    const lf_map mp = make_lf_map(g, 256, 8, 1, (R + RCHUNK - 1) / RCHUNK);
+    const int nc = nc_of<T>();
+    const lf_map mp = cn_map(R, nc);
     cplx<T> *cur = x, *other = xalt;
-    for (int j = 0; j < k; ++j) {
-        hipLaunchKernelGGL((k_cheb<RL, T>), dim3((unsigned)(mp.ngroup * mp.nsb)), dim3(256), 0, stream(), links,
-                           (const cplx<T> *)cur, other, b, g_ws.beta, g_ws.cheb + (size_t)2 * NSLOTS * j, g_ws.active, R, g,
-                           mp, j == 0 ? 1 : 0);
-        std::swap(cur, other);
+    for (int j = zero_start ? 1 : 0; j < k; ++j) {
+        const int mode = zero_start ? (j == 1 ? 2 : (j == 2 ? 3 : 0)) : (j == 0 ? 1 : 0);
+        // mode 2 reads b only and writes cur; the other modes read cur and
+        // write other
+        cplx<T> *dst = mode == 2 ? cur : other;
+        const cplx<double> *cf = g_ws.cheb + (size_t)2 * NSLOTS * j;
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cheb<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), links,
+                                           (const cplx<T> *)cur, dst, b, g_ws.beta, cf, g_ws.active, R, g, mp, mode));
+        if (mode != 2) std::swap(cur, other);
     }
     NRQ_HIP(hipGetLastError());
-    if (cur != x) {
-        // odd step count: the result is in xalt; only active slots were iterated
-        const size_t fb = sizeof(cplx<T>) * (size_t)3 * g.vol;
+    *out = cur;
+    return NRQ_OK;
+}
+
+// FP64 true residual of x (+ d for the slots with dact) with its norms into
+// g_ws.nrm[0, R); with d, the updated iterate goes to xout; res32 (may be
+// null) receives the residual in FP32.
+template <typename RL>
+static int cn_resid(const cplx<RL> *u, int R, const cplx<double> *x, const cplx<float> *d, cplx<double> *xout,
+                    cplx<float> *res32)
+{
+    const int nc = shape().nc64;
+    const lf_map mp = cn_map(R, nc);
+    const geom g = lat().g;
+    const cplx<double> *y = g_ws.y, *bt = g_ws.beta;
+    const int *da = g_ws.dact;
+    double *sp = g_ws.spart;
+    if (d)
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, true, float, true>), dim3(grid_of(mp)), dim3(SBS), 0,
+                                           stream(), u, x, d, da, xout, y, res32, bt, R, g, mp, sp));
+    else if (res32)
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, false, float, true>), dim3(grid_of(mp)), dim3(SBS), 0,
+                                           stream(), u, x, (const cplx<float> *)nullptr, da, (cplx<double> *)nullptr, y, res32,
+                                           bt, R, g, mp, sp));
+    else
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, false, float, false>), dim3(grid_of(mp)), dim3(SBS), 0,
+                                           stream(), u, x, (const cplx<float> *)nullptr, da, (cplx<double> *)nullptr, y,
+                                           (cplx<float> *)nullptr, bt, R, g, mp, sp));
+    NRQ_HIP(hipGetLastError());
+    return fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm);
+}
+
+// Per-slot convergence check of the residual norms in g_ws.nbuf; returns
+// true if a slot is still active.  need[r] receives the residual reduction
+// still needed.
+static bool check_slots(int R, const std::vector<double> &bn, double tol, std::vector<int> &act, std::vector<double> &rel,
+                        std::vector<double> &need)
+{
+    bool any = false;
+    for (int r = 0; r < R; ++r) {
+        if (!act[r]) continue;
+        const double rr = std::sqrt(std::max(g_ws.nbuf[r], 0.0));
+        rel[r] = bn[r] > 0 ? rr / bn[r] : 0.0;
+        if (rr <= tol * bn[r]) {
+            act[r] = 0;
+            continue;
+        }
+        any = true;
+        need[r] = 0.3 * tol * bn[r] / rr;
+    }
+    return any;
+}
+
+// Chebyshev solve from the guess in g_ws.x.  *xfinal receives the buffer
+// holding the iterate.  Slots still active on return go on with GMRES.
+template <typename RL>
+static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, const std::vector<zc> &beta,
+                      std::vector<double> &bn, const krylov_params &p, std::vector<int> &act, std::vector<double> &rel,
+                      std::vector<int64_t> &its, int64_t &cycles, cplx<double> **xfinal)
+{
+    const double kmax = cheb_kmax();
+    std::vector<zc> sig(R);
+    std::vector<double> b0(R);
+    for (int r = 0; r < R; ++r) {
+        const zc d = beta[r] * (kmax / 2);
+        sig[r] = std::abs(d) > 0 ? (1.0 + d) / d : zc(1e30, 0.0);
+        b0[r] = guess_bound(alpha[r], beta[r], p.extrapolate, kmax);
+    }
+    std::vector<double> need(R, 1.0);
+    auto steps_for = [&](double floor_) {
+        int k = 1;
         for (int r = 0; r < R; ++r)
-            if (act[r]) NRQ_HIP(hipMemcpyAsync((char *)x + fb * r, (const char *)cur + fb * r, fb, hipMemcpyDeviceToDevice, stream()));
+            if (act[r]) k = std::max(k, cheb_steps(sig[r], std::max(need[r], floor_)));
+        return k;
+    };
+    auto bn_from_host = [&]() {
+        for (int r = 0; r < R; ++r) bn[r] = std::sqrt(std::max(g_ws.nbuf[32 + r], 0.0));
+    };
+    if (p.kind == KRYLOV_CHEB_FP64) {
+        // the first round takes enough steps for the a priori guess bound
+        for (int r = 0; r < R; ++r) need[r] = 0.3 * p.tol / b0[r];
+        cplx<double> *x = g_ws.x;
+        for (int round = 0; round < 8; ++round) {
+            const int k = steps_for(0.0);
+            ++cycles;
+            cplx<double> *xk = nullptr;
+            if (int e = cheb_run<RL, double>(u, R, g_ws.y, x, x == g_ws.x ? g_ws.t : g_ws.x, false, beta, k, kmax, &xk)) return e;
+            x = xk;
+            for (int r = 0; r < R; ++r) its[r] += act[r] ? k : 0;
+            if (int e = cn_resid<RL>(u, R, x, nullptr, nullptr, nullptr)) return e;
+            if (int e = nrm_to_host()) return e;
+            if (round == 0) bn_from_host();
+            if (!check_slots(R, bn, p.tol, act, rel, need)) break;
+            if (int e = upload_active(act)) return e;
+        }
+        *xfinal = x;
+        return NRQ_OK;
+    }
+    // mixed: FP64 residual -> FP32 Chebyshev correction from zero -> x += d
+    const cplx<float> *l32;
+    if (sizeof(RL) == 8) {
+        hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)u, g_ws.links32, 27 * g_ws.vol);
+        NRQ_HIP(hipGetLastError());
+        l32 = g_ws.links32;
+    } else {
+        l32 = (const cplx<float> *)u;
+    }
+    cplx<double> *xcur = g_ws.x, *xalt = g_ws.t;
+    if (int e = cn_resid<RL>(u, R, xcur, nullptr, nullptr, g_ws.y32)) return e;
+    // corrections planned from the a priori bound, without synchronisation
+    std::vector<double> bnd = b0;
+    for (int plan = 0; plan < 3; ++plan) {
+        bool more = false;
+        for (int r = 0; r < R; ++r) {
+            need[r] = bnd[r] > 0.3 * p.tol ? 0.3 * p.tol / bnd[r] : 1.0;
+            more = more || bnd[r] > 0.3 * p.tol;
+        }
+        if (!more) break;
+        const int k = std::max(2, steps_for(3e-7));
+        ++cycles;
+        cplx<float> *d = nullptr;
+        if (int e = cheb_run<float, float>(l32, R, g_ws.y32, g_ws.x32, g_ws.t32, true, beta, k, kmax, &d)) return e;
+        if (int e = cn_resid<RL>(u, R, xcur, d, xalt, g_ws.y32)) return e;
+        std::swap(xcur, xalt);
+        for (int r = 0; r < R; ++r) {
+            its[r] += k;
+            bnd[r] *= cheb_bound(sig[r], k) + FP32_CORR_ERR;
+        }
+    }
+    if (int e = nrm_to_host()) return e;
+    bn_from_host();
+    // measured corrections until converged
+    for (int round = 0; round < 8; ++round) {
+        if (!check_slots(R, bn, p.tol, act, rel, need)) break;
+        if (int e = upload_active(act)) return e;
+        if (int e = upload_ints(g_ws.dact, act)) return e;
+        const int k = std::max(2, steps_for(3e-7));
+        ++cycles;
+        cplx<float> *d = nullptr;
+        if (int e = cheb_run<float, float>(l32, R, g_ws.y32, g_ws.x32, g_ws.t32, true, beta, k, kmax, &d)) return e;
+        if (int e = cn_resid<RL>(u, R, xcur, d, xalt, g_ws.y32)) return e;
+        std::swap(xcur, xalt);
+        for (int r = 0; r < R; ++r) its[r] += act[r] ? k : 0;
+        if (int e = nrm_to_host()) return e;
+    }
+    *xfinal = xcur;
+    return NRQ_OK;
+}
+
+// FP64 iterative refinement around FP32 GMRES corrections, from the guess in
+// g_ws.x.
+template <typename RL>
+static int gmres_mixed(const cplx<RL> *u, int R, std::vector<double> &bn, const krylov_params &p, std::vector<int> &act,
+                       std::vector<double> &rel, std::vector<int64_t> &its, int64_t &cycles)
+{
+    const int64_t n = 3 * g_ws.vol;
+    const cplx<float> *l32;
+    if (sizeof(RL) == 8) {
+        hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)u, g_ws.links32, 27 * g_ws.vol);
+        NRQ_HIP(hipGetLastError());
+        l32 = g_ws.links32;
+    } else {
+        l32 = (const cplx<float> *)u;
+    }
+    std::vector<double> one(R, 1.0);
+    const int nc = shape().nc64;
+    const lf_map mp = cn_map(R, nc);
+    for (int outer = 0; outer < 40; ++outer) {
+        // FP64 true residual r = y - A x
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, false, double, true>), dim3(grid_of(mp)), dim3(SBS), 0,
+                                           stream(), u, (const cplx<double> *)g_ws.x, (const cplx<float> *)nullptr, g_ws.dact,
+                                           (cplx<double> *)nullptr, g_ws.y, g_ws.res, g_ws.beta, R, lat().g, mp, g_ws.spart));
+        NRQ_HIP(hipGetLastError());
+        if (int e = fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm)) return e;
+        if (int e = nrm_to_host()) return e;
+        bool any = false;
+        for (int r = 0; r < R; ++r) {
+            if (!act[r]) continue;
+            const double rr = std::sqrt(std::max(g_ws.nbuf[r], 0.0));
+            rel[r] = bn[r] > 0 ? rr / bn[r] : 0.0;
+            if (rr <= p.tol * bn[r] || its[r] >= p.maxit) act[r] = 0;
+            any = any || act[r];
+        }
+        if (!any) break;
+        ++cycles;
+        // ||r||^2 of this refinement step at nrm[64..): the inner solve reuses nrm[0..R)
+        double *rnorm2 = g_ws.nrm + 64;
+        NRQ_HIP(hipMemcpyAsync(rnorm2, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToDevice, stream()));
+        // normalised FP32 correction problem A d = r / ||r||
+        if (int e = upload_active(act)) return e;
+        hipLaunchKernelGGL(k_demote, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<double> *)g_ws.res, g_ws.y32, n,
+                           (const double *)rnorm2, g_ws.active);
+        std::vector<double> inner_rel(R, 0.0);
+        int64_t inner_cycles = 0;
+        if (int e = gmres_core<float, float>(l32, R, g_ws.y32, g_ws.x32, (cplx<float> *)g_ws.basis, g_ws.t32, g_ws.res32, true,
+                                             one.data(), act, std::max(p.inner_tol, std::max(p.tol / 1e-3, 2e-7)), p.restart,
+                                             p.maxit, its, inner_rel.data(), inner_cycles))
+            return e;
+        // gmres_core left the slot masks of the inner solve on the device;
+        // the refinement updates every slot that is still active here
+        if (int e = upload_active(act)) return e;
+        hipLaunchKernelGGL(k_promote_add, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.x, (const cplx<float> *)g_ws.x32, n,
+                           (const double *)rnorm2, g_ws.active);
+        NRQ_HIP(hipGetLastError());
     }
     return NRQ_OK;
 }
 
-// FP64 true residual res = y - A x for the active slots; ||res||^2 to host.
-static int residual64(const nrq_gauge *gf, int R, double *rn2)
-{
-    if (gf->precision == 64) {
-        if (int e = apply_t<double, double>((const cplx<double> *)gauge_links(gf), g_ws.x, g_ws.res, R, nullptr, g_ws.y)) return e;
-    } else {
-        if (int e = apply_t<float, double>((const cplx<float> *)gauge_links(gf), g_ws.x, g_ws.res, R, nullptr, g_ws.y)) return e;
-    }
-    return norms2_t<double>(g_ws.res, R, rn2);
-}
-
-static int cheb_solve(const nrq_gauge *gf, int R, const std::vector<std::complex<double>> &beta, const double *bn,
-                      const krylov_params &p, std::vector<int> &act, std::vector<double> &rel, std::vector<int64_t> &its,
-                      int64_t &cycles)
+template <typename RL>
+static int cn_step_t(const cplx<RL> *u, cplx<double> *psi, int R, const std::vector<zc> &alpha, const std::vector<zc> &beta,
+                     const krylov_params &p, nrq_solve_stats *st)
 {
     const int64_t n = 3 * g_ws.vol;
-    const double kmax = cheb_kmax();
-    std::vector<std::complex<double>> sig(R);
-    for (int r = 0; r < R; ++r) {
-        const std::complex<double> d = beta[r] * (kmax / 2);
-        sig[r] = std::abs(d) > 0 ? (1.0 + d) / d : std::complex<double>(1e30, 0.0);
-    }
-    const bool mixed = p.kind == KRYLOV_CHEB_MIXED;
-    const cplx<float> *l32 = nullptr;
-    if (mixed) {
-        if (gf->precision == 64) {
-            hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)gauge_links(gf),
-                               g_ws.links32, 27 * g_ws.vol);
-            l32 = g_ws.links32;
-        } else {
-            l32 = (const cplx<float> *)gauge_links(gf);
-        }
-    }
-    double rn2[NSLOTS];
-    for (int outer = 0; outer < 8; ++outer) {
-        if (int e = upload_active(act)) return e;
-        if (int e = residual64(gf, R, rn2)) return e;
-        double need = 1.0;
-        bool any = false;
-        for (int r = 0; r < R; ++r) {
-            if (!act[r]) continue;
-            const double rr = std::sqrt(std::max(rn2[r], 0.0));
-            rel[r] = rr / bn[r];
-            if (rr <= p.tol * bn[r]) {
-                act[r] = 0;
-                continue;
-            }
-            any = true;
-            need = std::min(need, 0.3 * p.tol * bn[r] / rr);
-        }
-        if (!any) return NRQ_OK;
-        ++cycles;
-        if (int e = upload_active(act)) return e;
-        if (!mixed) {
-            const int k = cheb_steps(sig, need);
-            if (gf->precision == 64) {
-                if (int e = cheb_run<double, double>((const cplx<double> *)gauge_links(gf), R, g_ws.y, g_ws.x, g_ws.t, beta, k, kmax, act))
-                    return e;
-            } else {
-                if (int e = cheb_run<float, double>((const cplx<float> *)gauge_links(gf), R, g_ws.y, g_ws.x, g_ws.t, beta, k, kmax, act))
-                    return e;
-            }
-            for (int r = 0; r < R; ++r) its[r] += act[r] ? k : 0;
-        } else {
-            // correction A d = r / ||r|| in FP32 to the accuracy FP32 allows
-            const int k = cheb_steps(sig, std::max(need, 3e-7));
-            double *rnorm2 = g_ws.nrm + 32;
-            NRQ_HIP(hipMemcpyAsync(rnorm2, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToDevice, stream()));
-            hipLaunchKernelGGL(k_demote, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<double> *)g_ws.res, g_ws.y32, n,
-                               (const double *)rnorm2, g_ws.active);
-            NRQ_HIP(hipMemsetAsync(g_ws.x32, 0, sizeof(cplx<float>) * (size_t)n * R, stream()));
-            if (int e = cheb_run<float, float>(l32, R, g_ws.y32, g_ws.x32, g_ws.t32, beta, k, kmax, act)) return e;
-            hipLaunchKernelGGL(k_promote_add, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.x, (const cplx<float> *)g_ws.x32, n,
-                               (const double *)rnorm2, g_ws.active);
-            NRQ_HIP(hipGetLastError());
-            for (int r = 0; r < R; ++r) its[r] += act[r] ? k : 0;
-        }
-    }
-    return NRQ_OK; // slots still active fall back to GMRES in the caller
-}
-
-int krylov_solve(const nrq_gauge *gf, int R, const cplx<double> *beta, const cplx<double> *alpha,
-                 const cplx<double> *prev, const double *y_norm2, const krylov_params &p, nrq_solve_stats *st)
-{
-    const int64_t n = 3 * g_ws.vol;
-    std::vector<double> bn(R);
-    for (int r = 0; r < R; ++r) bn[r] = std::sqrt(y_norm2[r]);
+    std::vector<double> bn(R, 0.0), rel(R, 0.0);
     std::vector<int64_t> its(R, 0);
-    std::vector<double> rel(R, 0.0);
     int64_t cycles = 0;
     std::vector<int> act(R, 1);
-    if (int e = upload_beta(beta, R)) return e;
+    const bool guess = p.extrapolate && p.kind != KRYLOV_GMRES_FP64;
+    // right-hand sides and guesses; ||y||^2 to nrm[32..)
+    const int nc = shape().nc64;
+    const lf_map mp = cn_map(R, nc);
+    const int gs = guess ? 1 : 0;
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u,
+                                       (const cplx<double> *)psi, g_ws.fmap, g_ws.y, g_ws.x, g_ws.cpar, g_ws.cpar + NSLOTS, gs,
+                                       R, lat().g, mp, g_ws.spart));
+    NRQ_HIP(hipGetLastError());
+    if (int e = fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm + 32)) return e;
+    cplx<double> *xsol = g_ws.x;
     int status = NRQ_OK;
-
-    if (p.kind == KRYLOV_GMRES_FP64) {
-        if (gf->precision == 64)
-            status = gmres_core<double, double>((const cplx<double> *)gauge_links(gf), R, g_ws.y, g_ws.x,
-                                                (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, true, bn.data(), act, p.tol,
-                                                p.restart, p.maxit, its, rel.data(), cycles);
-        else
-            status = gmres_core<double, float>((const cplx<float> *)gauge_links(gf), R, g_ws.y, g_ws.x,
-                                               (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, true, bn.data(), act, p.tol,
-                                               p.restart, p.maxit, its, rel.data(), cycles);
-        if (status) return status;
-    } else if (p.kind == KRYLOV_CHEB_FP64 || p.kind == KRYLOV_CHEB_MIXED) {
-        std::vector<std::complex<double>> bz(R);
-        for (int r = 0; r < R; ++r) bz[r] = std::complex<double>(beta[r].re, beta[r].im);
-        if (int e = upload_active(act)) return e;
-        if (p.extrapolate && prev && alpha) {
-            std::vector<cplx<double>> coef(R);
-            for (int r = 0; r < R; ++r) {
-                const std::complex<double> a(alpha[r].re, alpha[r].im);
-                const std::complex<double> c = std::abs(a) > 0 ? (a - bz[r]) / a : std::complex<double>(0.0);
-                coef[r] = {c.real(), c.imag()};
-            }
-            NRQ_HIP(hipMemcpyAsync(g_ws.coef, coef.data(), sizeof(cplx<double>) * R, hipMemcpyHostToDevice, stream()));
-            hipLaunchKernelGGL(k_guess, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.x, prev, g_ws.y, g_ws.coef, n, g_ws.active);
-        } else {
-            NRQ_HIP(hipMemsetAsync(g_ws.x, 0, sizeof(cplx<double>) * (size_t)n * R, stream()));
-        }
-        if (int e = cheb_solve(gf, R, bz, bn.data(), p, act, rel, its, cycles)) return e;
+    if (p.kind == KRYLOV_CHEB_FP64 || p.kind == KRYLOV_CHEB_MIXED) {
+        if (int e = cheb_solve<RL>(u, R, alpha, beta, bn, p, act, rel, its, cycles, &xsol)) return e;
         bool left = false;
         for (int r = 0; r < R; ++r) left = left || act[r];
         if (left) {
-            // spectrum bound violated (e.g. far from unitary links): GMRES from here
-            if (gf->precision == 64)
-                status = gmres_core<double, double>((const cplx<double> *)gauge_links(gf), R, g_ws.y, g_ws.x,
-                                                    (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, false, bn.data(), act,
-                                                    p.tol, p.restart, p.maxit, its, rel.data(), cycles);
-            else
-                status = gmres_core<double, float>((const cplx<float> *)gauge_links(gf), R, g_ws.y, g_ws.x,
-                                                   (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, false, bn.data(), act,
-                                                   p.tol, p.restart, p.maxit, its, rel.data(), cycles);
-            if (status) return status;
+            // spectrum bound violated (e.g. links far from unitary): GMRES from here
+            if (xsol != g_ws.x)
+                NRQ_HIP(hipMemcpyAsync(g_ws.x, xsol, sizeof(cplx<double>) * (size_t)n * R, hipMemcpyDeviceToDevice, stream()));
+            xsol = g_ws.x;
+            status = gmres_core<double, RL>(u, R, g_ws.y, g_ws.x, (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, false,
+                                            bn.data(), act, p.tol, p.restart, p.maxit, its, rel.data(), cycles);
         }
     } else {
-        // FP32 links for the corrections
-        const cplx<float> *l32;
-        if (gf->precision == 64) {
-            hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)gauge_links(gf),
-                               g_ws.links32, 27 * g_ws.vol);
-            l32 = g_ws.links32;
-        } else {
-            l32 = (const cplx<float> *)gauge_links(gf);
-        }
-        // initial guess
-        if (int e = upload_active(act)) return e;
-        if (p.extrapolate && prev && alpha) {
-            std::vector<cplx<double>> coef(R);
-            for (int r = 0; r < R; ++r) {
-                const std::complex<double> a(alpha[r].re, alpha[r].im), b(beta[r].re, beta[r].im);
-                const std::complex<double> c = std::abs(a) > 0 ? (a - b) / a : std::complex<double>(0.0);
-                coef[r] = {c.real(), c.imag()};
-            }
-            NRQ_HIP(hipMemcpyAsync(g_ws.coef, coef.data(), sizeof(cplx<double>) * R, hipMemcpyHostToDevice, stream()));
-            hipLaunchKernelGGL(k_guess, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.x, prev, g_ws.y, g_ws.coef, n, g_ws.active);
-        } else {
-            NRQ_HIP(hipMemsetAsync(g_ws.x, 0, sizeof(cplx<double>) * (size_t)n * R, stream()));
-        }
-        std::vector<double> one(R, 1.0);
-        for (int outer = 0; outer < 40; ++outer) {
-            // FP64 true residual r = y - A x
-            if (int e = upload_active(act)) return e;
-            if (int e = apply64(gf, g_ws.x, g_ws.t, R)) return e;
-            hipLaunchKernelGGL(k_residual<double>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<double> *)g_ws.y,
-                               (const cplx<double> *)g_ws.t, g_ws.res, n, g_ws.active, g_ws.rpart);
-            hipLaunchKernelGGL(k_fold_r, dim3(1), dim3(64), 0, stream(), g_ws.rpart, VBLOCKS, R, g_ws.active, g_ws.nrm);
-            NRQ_HIP(hipGetLastError());
-            NRQ_HIP(hipMemcpyAsync(g_ws.nbuf, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToHost, stream()));
-            NRQ_HIP(hipStreamSynchronize(stream()));
-            bool any = false;
-            for (int r = 0; r < R; ++r) {
-                if (!act[r]) continue;
-                const double rr = std::sqrt(std::max(g_ws.nbuf[r], 0.0));
-                rel[r] = rr / bn[r];
-                if (rr <= p.tol * bn[r] || its[r] >= p.maxit) act[r] = 0;
-                any = any || act[r];
-            }
-            if (!any) break;
-            ++cycles;
-            // keep ||r||^2 of this refinement step in nrm[32..) (the inner
-            // solve reuses nrm[0..R))
-            double *rnorm2 = g_ws.nrm + 32;
-            NRQ_HIP(hipMemcpyAsync(rnorm2, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToDevice, stream()));
-            // normalised FP32 correction problem A d = r / ||r||
-            if (int e = upload_active(act)) return e;
-            hipLaunchKernelGGL(k_demote, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<double> *)g_ws.res, g_ws.y32, n,
-                               (const double *)rnorm2, g_ws.active);
-            std::vector<double> inner_rel(R, 0.0);
-            int64_t inner_cycles = 0;
-            if (int e = gmres_core<float, float>(l32, R, g_ws.y32, g_ws.x32, (cplx<float> *)g_ws.basis, g_ws.t32, g_ws.res32, true,
-                                                 one.data(), act, std::max(p.inner_tol, std::max(p.tol / 1e-3, 2e-7)), p.restart,
-                                                 p.maxit, its, inner_rel.data(), inner_cycles))
-                return e;
-            // gmres_core left the slot masks of the inner solve on the device;
-            // the refinement updates every slot that is still active here
-            if (int e = upload_active(act)) return e;
-            hipLaunchKernelGGL(k_promote_add, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), g_ws.x, (const cplx<float> *)g_ws.x32, n,
-                               (const double *)rnorm2, g_ws.active);
-            NRQ_HIP(hipGetLastError());
-        }
-        for (int r = 0; r < R; ++r)
-            if (act[r]) status = NRQ_ERR_NOCONV;
+        if (int e = nrm_to_host()) return e;
+        for (int r = 0; r < R; ++r) bn[r] = std::sqrt(std::max(g_ws.nbuf[32 + r], 0.0));
+        if (p.kind == KRYLOV_GMRES_FP64)
+            status = gmres_core<double, RL>(u, R, g_ws.y, g_ws.x, (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, true,
+                                            bn.data(), act, p.tol, p.restart, p.maxit, its, rel.data(), cycles);
+        else
+            status = gmres_mixed<RL>(u, R, bn, p, act, rel, its, cycles);
     }
+    if (status) return status;
+    // psi_f = x for every slot
+    hipLaunchKernelGGL(k_scatter, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<double> *)xsol, g_ws.fmap, psi, n);
+    NRQ_HIP(hipGetLastError());
     double worst = 0;
     int64_t total = 0;
     for (int r = 0; r < R; ++r) {
         worst = std::max(worst, rel[r]);
         total += its[r];
-        if (rel[r] > p.tol) status = NRQ_ERR_NOCONV;
     }
     if (st) {
         st->iterations += total;
         st->restarts += cycles;
         st->max_true_relres = std::max(st->max_true_relres, worst);
     }
-    if (status == NRQ_ERR_NOCONV)
+    if (worst > p.tol)
         return fail(NRQ_ERR_NOCONV, "CN: Krylov solver stopped (maxit=%lld) with true relative residual %.3e > %.1e",
                     (long long)p.maxit, worst, p.tol);
-    return status;
+    return NRQ_OK;
+}
+
+int krylov_cn_step(const nrq_gauge *gf, cplx<double> *psi, const int *fields, int R, const cplx<double> *alpha,
+                   const cplx<double> *beta, const krylov_params &p, nrq_solve_stats *st)
+{
+    if (R <= 0) return NRQ_OK;
+    if (R > NSLOTS) return fail(NRQ_ERR_ARG, "CN: %d fields exceed the batch of %d", R, NSLOTS);
+    if (g_ws.vol != lat().g.vol) return fail(NRQ_ERR_STATE, "CN: workspace not reserved for this lattice");
+    std::vector<int> fm(fields, fields + R), act(R, 1);
+    std::vector<cplx<double>> par(2 * NSLOTS, cplx<double>{0.0, 0.0});
+    std::vector<zc> a(R), b(R);
+    for (int r = 0; r < R; ++r) {
+        a[r] = zc(alpha[r].re, alpha[r].im);
+        b[r] = zc(beta[r].re, beta[r].im);
+        par[r] = alpha[r];
+        const zc g = a[r] - b[r];
+        par[NSLOTS + r] = {g.real(), g.imag()};
+    }
+    if (int e = upload_ints(g_ws.fmap, fm)) return e;
+    if (int e = upload_active(act)) return e;
+    if (int e = upload_ints(g_ws.dact, act)) return e;
+    NRQ_HIP(hipMemcpyAsync(g_ws.cpar, par.data(), sizeof(cplx<double>) * par.size(), hipMemcpyHostToDevice, stream()));
+    NRQ_HIP(hipMemcpyAsync(g_ws.beta, beta, sizeof(cplx<double>) * R, hipMemcpyHostToDevice, stream()));
+    if (gf->precision == 64) return cn_step_t<double>((const cplx<double> *)gauge_links(gf), psi, R, a, b, p, st);
+    return cn_step_t<float>((const cplx<float> *)gauge_links(gf), psi, R, a, b, p, st);
 }
 
 } // namespace nrq

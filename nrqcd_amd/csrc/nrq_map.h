@@ -25,32 +25,45 @@ struct lf_map {
     int ngroup;  // work items per site block
 };
 
-__device__ __forceinline__ int64_t lf_logical_block(const lf_map &m)
+__device__ __forceinline__ unsigned lf_logical_block(const lf_map &m)
 {
-    const int64_t b = blockIdx.x;
+    const unsigned b = blockIdx.x;
     if (m.xcd <= 1) return b;
-    const int64_t total = (int64_t)m.ngroup * m.nsb;
-    const int64_t q = total / m.xcd, r = total % m.xcd;
-    const int64_t x = b % m.xcd, k = b / m.xcd;
+    const unsigned total = (unsigned)m.ngroup * (unsigned)m.nsb, xcd = (unsigned)m.xcd;
+    const unsigned q = total / xcd, r = total % xcd;
+    const unsigned x = b % xcd, k = b / xcd;
     return x * q + (x < r ? x : r) + k;
 }
 
-// Site of this thread and the work item (direction / chunk) of its block.
-__device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, int64_t &s, int &item)
+// Site of this thread, the work item (direction / chunk) of its block and the
+// block's site-block index t (0 <= t < nsb).
+__device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p, int &item, int64_t &t)
 {
-    const int64_t lb = lf_logical_block(m);
-    item = (int)(lb % m.ngroup);
-    const int64_t t = lb / m.ngroup;
+    const unsigned lb = lf_logical_block(m);
+    const unsigned ng = (unsigned)m.ngroup;
+    const unsigned tt = lb / ng;
+    item = (int)(lb - tt * ng);
+    t = tt;
     if (m.mode == 0) {
-        s = t * m.bs + threadIdx.x;
-        return s < g.vol;
+        const int64_t s = (int64_t)tt * m.bs + threadIdx.x;
+        if (s >= g.vol) return false;
+        p = make_site(g, s);
+        return true;
     }
-    const int64_t z = t % g.n2, yb = t / g.n2;
+    const unsigned n2 = (unsigned)g.n2;
+    const unsigned yb = tt / n2;
+    const int z = (int)(tt - yb * n2);
     const int x = (int)threadIdx.x % g.n0, yy = (int)threadIdx.x / g.n0;
-    const int64_t y = yb * m.ty + yy;
+    const int y = (int)yb * m.ty + yy;
     if (yy >= m.ty || y >= g.n1) return false;
-    s = x + (int64_t)g.n0 * (y + (int64_t)g.n1 * z);
+    p = lsite{x + (int64_t)g.n0 * (y + (int64_t)g.n1 * z), x, y, z};
     return true;
+}
+
+__device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p, int &item)
+{
+    int64_t t;
+    return lf_site(m, g, p, item, t);
 }
 
 // Host: map for `bs` threads per block; tile mode needs n0 <= bs.

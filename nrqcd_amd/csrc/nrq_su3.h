@@ -241,36 +241,85 @@ struct geom {
     double a[4];          // a0 (time), a1, a2, a3
 };
 
-// Periodic neighbour of site s (0-based, x fastest) in direction d (0,1,2)
-// and sign +1/-1 (lattice.f90:846-874 on 0-based indices).
-__device__ __forceinline__ int64_t neib_up(const geom &g, int64_t s, int d)
+// A lattice site with its coordinates: s = x + n0 (y + n1 z), 0-based, x
+// fastest (lattice.f90:883-898 on 0-based indices).  Kernels derive the
+// coordinates once per site (32-bit arithmetic; the library limits V to
+// 2^31 - 1) and step to neighbours by comparison, never by division.
+struct lsite {
+    int64_t s;
+    int x, y, z;
+};
+
+__device__ __forceinline__ lsite make_site(const geom &g, int64_t s)
 {
-    if (d == 0) {
-        int64_t x = s % g.n0;
-        return x == g.n0 - 1 ? s - (g.n0 - 1) : s + 1;
-    } else if (d == 1) {
-        int64_t y = (s / g.n0) % g.n1;
-        return y == g.n1 - 1 ? s - (int64_t)(g.n1 - 1) * g.n0 : s + g.n0;
-    } else {
-        int64_t pl = (int64_t)g.n0 * g.n1;
-        int64_t z = s / pl;
-        return z == g.n2 - 1 ? s - (int64_t)(g.n2 - 1) * pl : s + pl;
-    }
+    const unsigned su = (unsigned)s, n0 = (unsigned)g.n0, n1 = (unsigned)g.n1;
+    const unsigned q = su / n0;
+    const unsigned z = q / n1;
+    return lsite{s, (int)(su - q * n0), (int)(q - z * n1), (int)z};
 }
 
-__device__ __forceinline__ int64_t neib_dn(const geom &g, int64_t s, int d)
+// Periodic neighbour p + d^ (d = 0, 1, 2), lattice.f90:846-874.
+__device__ __forceinline__ lsite site_up(const geom &g, lsite p, int d)
 {
     if (d == 0) {
-        int64_t x = s % g.n0;
-        return x == 0 ? s + (g.n0 - 1) : s - 1;
+        if (p.x == g.n0 - 1) {
+            p.x = 0;
+            p.s -= g.n0 - 1;
+        } else {
+            ++p.x;
+            ++p.s;
+        }
     } else if (d == 1) {
-        int64_t y = (s / g.n0) % g.n1;
-        return y == 0 ? s + (int64_t)(g.n1 - 1) * g.n0 : s - g.n0;
+        if (p.y == g.n1 - 1) {
+            p.y = 0;
+            p.s -= (int64_t)(g.n1 - 1) * g.n0;
+        } else {
+            ++p.y;
+            p.s += g.n0;
+        }
     } else {
-        int64_t pl = (int64_t)g.n0 * g.n1;
-        int64_t z = s / pl;
-        return z == 0 ? s + (int64_t)(g.n2 - 1) * pl : s - pl;
+        const int64_t pl = (int64_t)g.n0 * g.n1;
+        if (p.z == g.n2 - 1) {
+            p.z = 0;
+            p.s -= (int64_t)(g.n2 - 1) * pl;
+        } else {
+            ++p.z;
+            p.s += pl;
+        }
     }
+    return p;
+}
+
+// Periodic neighbour p - d^.
+__device__ __forceinline__ lsite site_dn(const geom &g, lsite p, int d)
+{
+    if (d == 0) {
+        if (p.x == 0) {
+            p.x = g.n0 - 1;
+            p.s += g.n0 - 1;
+        } else {
+            --p.x;
+            --p.s;
+        }
+    } else if (d == 1) {
+        if (p.y == 0) {
+            p.y = g.n1 - 1;
+            p.s += (int64_t)(g.n1 - 1) * g.n0;
+        } else {
+            --p.y;
+            p.s -= g.n0;
+        }
+    } else {
+        const int64_t pl = (int64_t)g.n0 * g.n1;
+        if (p.z == 0) {
+            p.z = g.n2 - 1;
+            p.s += (int64_t)(g.n2 - 1) * pl;
+        } else {
+            --p.z;
+            p.s -= pl;
+        }
+    }
+    return p;
 }
 
 } // namespace nrq

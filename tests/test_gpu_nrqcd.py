@@ -216,3 +216,65 @@ def test_chebyshev_complex_coefficient_and_anisotropy(nrq, oracle, cn):
     Qg, Ag = h.download()
     assert np.max(np.abs(Qg - Q)) < 2e-10 * np.max(np.abs(Q))
     assert np.max(np.abs(Ag - A)) < 2e-10 * np.max(np.abs(A))
+
+
+@pytest.mark.parametrize("kind", ["GMRES_FP64"] + SOLVERS)
+def test_partially_zero_propagators(nrq, oracle, cn, kind):
+    """Only the live (non-zero) fields enter the batched solve: a quark
+    propagator with two zero columns and a zero antiquark propagator evolve
+    like the oracle's, and the zero parts stay exactly zero.  A copy keeps the
+    same set of live fields."""
+    ext = (6, 4, 5)
+    _say(f"CN partial propagator {kind}")
+    cn(ext, SP, 1e-12)
+    nrq.nrqcd_set_solver(getattr(nrq, "NRQ_SOLVER_" + kind), 30, 10000)
+    oracle.lattice_init(ext, SP)
+    U, E = oracle.box_init(5, 0.5, 1.0, 1.0)
+    rng = np.random.default_rng(8)
+    v = oracle.vol
+    Q = np.asfortranarray(rng.normal(size=(6, 6, v)) + 1j * rng.normal(size=(6, 6, v)))
+    Q[:, 1, :] = 0
+    Q[:, 4, :] = 0
+    A = np.zeros((6, 6, v), dtype=np.complex128, order="F")
+    g = nrq.GaugeConfiguration()
+    g.upload(U, E)
+    h = nrq.NRQCDField()
+    h.upload(Q, A)
+    h2 = h.copy()
+    for _ in range(2):
+        st = h.Update(g, 2.0, C1)
+        assert st.solves == 8  # 4 non-zero columns x 2 spins
+        assert st.max_true_relres <= 1e-12
+        h2.Update(g, 2.0, C1)
+        oracle.cn_update(U, Q, A, 2.0, C1, 1.0, 1e-13)
+    Qg, Ag = h.download()
+    Q2, A2 = h2.download()
+    assert np.array_equal(Qg, Q2) and np.array_equal(Ag, A2)
+    assert not np.any(Qg[:, 1, :]) and not np.any(Qg[:, 4, :]) and not np.any(Ag)
+    assert np.max(np.abs(Qg - Q)) < 2e-10 * np.max(np.abs(Q))
+
+
+@pytest.mark.parametrize("kind", ["CHEBYSHEV_FP64", "CHEBYSHEV_MIXED"])
+def test_chebyshev_long_step(nrq, oracle, cn, kind):
+    """A step 40x longer than the default (|beta| kmax = 3): the a priori
+    plan no longer suffices and the measured refinement rounds finish the
+    solve to the same FP64 bar."""
+    ext = (6, 6, 6)
+    _say(f"CN {kind} long step")
+    cn(ext, SP, 1e-12)
+    nrq.nrqcd_set_solver(getattr(nrq, "NRQ_SOLVER_" + kind), 30, 10000)
+    oracle.lattice_init(ext, SP)
+    U, E = oracle.box_init(6, 0.5, 1.0, 1.0)
+    g = nrq.GaugeConfiguration()
+    g.upload(U, E)
+    h = nrq.NRQCDField()
+    h.InitSinglePoint(5, 9)
+    Q, A = oracle.point_prop(5), oracle.point_prop(9)
+    for _ in range(2):
+        st = h.Update(g, 2.0, C1, 40.0)
+        assert st.max_true_relres <= 1e-12
+        rc, _ = oracle.cn_update(U, Q, A, 2.0, C1, 40.0, 1e-13)
+        assert rc == 0
+    Qg, Ag = h.download()
+    assert np.max(np.abs(Qg - Q)) < 1e-9 * np.max(np.abs(Q))
+    assert np.max(np.abs(Ag - A)) < 1e-9 * np.max(np.abs(A))
