@@ -150,13 +150,22 @@ int nrq_gauge_plaquette(const nrq_gauge *h, int i, int j, int64_t idx, double P[
  * row 3 = conj(row1 x row2)).  No reference counterpart: the reference keeps
  * FP64 links and never reunitarises; used by the FP32 storage mode. */
 int nrq_gauge_reunitarise(nrq_gauge *h);
+/* Makes nrq_gauge_leapfrog reunitarise (as above) after every `interval`
+ * steps, counted across calls (0: never, the default).  No reference
+ * counterpart: it keeps FP32 link storage on the group over long runs. */
+int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval);
 /* Launch shape the leapfrog autotuner chose for this configuration's
- * precision and lattice size: threads per block, XCD remap factor (1 = none)
- * and tiling (0: consecutive sites, 1: z-sweep tiles of full x-rows).  The
- * first leapfrog call per (precision, size) times every shape on scratch
- * copies; NRQ_AUTOTUNE=0 in the environment disables it.  All shapes compute
- * identical results.  No reference counterpart. */
-int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling);
+ * precision and lattice size: threads per block, XCD remap factor (1 = none),
+ * tiling (0: consecutive sites, 1: z-sweep tiles of full x-rows, 2: z-sweep
+ * tiles 16 sites wide), streaming (1: nontemporal loads/stores of E and the
+ * new links) and two_row (1: neighbour links read as two rows, the third
+ * rebuilt as the conjugate cross product, exact for SU(3)).  The first
+ * leapfrog call per (precision, size) times every shape on scratch copies;
+ * NRQ_AUTOTUNE=0 in the environment disables it and fixes the first shape.
+ * All shapes compute the same update, equal up to rounding.  No reference
+ * counterpart. */
+int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling, int *streaming,
+                           int *two_row);
 
 /* ---- nrqcd ------------------------------------------------------------ */
 

@@ -122,7 +122,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_gauge_observables": (I, [P, P]),
         "nrq_gauge_plaquette": (I, [P, I, I, I64, P]),
         "nrq_gauge_reunitarise": (I, [P]),
-        "nrq_gauge_launch_shape": (I, [P, P, P, P]),
+        "nrq_gauge_set_reunitarise_interval": (I, [P, I64]),
+        "nrq_gauge_launch_shape": (I, [P, P, P, P, P, P]),
         "nrq_cn_init": (I, [D, P, D, D]),
         "nrq_cn_finalize": (I, []),
         "nrq_cn_set_solver": (I, [I, I, I64]),
@@ -320,12 +321,18 @@ class GaugeConfiguration:
     def Reunitarise(self) -> None:
         _check(library().nrq_gauge_reunitarise(self._h))
 
+    def SetReunitarisationInterval(self, interval: int) -> None:
+        """Reunitarise (FP64) after every `interval` leapfrog steps; 0: never."""
+        _check(library().nrq_gauge_set_reunitarise_interval(self._h, int(interval)))
+
     def launch_shape(self) -> dict:
         """Launch shape chosen by the leapfrog autotuner."""
-        bs, xcd, tiling = ctypes.c_int(), ctypes.c_int(), ctypes.c_int()
-        _check(library().nrq_gauge_launch_shape(self._h, ctypes.byref(bs), ctypes.byref(xcd), ctypes.byref(tiling)))
+        bs, xcd, tiling, nt, rec = (ctypes.c_int() for _ in range(5))
+        _check(library().nrq_gauge_launch_shape(self._h, ctypes.byref(bs), ctypes.byref(xcd), ctypes.byref(tiling),
+                                                ctypes.byref(nt), ctypes.byref(rec)))
         return {"threads_per_block": bs.value, "xcd_remap": xcd.value,
-                "tiling": "z-sweep tiles" if tiling.value else "consecutive sites"}
+                "tiling": ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles")[tiling.value],
+                "streaming": bool(nt.value), "two_row_neighbours": bool(rec.value)}
 
     # -- observables -----------------------------------------------------
     def observables(self) -> dict:

@@ -10,6 +10,7 @@
 // new links.
 #include "nrq_internal.h"
 #include "nrq_map.h"
+#include <algorithm>
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
@@ -36,6 +37,55 @@ __device__ __forceinline__ void store_link(cplx<R> *__restrict__ u, int64_t vol,
     for (int e = 0; e < 9; ++e) p[(int64_t)e * vol] = m.m[e];
 }
 
+// Neighbour link for the staples.  REC: only rows 1 and 2 are read and row 3
+// is rebuilt as conj(row1 x row2), exact for SU(3) (unit determinant,
+// unitary), which cuts the bytes of every neighbour fetch by a third; the
+// stored links keep all nine entries.
+template <bool REC, typename R>
+__device__ __forceinline__ mat3<R> load_nb(const cplx<R> *__restrict__ u, int64_t vol, int dir, int64_t s)
+{
+    if (!REC) return load_link(u, vol, dir, s);
+    mat3<R> m;
+    const cplx<R> *p = u + (int64_t)dir * 9 * vol + s;
+#pragma unroll
+    for (int e = 0; e < 6; ++e) m.m[e] = p[(int64_t)e * vol];
+    // row3_c = conj(row1_{c+1} row2_{c+2} - row1_{c+2} row2_{c+1})
+#pragma unroll
+    for (int c = 0; c < 3; ++c) {
+        const int c1 = (c + 1) % 3, c2 = (c + 2) % 3;
+        cplx<R> t = {R(0), R(0)};
+        mac_cc(t, m.m[c1], m.m[3 + c2]);
+        const cplx<R> a = {-m.m[c2].re, -m.m[c2].im};
+        mac_cc(t, a, m.m[3 + c1]);
+        m.m[6 + c] = t;
+    }
+    return m;
+}
+
+// Per-(site, dir) E and output-link access of the leapfrog kernels; NT:
+// streaming (nontemporal) loads and stores, since no other thread reads them
+// in the same pass.
+template <bool NT, typename R> __device__ __forceinline__ R ld_e(const R *p)
+{
+    if (NT) return __builtin_nontemporal_load(p);
+    return *p;
+}
+template <bool NT, typename R> __device__ __forceinline__ void st_e(R *p, R v)
+{
+    if (NT) __builtin_nontemporal_store(v, p);
+    else *p = v;
+}
+template <bool NT, typename R>
+__device__ __forceinline__ void store_link_out(cplx<R> *__restrict__ u, int64_t vol, int dir, int64_t s, const mat3<R> &m)
+{
+    cplx<R> *p = u + (int64_t)dir * 9 * vol + s;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        if (NT) store_nt(p + (int64_t)e * vol, m.m[e]);
+        else p[(int64_t)e * vol] = m.m[e];
+    }
+}
+
 // Update_Links_Leapfrog at one (site, dir), gaugeconfiguration_su3.f90:
 // 2133-2139: U <- exp(i dt E^a T^a) U.
 template <typename R>
@@ -52,7 +102,7 @@ __device__ __forceinline__ mat3<R> evolve_link(const mat3<R> &u, const R *e, R d
 // 2164-2198: E^a_i(x) -= 2 Im Tr[T^a U_i(x) sum_{k!=i} dt (a0/a_k)^2 (S^U+S^D)]
 // with S^U = U_k(x+i) U_i(x+k)^+ U_k(x)^+ and
 //      S^D = U_k(x+i-k)^+ U_i(x-k)^+ U_k(x-k).
-template <typename R>
+template <bool REC, typename R>
 __device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, const geom &g, int i, const lsite &p,
                                               const mat3<R> &ui, R *e, R dt)
 {
@@ -67,10 +117,10 @@ __device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, con
         const int64_t xk = site_up(g, p, k).s;
         const int64_t xmk = pmk.s;
         const int64_t ximk = site_up(g, pmk, i).s;
-        mat3<R> su = mul_nd(load_link(u, g.vol, k, xi), load_link(u, g.vol, i, xk));
-        su = mul_nd(su, load_link(u, g.vol, k, p.s));
-        mat3<R> sd = mul_dd(load_link(u, g.vol, k, ximk), load_link(u, g.vol, i, xmk));
-        sd = mul(sd, load_link(u, g.vol, k, xmk));
+        mat3<R> su = mul_nd(load_nb<REC>(u, g.vol, k, xi), load_nb<REC>(u, g.vol, i, xk));
+        su = mul_nd(su, load_nb<REC>(u, g.vol, k, p.s));
+        mat3<R> sd = mul_dd(load_nb<REC>(u, g.vol, k, ximk), load_nb<REC>(u, g.vol, i, xmk));
+        sd = mul(sd, load_nb<REC>(u, g.vol, k, xmk));
 #pragma unroll
         for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * (su.m[m] + sd.m[m]);
     }
@@ -81,7 +131,7 @@ __device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, con
     for (int a = 0; a < 8; ++a) e[a] -= t[a];
 }
 
-template <typename R, int BS>
+template <typename R, int BS, bool NT>
 __global__ __launch_bounds__(BS) void k_links(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
                                               const R *__restrict__ ef, geom g, R dt, lf_map m)
 {
@@ -92,12 +142,12 @@ __global__ __launch_bounds__(BS) void k_links(const cplx<R> *__restrict__ uin, c
     const int64_t vol = g.vol;
     R e[8];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * vol + s];
+    for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * vol + s);
     mat3<R> u = load_link(uin, vol, i, s);
-    store_link(uout, vol, i, s, evolve_link(u, e, dt));
+    store_link_out<NT>(uout, vol, i, s, evolve_link(u, e, dt));
 }
 
-template <typename R, int BS>
+template <typename R, int BS, bool NT, bool REC>
 __global__ __launch_bounds__(BS) void k_efield(const cplx<R> *__restrict__ u, R *__restrict__ ef, geom g, R dt,
                                                lf_map m)
 {
@@ -107,11 +157,11 @@ __global__ __launch_bounds__(BS) void k_efield(const cplx<R> *__restrict__ u, R 
     const int64_t s = p.s;
     R e[8];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * g.vol + s];
+    for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s);
     mat3<R> ui = load_link(u, g.vol, i, s);
-    evolve_efield(u, g, i, p, ui, e, dt);
+    evolve_efield<REC>(u, g, i, p, ui, e, dt);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) ef[((int64_t)i * 8 + a) * g.vol + s] = e[a];
+    for (int a = 0; a < 8; ++a) st_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s, e[a]);
 }
 
 // One fused pass: the E update of a leapfrog step followed by the link update
@@ -119,7 +169,7 @@ __global__ __launch_bounds__(BS) void k_efield(const cplx<R> *__restrict__ u, R 
 // per-(site, dir) operations of the reference; fusing them only removes one
 // HBM round trip of E and U.  Neighbour links are read from uin, new links
 // go to uout (ping-pong), E is updated in place (each thread owns its E).
-template <typename R, int BS>
+template <typename R, int BS, bool NT, bool REC>
 __global__ __launch_bounds__(BS) void k_fused(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
                                               R *__restrict__ ef, geom g, R dt, lf_map m)
 {
@@ -129,12 +179,12 @@ __global__ __launch_bounds__(BS) void k_fused(const cplx<R> *__restrict__ uin, c
     const int64_t s = p.s;
     R e[8];
 #pragma unroll
-    for (int a = 0; a < 8; ++a) e[a] = ef[((int64_t)i * 8 + a) * g.vol + s];
+    for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s);
     mat3<R> ui = load_link(uin, g.vol, i, s);
-    evolve_efield(uin, g, i, p, ui, e, dt);
+    evolve_efield<REC>(uin, g, i, p, ui, e, dt);
 #pragma unroll
-    for (int a = 0; a < 8; ++a) ef[((int64_t)i * 8 + a) * g.vol + s] = e[a];
-    store_link(uout, g.vol, i, s, evolve_link(ui, e, dt));
+    for (int a = 0; a < 8; ++a) st_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s, e[a]);
+    store_link_out<NT>(uout, g.vol, i, s, evolve_link(ui, e, dt));
 }
 
 template <typename R> __global__ void k_cold(cplx<R> *u, R *ef, int64_t vol)
@@ -392,17 +442,23 @@ int gauge_obs(const nrq_gauge *h, double out[NOBS])
     return NRQ_OK;
 }
 
-// Launch shapes the autotuner chooses from (block size, XCD remap).  All
-// variants perform identical arithmetic per (site, dir), so the choice never
-// changes results, only speed.
+// Launch shapes the autotuner chooses from (block size, XCD remap, tiling,
+// streaming, two-row neighbour links).  All variants compute the same
+// update per (site, dir); the two-row form rebuilds the third row of each
+// neighbour link, so results agree up to rounding.
 struct lf_variant {
     int bs;
     int xcd;
     int mode;
+    int nt;  // streaming loads/stores of E and of the new links
+    int rec; // neighbour links read as two rows (third rebuilt)
 };
-static const lf_variant LF_VARIANTS[] = {{256, 8, 1}, {128, 8, 1}, {64, 8, 1}, {256, 8, 0},
-                                         {128, 8, 0}, {64, 8, 0},  {256, 1, 0}};
+static const lf_variant LF_VARIANTS[] = {
+    {256, 8, 1, 1, 1}, {128, 8, 1, 1, 1}, {64, 8, 1, 1, 1}, {256, 8, 0, 1, 1}, {128, 8, 0, 1, 1}, {64, 8, 0, 1, 1},
+    {256, 8, 2, 1, 1}, {256, 8, 2, 0, 1}, {256, 8, 0, 0, 1}, {64, 8, 0, 0, 1},  {256, 8, 1, 1, 0}, {256, 8, 0, 0, 0},
+    {256, 8, 2, 0, 0}, {64, 8, 0, 0, 0},  {256, 1, 0, 0, 1}};
 constexpr int LF_NVARIANTS = sizeof(LF_VARIANTS) / sizeof(LF_VARIANTS[0]);
+constexpr int LF_DEFAULT = 11; // {256, 8, 0, 0, 0}: used without tuning, valid on every lattice
 
 struct lf_choice {
     int precision;
@@ -415,8 +471,29 @@ static int g_autotune = -1; // -1: unset (env NRQ_AUTOTUNE, default on)
 static bool variant_ok(int variant, const geom &g)
 {
     const lf_variant v = LF_VARIANTS[variant];
-    return v.mode == 0 || g.n0 <= v.bs;
+    if (v.mode == 1) return g.n0 <= v.bs;
+    if (v.mode == 2) return g.n0 >= 16;
+    return true;
 }
+
+// Runs `launch` with NTV, RECV = the streaming and two-row flags of `variant`.
+#define LF_NT(launch)                                                      \
+    do {                                                                   \
+        const lf_variant lv_ = LF_VARIANTS[variant];                       \
+        if (lv_.nt && lv_.rec) {                                           \
+            constexpr bool NTV = true, RECV = true;                        \
+            launch;                                                        \
+        } else if (lv_.nt) {                                               \
+            constexpr bool NTV = true, RECV = false;                       \
+            launch;                                                        \
+        } else if (lv_.rec) {                                              \
+            constexpr bool NTV = false, RECV = true;                       \
+            launch;                                                        \
+        } else {                                                           \
+            constexpr bool NTV = false, RECV = false;                      \
+            launch;                                                        \
+        }                                                                  \
+    } while (0)
 
 static lf_map make_map(int variant, const geom &g)
 {
@@ -430,9 +507,10 @@ static void launch_links(int variant, const cplx<R> *uin, cplx<R> *uout, const R
     const lf_map m = make_map(variant, g);
     const dim3 grd((unsigned)(m.ngroup * m.nsb));
     switch (m.bs) {
-    case 64: hipLaunchKernelGGL((k_links<R, 64>), grd, dim3(64), 0, stream(), uin, uout, ef, g, dt, m); break;
-    case 128: hipLaunchKernelGGL((k_links<R, 128>), grd, dim3(128), 0, stream(), uin, uout, ef, g, dt, m); break;
-    default: hipLaunchKernelGGL((k_links<R, 256>), grd, dim3(256), 0, stream(), uin, uout, ef, g, dt, m); break;
+    case 64: LF_NT(hipLaunchKernelGGL((k_links<R, 64, NTV>), grd, dim3(64), 0, stream(), uin, uout, ef, g, dt, m)); break;
+    case 128: LF_NT(hipLaunchKernelGGL((k_links<R, 128, NTV>), grd, dim3(128), 0, stream(), uin, uout, ef, g, dt, m)); break;
+    case 512: LF_NT(hipLaunchKernelGGL((k_links<R, 512, NTV>), grd, dim3(512), 0, stream(), uin, uout, ef, g, dt, m)); break;
+    default: LF_NT(hipLaunchKernelGGL((k_links<R, 256, NTV>), grd, dim3(256), 0, stream(), uin, uout, ef, g, dt, m)); break;
     }
 }
 
@@ -442,9 +520,10 @@ static void launch_efield(int variant, const cplx<R> *u, R *ef, const geom &g, R
     const lf_map m = make_map(variant, g);
     const dim3 grd((unsigned)(m.ngroup * m.nsb));
     switch (m.bs) {
-    case 64: hipLaunchKernelGGL((k_efield<R, 64>), grd, dim3(64), 0, stream(), u, ef, g, dt, m); break;
-    case 128: hipLaunchKernelGGL((k_efield<R, 128>), grd, dim3(128), 0, stream(), u, ef, g, dt, m); break;
-    default: hipLaunchKernelGGL((k_efield<R, 256>), grd, dim3(256), 0, stream(), u, ef, g, dt, m); break;
+    case 64: LF_NT(hipLaunchKernelGGL((k_efield<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), u, ef, g, dt, m)); break;
+    case 128: LF_NT(hipLaunchKernelGGL((k_efield<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), u, ef, g, dt, m)); break;
+    case 512: LF_NT(hipLaunchKernelGGL((k_efield<R, 512, NTV, RECV>), grd, dim3(512), 0, stream(), u, ef, g, dt, m)); break;
+    default: LF_NT(hipLaunchKernelGGL((k_efield<R, 256, NTV, RECV>), grd, dim3(256), 0, stream(), u, ef, g, dt, m)); break;
     }
 }
 
@@ -454,9 +533,10 @@ static void launch_fused(int variant, const cplx<R> *uin, cplx<R> *uout, R *ef, 
     const lf_map m = make_map(variant, g);
     const dim3 grd((unsigned)(m.ngroup * m.nsb));
     switch (m.bs) {
-    case 64: hipLaunchKernelGGL((k_fused<R, 64>), grd, dim3(64), 0, stream(), uin, uout, ef, g, dt, m); break;
-    case 128: hipLaunchKernelGGL((k_fused<R, 128>), grd, dim3(128), 0, stream(), uin, uout, ef, g, dt, m); break;
-    default: hipLaunchKernelGGL((k_fused<R, 256>), grd, dim3(256), 0, stream(), uin, uout, ef, g, dt, m); break;
+    case 64: LF_NT(hipLaunchKernelGGL((k_fused<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), uin, uout, ef, g, dt, m)); break;
+    case 128: LF_NT(hipLaunchKernelGGL((k_fused<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), uin, uout, ef, g, dt, m)); break;
+    case 512: LF_NT(hipLaunchKernelGGL((k_fused<R, 512, NTV, RECV>), grd, dim3(512), 0, stream(), uin, uout, ef, g, dt, m)); break;
+    default: LF_NT(hipLaunchKernelGGL((k_fused<R, 256, NTV, RECV>), grd, dim3(256), 0, stream(), uin, uout, ef, g, dt, m)); break;
     }
 }
 
@@ -482,7 +562,7 @@ template <typename R> static int tune_leapfrog(const nrq_gauge *h, R dt, int *be
     NRQ_HIP(hipEventCreate(&e1));
     float best_ms = 1e30f;
     const int reps = g.vol >= (1 << 18) ? 6 : 20;
-    *best = variant_ok(0, g) ? 0 : 3;
+    *best = LF_DEFAULT;
     for (int v = 0; v < LF_NVARIANTS; ++v) {
         if (!variant_ok(v, g)) continue;
         launch_fused<R>(v, u0, u1, ef, g, dt); // warm-up
@@ -493,8 +573,9 @@ template <typename R> static int tune_leapfrog(const nrq_gauge *h, R dt, int *be
         float ms = 0;
         NRQ_HIP(hipEventElapsedTime(&ms, e0, e1));
         if (getenv("NRQ_AUTOTUNE_VERBOSE"))
-            fprintf(stderr, "nrqcd_amd autotune: precision %d V=%lld bs=%d xcd=%d tiling=%d: %.2f us/step\n", (int)(8 * sizeof(R)),
-                    (long long)g.vol, LF_VARIANTS[v].bs, LF_VARIANTS[v].xcd, LF_VARIANTS[v].mode, 1e3 * ms / reps);
+            fprintf(stderr, "nrqcd_amd autotune: precision %d V=%lld bs=%d xcd=%d tiling=%d streaming=%d two-row=%d: %.2f us/step\n",
+                    (int)(8 * sizeof(R)), (long long)g.vol, LF_VARIANTS[v].bs, LF_VARIANTS[v].xcd, LF_VARIANTS[v].mode,
+                    LF_VARIANTS[v].nt, LF_VARIANTS[v].rec, 1e3 * ms / reps);
         if (ms < best_ms) {
             best_ms = ms;
             *best = v;
@@ -514,7 +595,7 @@ template <typename R> static int leapfrog_variant(const nrq_gauge *h, R dt, int 
         const char *env = getenv("NRQ_AUTOTUNE");
         g_autotune = (env && env[0] == '0') ? 0 : 1;
     }
-    *variant = variant_ok(0, lat().g) ? 0 : 3;
+    *variant = LF_DEFAULT;
     if (!g_autotune) return NRQ_OK;
     for (const lf_choice &c : g_lf_choice)
         if (c.precision == h->precision && c.vol == h->vol) {
@@ -552,7 +633,7 @@ template <typename R> static int leapfrog_t(nrq_gauge *h, double sw, int64_t n)
     return NRQ_OK;
 }
 
-int leapfrog_choice(const nrq_gauge *h, int *bs, int *xcd, int *mode)
+int leapfrog_choice(const nrq_gauge *h, int *bs, int *xcd, int *mode, int *nt, int *rec)
 {
     int v = 0;
     if (int r = h->precision == 64 ? leapfrog_variant<double>(h, 1.0, &v) : leapfrog_variant<float>(h, 1.0f, &v))
@@ -560,6 +641,8 @@ int leapfrog_choice(const nrq_gauge *h, int *bs, int *xcd, int *mode)
     *bs = LF_VARIANTS[v].bs;
     *xcd = LF_VARIANTS[v].xcd;
     *mode = LF_VARIANTS[v].mode;
+    *nt = LF_VARIANTS[v].nt;
+    *rec = LF_VARIANTS[v].rec;
     return NRQ_OK;
 }
 
@@ -615,6 +698,7 @@ extern "C" int nrq_gauge_copy(nrq_gauge *dst, const nrq_gauge *src)
     NRQ_HIP(hipMemcpyAsync(dst->e, src->e, efield_bytes(src), hipMemcpyDeviceToDevice, stream()));
     dst->cur = 0;
     dst->steps_since_reunit = src->steps_since_reunit;
+    dst->reunit_interval = src->reunit_interval;
     return NRQ_OK;
 }
 
@@ -755,10 +839,25 @@ extern "C" int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps
 {
     NEED_GAUGE(h);
     if (nsteps < 0 || !std::isfinite(stepwidth)) return fail(NRQ_ERR_ARG, "leapfrog: bad step count or width");
-    if (nsteps == 0) return NRQ_OK;
-    int r = h->precision == 64 ? leapfrog_t<double>(h, stepwidth, nsteps) : leapfrog_t<float>(h, stepwidth, nsteps);
-    h->steps_since_reunit += nsteps;
-    return r;
+    while (nsteps > 0) {
+        // with a reunitarisation interval, stop at each multiple of it
+        int64_t n = nsteps;
+        if (h->reunit_interval > 0) n = std::min(n, std::max<int64_t>(1, h->reunit_interval - h->steps_since_reunit));
+        if (int r = h->precision == 64 ? leapfrog_t<double>(h, stepwidth, n) : leapfrog_t<float>(h, stepwidth, n)) return r;
+        h->steps_since_reunit += n;
+        nsteps -= n;
+        if (h->reunit_interval > 0 && h->steps_since_reunit >= h->reunit_interval)
+            if (int r = nrq_gauge_reunitarise(h)) return r;
+    }
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval)
+{
+    NEED_GAUGE(h);
+    if (interval < 0) return fail(NRQ_ERR_ARG, "reunitarisation interval %lld < 0", (long long)interval);
+    h->reunit_interval = interval;
+    return NRQ_OK;
 }
 
 extern "C" int nrq_gauge_observables(const nrq_gauge *h, double out[4])
@@ -849,9 +948,10 @@ void *gauge_links_mut(nrq_gauge *h) { return h->u[h->cur]; }
 void *gauge_efield(nrq_gauge *h) { return h->e; }
 }
 
-extern "C" int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling)
+extern "C" int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling, int *streaming,
+                                      int *two_row)
 {
     NEED_GAUGE(h);
-    if (!block_sites || !xcd_remap || !tiling) return fail(NRQ_ERR_ARG, "null argument");
-    return leapfrog_choice(h, block_sites, xcd_remap, tiling);
+    if (!block_sites || !xcd_remap || !tiling || !streaming || !two_row) return fail(NRQ_ERR_ARG, "null argument");
+    return leapfrog_choice(h, block_sites, xcd_remap, tiling, streaming, two_row);
 }

@@ -59,6 +59,7 @@ struct nrq_gauge {
     void *partials = nullptr; // reduction partials
     double *result = nullptr; // pinned host result slots
     int64_t steps_since_reunit = 0;
+    int64_t reunit_interval = 0; // reunitarise after this many steps (0: never)
 };
 
 struct nrq_heavy {

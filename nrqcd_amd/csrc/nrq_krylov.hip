@@ -69,6 +69,13 @@ __device__ __forceinline__ double g_dsum(const geom &g)
     return 1.0 / (g.a[1] * g.a[1]) + 1.0 / (g.a[2] * g.a[2]) + 1.0 / (g.a[3] * g.a[3]);
 }
 
+// Base pointers of the slot fields of one launch (3V complex numbers each,
+// colour-major); a slot's field may live in the workspace or in the heavy
+// field's own storage.  Passed to the kernels by value.
+template <typename T> struct slots {
+    cplx<T> *p[NSLOTS];
+};
+
 // Per-slot fields of one chunk, read in the compute type T.
 template <int NC, typename T, typename S> struct ld_fields {
     const cplx<S> *f[NC];
@@ -76,21 +83,6 @@ template <int NC, typename T, typename S> struct ld_fields {
     {
         const cplx<S> v = f[q][at];
         return {(T)v.re, (T)v.im};
-    }
-};
-
-// z = x + d per slot (d[q] null: no correction for that slot).
-template <int NC, typename TD> struct ld_xd {
-    const cplx<double> *x[NC];
-    const cplx<TD> *d[NC];
-    __device__ __forceinline__ cplx<double> operator()(int q, int64_t at) const
-    {
-        cplx<double> v = x[q][at];
-        if (d[q]) {
-            const cplx<TD> e = d[q][at];
-            v = v + cplx<double>{(double)e.re, (double)e.im};
-        }
-        return v;
     }
 };
 
@@ -125,8 +117,11 @@ __device__ __forceinline__ void hop_chunk(const cplx<RL> *__restrict__ u, const 
 #pragma unroll
             for (int c = 0; c < 3; ++c) vp[c] = ld(q, (int64_t)c * g.vol + sp);
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                hop[q][c] = hop[q][c] + up[3 * c] * vp[0] + up[3 * c + 1] * vp[1] + up[3 * c + 2] * vp[2];
+            for (int c = 0; c < 3; ++c) {
+                mac(hop[q][c], up[3 * c], vp[0]);
+                mac(hop[q][c], up[3 * c + 1], vp[1]);
+                mac(hop[q][c], up[3 * c + 2], vp[2]);
+            }
         }
 #pragma unroll
         for (int e = 0; e < 9; ++e) {
@@ -140,8 +135,11 @@ __device__ __forceinline__ void hop_chunk(const cplx<RL> *__restrict__ u, const 
 #pragma unroll
             for (int c = 0; c < 3; ++c) vm[c] = ld(q, (int64_t)c * g.vol + sm);
 #pragma unroll
-            for (int c = 0; c < 3; ++c)
-                hop[q][c] = hop[q][c] + cmul(up[c], vm[0]) + cmul(up[3 + c], vm[1]) + cmul(up[6 + c], vm[2]);
+            for (int c = 0; c < 3; ++c) {
+                mac_cn(hop[q][c], up[c], vm[0]);
+                mac_cn(hop[q][c], up[3 + c], vm[1]);
+                mac_cn(hop[q][c], up[6 + c], vm[2]);
+            }
         }
     }
 }
@@ -174,22 +172,21 @@ __device__ __forceinline__ void chunk_partials(double (&v)[NC], int r0, int nf, 
 
 // out_r = in_r + beta_r K in_r for the active slots.
 template <int NC, typename RL, typename T>
-__global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u, const cplx<T> *__restrict__ in,
-                                                 cplx<T> *__restrict__ out, const cplx<double> *__restrict__ beta,
-                                                 const int *__restrict__ active, int nf, geom g, lf_map mp)
+__global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u, slots<T> in, slots<T> out,
+                                                 const cplx<double> *__restrict__ beta, const int *__restrict__ active,
+                                                 int nf, geom g, lf_map mp)
 {
     lsite p;
     int chunk;
     if (!lf_site(mp, g, p, chunk)) return;
     const int64_t s = p.s;
     const int r0 = chunk * NC;
-    const int64_t fs = 3 * g.vol;
     bool on[NC];
     ld_fields<NC, T, T> ld;
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         on[q] = (r0 + q < nf) && active[r0 + q];
-        ld.f[q] = in + (int64_t)(on[q] ? r0 + q : 0) * fs;
+        ld.f[q] = in.p[on[q] ? r0 + q : 0];
     }
     cplx<T> hop[NC][3];
     hop_chunk<NC, T>(u, ld, on, p, g, hop);
@@ -198,25 +195,23 @@ __global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u,
     for (int q = 0; q < NC; ++q) {
         if (!on[q]) continue;
         const cplx<T> bt = to<T>(beta[r0 + q]);
-        cplx<T> *o = out + (int64_t)(r0 + q) * fs + s;
+        cplx<T> *o = out.p[r0 + q] + s;
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const cplx<T> v0 = ld(q, (int64_t)c * g.vol + s);
-            o[(int64_t)c * g.vol] = v0 + bt * (dsum * v0 - T(0.5) * hop[q][c]);
+            store_nt(o + (int64_t)c * g.vol, v0 + bt * (dsum * v0 - T(0.5) * hop[q][c]));
         }
     }
 }
 
-// Right-hand side and initial guess of a CN step, for slot r and field
-// f = fmap[r] of psi:
-//   y_r = psi_f + alpha_r K psi_f                     (S psi, nrqcd.f90:1112)
-//   x_r = psi_f + g_r K psi_f  (guess != 0)  or  x_r = 0
+// Right-hand side and initial guess of a CN step for slot r:
+//   y_r = psi_r + alpha_r K psi_r                     (S psi, nrqcd.f90:1112)
+//   x_r = psi_r + g_r K psi_r  (guess != 0)  or  x_r = 0
 // with g_r = alpha_r - beta_r the first-order expansion of
 // (1 + beta K)^-1 (1 + alpha K).  Partial ||y_r||^2 per site block.
 template <int NC, typename RL>
-__global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, const cplx<double> *__restrict__ psi,
-                                                const int *__restrict__ fmap, cplx<double> *__restrict__ y,
-                                                cplx<double> *__restrict__ x, const cplx<double> *__restrict__ alpha,
+__global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, slots<double> psi, slots<double> y,
+                                                slots<double> x, const cplx<double> *__restrict__ alpha,
                                                 const cplx<double> *__restrict__ gco, int guess, int nf, geom g, lf_map mp,
                                                 double *__restrict__ partial)
 {
@@ -226,13 +221,12 @@ __global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, 
     const bool valid = lf_site(mp, g, p, chunk, t);
     const int64_t s = p.s;
     const int r0 = chunk * NC;
-    const int64_t fs = 3 * g.vol;
     bool on[NC];
     ld_fields<NC, double, double> ld;
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         on[q] = valid && (r0 + q < nf);
-        ld.f[q] = psi + (int64_t)(on[q] ? fmap[r0 + q] : 0) * fs;
+        ld.f[q] = psi.p[on[q] ? r0 + q : 0];
     }
     double nrm[NC] = {};
     if (valid) {
@@ -246,12 +240,12 @@ __global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, 
             const cplx<double> a = alpha[r], gc = gco[r];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const int64_t at = (int64_t)r * fs + (int64_t)c * g.vol + s;
-                const cplx<double> v0 = ld(q, (int64_t)c * g.vol + s);
+                const int64_t at = (int64_t)c * g.vol + s;
+                const cplx<double> v0 = ld(q, at);
                 const cplx<double> kf = dsum * v0 - 0.5 * hop[q][c];
                 const cplx<double> yv = v0 + a * kf;
-                y[at] = yv;
-                x[at] = guess ? v0 + gc * kf : cplx<double>{0.0, 0.0};
+                store_nt(y.p[r] + at, yv);
+                store_nt(x.p[r] + at, guess ? v0 + gc * kf : cplx<double>{0.0, 0.0});
                 nrm[q] += nrm2(yv);
             }
         }
@@ -259,17 +253,12 @@ __global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, 
     chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial);
 }
 
-// FP64 true residual r_r = y_r - (1 + beta_r K) z_r of z_r = x_r + d_r, where
-// the correction d_r (type TD) is added for slots with dact[r] when HAS_D.
-// With HAS_D, z is stored to xout (every slot, so xout holds the new
-// iterate); with WRITE_RES, r is stored to res as type TR.  Partial ||r_r||^2
-// per site block.
-template <int NC, typename RL, typename TD, bool HAS_D, typename TR, bool WRITE_RES>
-__global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u, const cplx<double> *__restrict__ x,
-                                                  const cplx<TD> *__restrict__ d, const int *__restrict__ dact,
-                                                  cplx<double> *__restrict__ xout, const cplx<double> *__restrict__ y,
-                                                  cplx<TR> *__restrict__ res, const cplx<double> *__restrict__ beta, int nf,
-                                                  geom g, lf_map mp, double *__restrict__ partial)
+// FP64 true residual r_r = y_r - (1 + beta_r K) x_r, stored to res as type
+// TR when WRITE_RES; partial ||r_r||^2 per site block.
+template <int NC, typename RL, typename TR, bool WRITE_RES>
+__global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u, slots<double> x, slots<double> y,
+                                                  slots<TR> res, const cplx<double> *__restrict__ beta, int nf, geom g,
+                                                  lf_map mp, double *__restrict__ partial)
 {
     lsite p = {0, 0, 0, 0};
     int64_t t = 0;
@@ -277,15 +266,12 @@ __global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u
     const bool valid = lf_site(mp, g, p, chunk, t);
     const int64_t s = p.s;
     const int r0 = chunk * NC;
-    const int64_t fs = 3 * g.vol;
     bool on[NC];
-    ld_xd<NC, TD> ld;
+    ld_fields<NC, double, double> ld;
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         on[q] = valid && (r0 + q < nf);
-        const int64_t off = (int64_t)(on[q] ? r0 + q : 0) * fs;
-        ld.x[q] = x + off;
-        ld.d[q] = (HAS_D && on[q] && dact[r0 + q]) ? d + off : nullptr;
+        ld.f[q] = x.p[on[q] ? r0 + q : 0];
     }
     double nrm[NC] = {};
     if (valid) {
@@ -299,11 +285,10 @@ __global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u
             const cplx<double> bt = beta[r];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const int64_t at = (int64_t)r * fs + (int64_t)c * g.vol + s;
-                const cplx<double> v0 = ld(q, (int64_t)c * g.vol + s);
-                const cplx<double> rr = y[at] - (v0 + bt * (dsum * v0 - 0.5 * hop[q][c]));
-                if (HAS_D) xout[at] = v0;
-                if (WRITE_RES) res[at] = {(TR)rr.re, (TR)rr.im};
+                const int64_t at = (int64_t)c * g.vol + s;
+                const cplx<double> v0 = ld(q, at);
+                const cplx<double> rr = load_nt(y.p[r] + at) - (v0 + bt * (dsum * v0 - 0.5 * hop[q][c]));
+                if (WRITE_RES) store_nt(res.p[r] + at, cplx<TR>{(TR)rr.re, (TR)rr.im});
                 nrm[q] += nrm2(rr);
             }
         }
@@ -314,28 +299,29 @@ __global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u
 // One Chebyshev step on the segment [1, 1 + beta_r kmax] of the spectrum of
 // A_r = 1 + beta_r K, fused with the operator application:
 //   x_{k+1} = rho_r (x_k + (b_r - A_r x_k) / c_r) + (1 - rho_r) x_{k-1}
-// c_r = 1 + beta_r kmax / 2 is the segment centre.  x_{k+1} is written where
-// x_{k-1} was read (`xio`).  mode 0: as above; 1: first step (rho = 1, no
-// x_{k-1}); 2: from a zero start, x_k = b / c implicit (read from b) and
-// x_{k-1} = 0; 3: x_{k-1} = b / c implicit.
+// c_r = 1 + beta_r kmax / 2 is the segment centre.  mode 0: as above, x_{k-1}
+// read from xprev; 1: first step (rho = 1, no x_{k-1}); 2: from a zero start,
+// x_k = b / c implicit (xk is b) and x_{k-1} = 0; 3: x_{k-1} = b / c implicit.
+// x_{k+1} goes to xnext, or with fin to fout = fin + x_{k+1} in FP64 (the
+// last step of a correction solve adds the correction to the FP64 iterate).
 template <int NC, typename RL, typename T>
-__global__ __launch_bounds__(SBS) void k_cheb(const cplx<RL> *__restrict__ u, const cplx<T> *__restrict__ xk,
-                                              cplx<T> *__restrict__ xio, const cplx<T> *__restrict__ b,
-                                              const cplx<double> *__restrict__ beta, const cplx<double> *__restrict__ coef,
-                                              const int *__restrict__ active, int nf, geom g, lf_map mp, int mode)
+__global__ __launch_bounds__(SBS) void k_cheb(const cplx<RL> *__restrict__ u, slots<T> xk, slots<T> xprev, slots<T> xnext,
+                                              slots<T> b, const cplx<double> *__restrict__ beta,
+                                              const cplx<double> *__restrict__ coef, const int *__restrict__ active, int nf,
+                                              geom g, lf_map mp, int mode, int fin, slots<double> fin_in,
+                                              slots<double> fin_out)
 {
     lsite p;
     int chunk;
     if (!lf_site(mp, g, p, chunk)) return;
     const int64_t s = p.s;
     const int r0 = chunk * NC;
-    const int64_t fs = 3 * g.vol;
     bool on[NC];
     ld_fields<NC, T, T> ld;
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         on[q] = (r0 + q < nf) && active[r0 + q];
-        ld.f[q] = (mode == 2 ? b : xk) + (int64_t)(on[q] ? r0 + q : 0) * fs;
+        ld.f[q] = xk.p[on[q] ? r0 + q : 0];
     }
     cplx<T> hop[NC][3];
     hop_chunk<NC, T>(u, ld, on, p, g, hop);
@@ -350,16 +336,21 @@ __global__ __launch_bounds__(SBS) void k_cheb(const cplx<RL> *__restrict__ u, co
         const cplx<T> one_m_rho = {T(1) - rho.re, -rho.im};
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const int64_t at = (int64_t)r * fs + (int64_t)c * g.vol + s;
-            const cplx<T> v0 = ld(q, (int64_t)c * g.vol + s);
+            const int64_t at = (int64_t)c * g.vol + s;
+            const cplx<T> v0 = ld(q, at);
             const cplx<T> kf = dsum * v0 - T(0.5) * hop[q][c];
-            const cplx<T> bs = mode == 2 ? v0 : b[at];
+            const cplx<T> bs = mode == 2 ? v0 : load_nt(b.p[r] + at);
             const cplx<T> x0 = mode == 2 ? ic * v0 : v0;
             const cplx<T> ax = mode == 2 ? ic * kf : kf;
-            const cplx<T> nx = rho * (x0 + ic * (bs - (x0 + bt * ax)));
-            if (mode == 1 || mode == 2) xio[at] = nx;
-            else if (mode == 3) xio[at] = nx + one_m_rho * (ic * bs);
-            else xio[at] = nx + one_m_rho * xio[at];
+            cplx<T> nx = rho * (x0 + ic * (bs - (x0 + bt * ax)));
+            if (mode == 3) nx = nx + one_m_rho * (ic * bs);
+            else if (mode == 0) nx = nx + one_m_rho * load_nt(xprev.p[r] + at);
+            if (fin) {
+                const cplx<double> xo = load_nt(fin_in.p[r] + at);
+                store_nt(fin_out.p[r] + at, cplx<double>{xo.re + (double)nx.re, xo.im + (double)nx.im});
+            } else {
+                store_nt(xnext.p[r] + at, nx);
+            }
         }
     }
 }
@@ -542,15 +533,13 @@ __global__ void k_links_to_f32(const cplx<double> *__restrict__ u, cplx<float> *
         v[k] = {(float)u[k].re, (float)u[k].im};
 }
 
-// psi_{fmap[r]} = x_r
-__global__ void k_scatter(const cplx<double> *__restrict__ x, const int *__restrict__ fmap, cplx<double> *__restrict__ psi,
-                          int64_t n)
+// dst_r = src_r (n complex numbers per slot)
+__global__ void k_copy_slots(slots<double> src, slots<double> dst, int64_t n)
 {
     const int r = blockIdx.y;
-    cplx<double> *dst = psi + (int64_t)fmap[r] * n;
-    const cplx<double> *src = x + (int64_t)r * n;
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
-        dst[k] = src[k];
+    const cplx<double> *a = src.p[r];
+    cplx<double> *d = dst.p[r];
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) d[k] = a[k];
 }
 
 // ---------------------------------------------------------------------------
@@ -567,7 +556,9 @@ struct workspace {
     cplx<double> *beta = nullptr, *cpar = nullptr, *hdev = nullptr, *ydev = nullptr, *cpart = nullptr;
     cplx<double> *cheb = nullptr;      // Chebyshev coefficients [iteration][slot][2]
     double *rpart = nullptr, *spart = nullptr, *nrm = nullptr;
-    int *active = nullptr, *nused = nullptr, *fmap = nullptr, *dact = nullptr;
+    int *active = nullptr, *nused = nullptr;
+    cplx<double> *psi = nullptr;       // heavy-field storage of the current CN step
+    int fields[NSLOTS] = {};           // its field of each slot
     cplx<double> *hbuf = nullptr;      // pinned host
     double *nbuf = nullptr;            // pinned host (NRM doubles)
 
@@ -578,7 +569,7 @@ struct workspace {
     void release()
     {
         void *dev[] = {basis, y, x, t, res, y32, x32, t32, res32, links32, beta, cpar, hdev, ydev, cpart,
-                       rpart, spart, nrm, active, nused, fmap, dact, cheb};
+                       rpart, spart, nrm, active, nused, cheb};
         for (void *q : dev)
             if (q) (void)hipFree(q);
         if (hbuf) (void)hipHostFree(hbuf);
@@ -592,7 +583,8 @@ struct workspace {
         release();
         const size_t f64 = (size_t)3 * (size_t)v * sizeof(cplx<double>) * NSLOTS;
         const size_t f32 = f64 / 2;
-        const int64_t sb = std::max(make_lf_map(lat().g, SBS, 8, 1, 1).nsb, make_lf_map(lat().g, SBS, 8, 0, 1).nsb);
+        int64_t sb = 0;
+        for (int md = 0; md <= 2; ++md) sb = std::max(sb, make_lf_map(lat().g, SBS, 8, md, 1).nsb);
         auto dm = [](void **p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
         bool ok = dm(&basis, f64 * (size_t)(mm + 1)) && dm((void **)&y, f64) && dm((void **)&x, f64) &&
                   dm((void **)&t, f64) && dm((void **)&res, f64) && dm((void **)&y32, f32) &&
@@ -606,7 +598,6 @@ struct workspace {
                   dm((void **)&rpart, sizeof(double) * NSLOTS * VBLOCKS * 8) &&
                   dm((void **)&spart, sizeof(double) * NSLOTS * (size_t)sb) && dm((void **)&nrm, sizeof(double) * NRM) &&
                   dm((void **)&active, sizeof(int) * NSLOTS) && dm((void **)&nused, sizeof(int) * NSLOTS) &&
-                  dm((void **)&fmap, sizeof(int) * NSLOTS) && dm((void **)&dact, sizeof(int) * NSLOTS) &&
                   hipHostMalloc((void **)&hbuf, sizeof(cplx<double>) * NSLOTS * (mm + 1), hipHostMallocDefault) == hipSuccess &&
                   hipHostMalloc((void **)&nbuf, sizeof(double) * NRM, hipHostMallocDefault) == hipSuccess;
         if (!ok) {
@@ -638,7 +629,7 @@ static int upload_active(const std::vector<int> &act) { return upload_ints(g_ws.
 // link load) for FP64 and for FP32 arithmetic, and the site-block mapping of
 // make_lf_map.  NRQ_CN_SHAPE="nc64,nc32,mode,xcd" overrides the defaults.
 struct cn_shape {
-    int nc64 = 2, nc32 = 4, mode = 1, xcd = 8;
+    int nc64 = 2, nc32 = 4, mode = 0, xcd = 8;
 };
 static cn_shape read_shape()
 {
@@ -649,7 +640,7 @@ static cn_shape read_shape()
             auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
             if (ok(a)) sh.nc64 = a;
             if (ok(b)) sh.nc32 = b;
-            if (c == 0 || c == 1) sh.mode = c;
+            if (c >= 0 && c <= 2) sh.mode = c;
             if (d == 1 || d == 8) sh.xcd = d;
         }
     }
@@ -685,6 +676,23 @@ static lf_map cn_map(int nf, int nc)
 
 static unsigned grid_of(const lf_map &mp) { return (unsigned)(mp.ngroup * mp.nsb); }
 
+// slot r -> base + r * 3V (workspace fields)
+template <typename T> static slots<T> ws_slots(cplx<T> *base, int R)
+{
+    slots<T> t = {};
+    for (int r = 0; r < R; ++r) t.p[r] = base + (int64_t)r * 3 * g_ws.vol;
+    return t;
+}
+template <typename T> static slots<T> ws_slots(const cplx<T> *base, int R) { return ws_slots((cplx<T> *)base, R); }
+
+// slot r -> the heavy field of the current CN step
+static slots<double> psi_slots(int R)
+{
+    slots<double> t = {};
+    for (int r = 0; r < R; ++r) t.p[r] = g_ws.psi + (int64_t)g_ws.fields[r] * 3 * g_ws.vol;
+    return t;
+}
+
 // out[r] = sum of the nblk partials of slot r (r < nf; active may be null)
 static int fold_slots(const double *partial, int64_t nblk, int nf, const int *active, double *out)
 {
@@ -706,7 +714,8 @@ static int apply_t(const cplx<RL> *u, const cplx<T> *in, cplx<T> *out, int nf)
 {
     const int nc = nc_of<T>();
     const lf_map mp = cn_map(nf, nc);
-    NC_DISPATCH(nc, hipLaunchKernelGGL((k_stencil<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, in, out,
+    const slots<T> ti = ws_slots(in, nf), to_ = ws_slots(out, nf);
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_stencil<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, ti, to_,
                                        g_ws.beta, g_ws.active, nf, lat().g, mp));
     NRQ_HIP(hipGetLastError());
     return NRQ_OK;
@@ -936,10 +945,12 @@ static int cheb_table(int R, const std::vector<zc> &beta, int k, double kmax)
 // k Chebyshev steps on A x = b for the active slots.  zero_start: x starts at
 // 0 and steps 0 and 1 are folded into the first launch (k >= 2); otherwise
 // from the iterate in `x`.  x and xalt alternate; *out receives the buffer
-// holding x_k.
+// holding x_k.  With fin_in/fin_out the last step instead stores
+// fin_out = fin_in + x_k in FP64.
 template <typename RL, typename T>
 static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, cplx<T> *xalt, bool zero_start,
-                    const std::vector<zc> &beta, int k, double kmax, cplx<T> **out)
+                    const std::vector<zc> &beta, int k, double kmax, cplx<T> **out, const slots<double> *fin_in = nullptr,
+                    const slots<double> *fin_out = nullptr)
 {
     if (zero_start && k < 2) k = 2;
     if (k > CHEB_MAX) k = CHEB_MAX;
@@ -947,15 +958,19 @@ static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, 
     const geom g = lat().g;
     const int nc = nc_of<T>();
     const lf_map mp = cn_map(R, nc);
+    const slots<T> tb = ws_slots(b, R);
+    const slots<double> none = {};
     cplx<T> *cur = x, *other = xalt;
     for (int j = zero_start ? 1 : 0; j < k; ++j) {
         const int mode = zero_start ? (j == 1 ? 2 : (j == 2 ? 3 : 0)) : (j == 0 ? 1 : 0);
-        // mode 2 reads b only and writes cur; the other modes read cur and
-        // write other
-        cplx<T> *dst = mode == 2 ? cur : other;
+        // mode 2 reads b and writes cur; the other modes read cur and write other
+        const slots<T> txk = mode == 2 ? tb : ws_slots(cur, R);
+        const slots<T> tnext = ws_slots(mode == 2 ? cur : other, R);
+        const int fin = (fin_in && j == k - 1) ? 1 : 0;
+        const slots<double> fi = fin ? *fin_in : none, fo = fin ? *fin_out : none;
         const cplx<double> *cf = g_ws.cheb + (size_t)2 * NSLOTS * j;
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cheb<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), links,
-                                           (const cplx<T> *)cur, dst, b, g_ws.beta, cf, g_ws.active, R, g, mp, mode));
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cheb<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), links, txk, tnext,
+                                           tnext, tb, g_ws.beta, cf, g_ws.active, R, g, mp, mode, fin, fi, fo));
         if (mode != 2) std::swap(cur, other);
     }
     NRQ_HIP(hipGetLastError());
@@ -963,30 +978,26 @@ static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, 
     return NRQ_OK;
 }
 
-// FP64 true residual of x (+ d for the slots with dact) with its norms into
-// g_ws.nrm[0, R); with d, the updated iterate goes to xout; res32 (may be
-// null) receives the residual in FP32.
+// FP64 true residual of the slot fields x with its norms into g_ws.nrm[0, R);
+// res32 (may be null) receives the residual in FP32.
 template <typename RL>
-static int cn_resid(const cplx<RL> *u, int R, const cplx<double> *x, const cplx<float> *d, cplx<double> *xout,
-                    cplx<float> *res32)
+static int cn_resid(const cplx<RL> *u, int R, const slots<double> &x, cplx<float> *res32)
 {
     const int nc = shape().nc64;
     const lf_map mp = cn_map(R, nc);
     const geom g = lat().g;
-    const cplx<double> *y = g_ws.y, *bt = g_ws.beta;
-    const int *da = g_ws.dact;
+    const slots<double> ty = ws_slots(g_ws.y, R);
+    const cplx<double> *bt = g_ws.beta;
     double *sp = g_ws.spart;
-    if (d)
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, true, float, true>), dim3(grid_of(mp)), dim3(SBS), 0,
-                                           stream(), u, x, d, da, xout, y, res32, bt, R, g, mp, sp));
-    else if (res32)
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, false, float, true>), dim3(grid_of(mp)), dim3(SBS), 0,
-                                           stream(), u, x, (const cplx<float> *)nullptr, da, (cplx<double> *)nullptr, y, res32,
-                                           bt, R, g, mp, sp));
-    else
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, false, float, false>), dim3(grid_of(mp)), dim3(SBS), 0,
-                                           stream(), u, x, (const cplx<float> *)nullptr, da, (cplx<double> *)nullptr, y,
-                                           (cplx<float> *)nullptr, bt, R, g, mp, sp));
+    if (res32) {
+        const slots<float> tr = ws_slots(res32, R);
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, true>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u,
+                                           x, ty, tr, bt, R, g, mp, sp));
+    } else {
+        const slots<float> tr = {};
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, false>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u,
+                                           x, ty, tr, bt, R, g, mp, sp));
+    }
     NRQ_HIP(hipGetLastError());
     return fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm);
 }
@@ -1012,12 +1023,13 @@ static bool check_slots(int R, const std::vector<double> &bn, double tol, std::v
     return any;
 }
 
-// Chebyshev solve from the guess in g_ws.x.  *xfinal receives the buffer
-// holding the iterate.  Slots still active on return go on with GMRES.
+// Chebyshev solve from the guess in g_ws.x.  On return *in_psi tells whether
+// the iterate is already in the heavy field (else in *xfinal).  Slots still
+// active on return go on with GMRES.
 template <typename RL>
 static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, const std::vector<zc> &beta,
                       std::vector<double> &bn, const krylov_params &p, std::vector<int> &act, std::vector<double> &rel,
-                      std::vector<int64_t> &its, int64_t &cycles, cplx<double> **xfinal)
+                      std::vector<int64_t> &its, int64_t &cycles, cplx<double> **xfinal, bool *in_psi)
 {
     const double kmax = cheb_kmax();
     std::vector<zc> sig(R);
@@ -1037,6 +1049,7 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
     auto bn_from_host = [&]() {
         for (int r = 0; r < R; ++r) bn[r] = std::sqrt(std::max(g_ws.nbuf[32 + r], 0.0));
     };
+    *in_psi = false;
     if (p.kind == KRYLOV_CHEB_FP64) {
         // the first round takes enough steps for the a priori guess bound
         for (int r = 0; r < R; ++r) need[r] = 0.3 * p.tol / b0[r];
@@ -1048,7 +1061,7 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
             if (int e = cheb_run<RL, double>(u, R, g_ws.y, x, x == g_ws.x ? g_ws.t : g_ws.x, false, beta, k, kmax, &xk)) return e;
             x = xk;
             for (int r = 0; r < R; ++r) its[r] += act[r] ? k : 0;
-            if (int e = cn_resid<RL>(u, R, x, nullptr, nullptr, nullptr)) return e;
+            if (int e = cn_resid<RL>(u, R, ws_slots(x, R), nullptr)) return e;
             if (int e = nrm_to_host()) return e;
             if (round == 0) bn_from_host();
             if (!check_slots(R, bn, p.tol, act, rel, need)) break;
@@ -1057,7 +1070,9 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
         *xfinal = x;
         return NRQ_OK;
     }
-    // mixed: FP64 residual -> FP32 Chebyshev correction from zero -> x += d
+    // mixed: FP64 residual -> FP32 Chebyshev correction d from zero whose
+    // last step stores x + d into the heavy field (from the guess the first
+    // time, in place afterwards)
     const cplx<float> *l32;
     if (sizeof(RL) == 8) {
         hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)u, g_ws.links32, 27 * g_ws.vol);
@@ -1066,8 +1081,16 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
     } else {
         l32 = (const cplx<float> *)u;
     }
-    cplx<double> *xcur = g_ws.x, *xalt = g_ws.t;
-    if (int e = cn_resid<RL>(u, R, xcur, nullptr, nullptr, g_ws.y32)) return e;
+    const slots<double> tpsi = psi_slots(R), tx = ws_slots(g_ws.x, R);
+    if (int e = cn_resid<RL>(u, R, tx, g_ws.y32)) return e;
+    auto correct = [&](int k) -> int {
+        cplx<float> *d = nullptr;
+        const slots<double> &from = *in_psi ? tpsi : tx;
+        if (int e = cheb_run<float, float>(l32, R, g_ws.y32, g_ws.x32, g_ws.t32, true, beta, k, kmax, &d, &from, &tpsi))
+            return e;
+        *in_psi = true;
+        return cn_resid<RL>(u, R, tpsi, g_ws.y32);
+    };
     // corrections planned from the a priori bound, without synchronisation
     std::vector<double> bnd = b0;
     for (int plan = 0; plan < 3; ++plan) {
@@ -1079,10 +1102,7 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
         if (!more) break;
         const int k = std::max(2, steps_for(3e-7));
         ++cycles;
-        cplx<float> *d = nullptr;
-        if (int e = cheb_run<float, float>(l32, R, g_ws.y32, g_ws.x32, g_ws.t32, true, beta, k, kmax, &d)) return e;
-        if (int e = cn_resid<RL>(u, R, xcur, d, xalt, g_ws.y32)) return e;
-        std::swap(xcur, xalt);
+        if (int e = correct(k)) return e;
         for (int r = 0; r < R; ++r) {
             its[r] += k;
             bnd[r] *= cheb_bound(sig[r], k) + FP32_CORR_ERR;
@@ -1090,21 +1110,22 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
     }
     if (int e = nrm_to_host()) return e;
     bn_from_host();
-    // measured corrections until converged
+    if (!*in_psi) {
+        // no correction was needed: the guess goes to the heavy field
+        hipLaunchKernelGGL(k_copy_slots, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), tx, tpsi, 3 * g_ws.vol);
+        NRQ_HIP(hipGetLastError());
+        *in_psi = true;
+    }
+    // measured corrections (in place) until converged
     for (int round = 0; round < 8; ++round) {
         if (!check_slots(R, bn, p.tol, act, rel, need)) break;
         if (int e = upload_active(act)) return e;
-        if (int e = upload_ints(g_ws.dact, act)) return e;
         const int k = std::max(2, steps_for(3e-7));
         ++cycles;
-        cplx<float> *d = nullptr;
-        if (int e = cheb_run<float, float>(l32, R, g_ws.y32, g_ws.x32, g_ws.t32, true, beta, k, kmax, &d)) return e;
-        if (int e = cn_resid<RL>(u, R, xcur, d, xalt, g_ws.y32)) return e;
-        std::swap(xcur, xalt);
+        if (int e = correct(k)) return e;
         for (int r = 0; r < R; ++r) its[r] += act[r] ? k : 0;
         if (int e = nrm_to_host()) return e;
     }
-    *xfinal = xcur;
     return NRQ_OK;
 }
 
@@ -1126,11 +1147,11 @@ static int gmres_mixed(const cplx<RL> *u, int R, std::vector<double> &bn, const 
     std::vector<double> one(R, 1.0);
     const int nc = shape().nc64;
     const lf_map mp = cn_map(R, nc);
+    const slots<double> tx = ws_slots(g_ws.x, R), ty = ws_slots(g_ws.y, R), tres = ws_slots(g_ws.res, R);
     for (int outer = 0; outer < 40; ++outer) {
         // FP64 true residual r = y - A x
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, false, double, true>), dim3(grid_of(mp)), dim3(SBS), 0,
-                                           stream(), u, (const cplx<double> *)g_ws.x, (const cplx<float> *)nullptr, g_ws.dact,
-                                           (cplx<double> *)nullptr, g_ws.y, g_ws.res, g_ws.beta, R, lat().g, mp, g_ws.spart));
+        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, double, true>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u,
+                                           tx, ty, tres, g_ws.beta, R, lat().g, mp, g_ws.spart));
         NRQ_HIP(hipGetLastError());
         if (int e = fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm)) return e;
         if (int e = nrm_to_host()) return e;
@@ -1181,21 +1202,27 @@ static int cn_step_t(const cplx<RL> *u, cplx<double> *psi, int R, const std::vec
     const int nc = shape().nc64;
     const lf_map mp = cn_map(R, nc);
     const int gs = guess ? 1 : 0;
-    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u,
-                                       (const cplx<double> *)psi, g_ws.fmap, g_ws.y, g_ws.x, g_ws.cpar, g_ws.cpar + NSLOTS, gs,
-                                       R, lat().g, mp, g_ws.spart));
+    const slots<double> tpsi = psi_slots(R), ty = ws_slots(g_ws.y, R), tx = ws_slots(g_ws.x, R);
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, tpsi, ty, tx,
+                                       g_ws.cpar, g_ws.cpar + NSLOTS, gs, R, lat().g, mp, g_ws.spart));
     NRQ_HIP(hipGetLastError());
     if (int e = fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm + 32)) return e;
     cplx<double> *xsol = g_ws.x;
+    bool in_psi = false;
     int status = NRQ_OK;
     if (p.kind == KRYLOV_CHEB_FP64 || p.kind == KRYLOV_CHEB_MIXED) {
-        if (int e = cheb_solve<RL>(u, R, alpha, beta, bn, p, act, rel, its, cycles, &xsol)) return e;
+        if (int e = cheb_solve<RL>(u, R, alpha, beta, bn, p, act, rel, its, cycles, &xsol, &in_psi)) return e;
         bool left = false;
         for (int r = 0; r < R; ++r) left = left || act[r];
         if (left) {
             // spectrum bound violated (e.g. links far from unitary): GMRES from here
-            if (xsol != g_ws.x)
+            if (in_psi) {
+                hipLaunchKernelGGL(k_copy_slots, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), tpsi, tx, n);
+                NRQ_HIP(hipGetLastError());
+                in_psi = false;
+            } else if (xsol != g_ws.x) {
                 NRQ_HIP(hipMemcpyAsync(g_ws.x, xsol, sizeof(cplx<double>) * (size_t)n * R, hipMemcpyDeviceToDevice, stream()));
+            }
             xsol = g_ws.x;
             status = gmres_core<double, RL>(u, R, g_ws.y, g_ws.x, (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, false,
                                             bn.data(), act, p.tol, p.restart, p.maxit, its, rel.data(), cycles);
@@ -1210,9 +1237,11 @@ static int cn_step_t(const cplx<RL> *u, cplx<double> *psi, int R, const std::vec
             status = gmres_mixed<RL>(u, R, bn, p, act, rel, its, cycles);
     }
     if (status) return status;
-    // psi_f = x for every slot
-    hipLaunchKernelGGL(k_scatter, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<double> *)xsol, g_ws.fmap, psi, n);
-    NRQ_HIP(hipGetLastError());
+    if (!in_psi) {
+        // psi_f = x for every slot
+        hipLaunchKernelGGL(k_copy_slots, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), ws_slots(xsol, R), tpsi, n);
+        NRQ_HIP(hipGetLastError());
+    }
     double worst = 0;
     int64_t total = 0;
     for (int r = 0; r < R; ++r) {
@@ -1236,19 +1265,20 @@ int krylov_cn_step(const nrq_gauge *gf, cplx<double> *psi, const int *fields, in
     if (R <= 0) return NRQ_OK;
     if (R > NSLOTS) return fail(NRQ_ERR_ARG, "CN: %d fields exceed the batch of %d", R, NSLOTS);
     if (g_ws.vol != lat().g.vol) return fail(NRQ_ERR_STATE, "CN: workspace not reserved for this lattice");
-    std::vector<int> fm(fields, fields + R), act(R, 1);
+    std::vector<int> act(R, 1);
     std::vector<cplx<double>> par(2 * NSLOTS, cplx<double>{0.0, 0.0});
     std::vector<zc> a(R), b(R);
     for (int r = 0; r < R; ++r) {
+        if (fields[r] < 0 || fields[r] >= NSLOTS) return fail(NRQ_ERR_ARG, "CN: field %d out of range", fields[r]);
+        g_ws.fields[r] = fields[r];
         a[r] = zc(alpha[r].re, alpha[r].im);
         b[r] = zc(beta[r].re, beta[r].im);
         par[r] = alpha[r];
         const zc g = a[r] - b[r];
         par[NSLOTS + r] = {g.real(), g.imag()};
     }
-    if (int e = upload_ints(g_ws.fmap, fm)) return e;
+    g_ws.psi = psi;
     if (int e = upload_active(act)) return e;
-    if (int e = upload_ints(g_ws.dact, act)) return e;
     NRQ_HIP(hipMemcpyAsync(g_ws.cpar, par.data(), sizeof(cplx<double>) * par.size(), hipMemcpyHostToDevice, stream()));
     NRQ_HIP(hipMemcpyAsync(g_ws.beta, beta, sizeof(cplx<double>) * R, hipMemcpyHostToDevice, stream()));
     if (gf->precision == 64) return cn_step_t<double>((const cplx<double> *)gauge_links(gf), psi, R, a, b, p, st);

@@ -20,9 +20,11 @@ struct lf_map {
     int64_t nsb; // site blocks (mode 0) or tiles (mode 1)
     int bs;      // threads per block
     int xcd;     // XCD remap factor (1: none)
-    int mode;    // 0: bs consecutive sites; 1: z-sweep tiles
-    int ty;      // mode 1: y rows per tile
+    int mode;    // 0: bs consecutive sites; 1: z-sweep tiles of full x-rows; 2: z-sweep tx x ty tiles
+    int ty;      // modes 1, 2: y rows per tile
     int ngroup;  // work items per site block
+    int tx;      // mode 2: x extent of a tile
+    int nxb;     // mode 2: tiles along x
 };
 
 __device__ __forceinline__ unsigned lf_logical_block(const lf_map &m)
@@ -51,11 +53,18 @@ __device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p
         return true;
     }
     const unsigned n2 = (unsigned)g.n2;
-    const unsigned yb = tt / n2;
-    const int z = (int)(tt - yb * n2);
-    const int x = (int)threadIdx.x % g.n0, yy = (int)threadIdx.x / g.n0;
-    const int y = (int)yb * m.ty + yy;
-    if (yy >= m.ty || y >= g.n1) return false;
+    const unsigned col = tt / n2;
+    const int z = (int)(tt - col * n2);
+    if (m.mode == 1) {
+        const int x = (int)threadIdx.x % g.n0, yy = (int)threadIdx.x / g.n0;
+        const int y = (int)col * m.ty + yy;
+        if (yy >= m.ty || y >= g.n1) return false;
+        p = lsite{x + (int64_t)g.n0 * (y + (int64_t)g.n1 * z), x, y, z};
+        return true;
+    }
+    const unsigned yb = col / (unsigned)m.nxb, xb = col - yb * (unsigned)m.nxb;
+    const int x = (int)xb * m.tx + (int)threadIdx.x % m.tx, y = (int)yb * m.ty + (int)threadIdx.x / m.tx;
+    if (x >= g.n0 || y >= g.n1) return false;
     p = lsite{x + (int64_t)g.n0 * (y + (int64_t)g.n1 * z), x, y, z};
     return true;
 }
@@ -66,15 +75,29 @@ __device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p
     return lf_site(m, g, p, item, t);
 }
 
-// Host: map for `bs` threads per block; tile mode needs n0 <= bs.
+// Host: map for `bs` threads per block.  Mode 1 needs n0 <= bs; mode 2
+// uses 16-wide tiles (bs a multiple of 16); otherwise consecutive sites.
 inline lf_map make_lf_map(const geom &g, int bs, int xcd, int mode, int ngroup)
 {
+    lf_map m{};
+    m.bs = bs;
+    m.xcd = xcd;
+    m.ngroup = ngroup;
     if (mode == 1 && g.n0 <= bs) {
-        const int ty = bs / g.n0 < g.n1 ? bs / g.n0 : g.n1;
-        const int64_t nyb = (g.n1 + ty - 1) / ty;
-        return lf_map{nyb * g.n2, bs, xcd, 1, ty, ngroup};
+        m.mode = 1;
+        m.ty = bs / g.n0 < g.n1 ? bs / g.n0 : g.n1;
+        m.nsb = (int64_t)((g.n1 + m.ty - 1) / m.ty) * g.n2;
+    } else if (mode == 2 && bs % 16 == 0 && g.n0 >= 16) {
+        m.mode = 2;
+        m.tx = 16;
+        m.ty = bs / 16;
+        m.nxb = (g.n0 + 15) / 16;
+        m.nsb = (int64_t)m.nxb * ((g.n1 + m.ty - 1) / m.ty) * g.n2;
+    } else {
+        m.mode = 0;
+        m.nsb = (g.vol + bs - 1) / bs;
     }
-    return lf_map{(g.vol + bs - 1) / bs, bs, xcd, 0, 0, ngroup};
+    return m;
 }
 
 } // namespace nrq

@@ -39,6 +39,30 @@ template <typename R> __device__ __forceinline__ cplx<R> cmul(cplx<R> a, cplx<R>
     return {a.re * b.re + a.im * b.im, a.re * b.im - a.im * b.re};
 }
 
+// Streaming (nontemporal) access for data touched once per pass, so that it
+// does not displace reused data from the caches.
+template <typename R> struct vec2_of;
+template <> struct vec2_of<double> {
+    typedef double type __attribute__((ext_vector_type(2)));
+};
+template <> struct vec2_of<float> {
+    typedef float type __attribute__((ext_vector_type(2)));
+};
+template <typename R> __device__ __forceinline__ void store_nt(cplx<R> *p, cplx<R> v)
+{
+    typedef typename vec2_of<R>::type V;
+    V w;
+    w.x = v.re;
+    w.y = v.im;
+    __builtin_nontemporal_store(w, (V *)p);
+}
+template <typename R> __device__ __forceinline__ cplx<R> load_nt(const cplx<R> *p)
+{
+    typedef typename vec2_of<R>::type V;
+    const V w = __builtin_nontemporal_load((const V *)p);
+    return {w.x, w.y};
+}
+
 template <typename R> struct mat3 {
     cplx<R> m[9];
 };
@@ -49,6 +73,44 @@ template <typename R> __device__ __forceinline__ void set_zero(mat3<R> &a)
     for (int k = 0; k < 9; ++k) a.m[k] = {R(0), R(0)};
 }
 
+// Complex multiply-accumulate as four FMAs (the plain s + a * b form
+// cannot be contracted without reassociation and costs six instructions).
+__device__ __forceinline__ double fmx(double a, double b, double c) { return __builtin_fma(a, b, c); }
+__device__ __forceinline__ float fmx(float a, float b, float c) { return __builtin_fmaf(a, b, c); }
+
+// s += a b
+template <typename R> __device__ __forceinline__ void mac(cplx<R> &s, cplx<R> a, cplx<R> b)
+{
+    s.re = fmx(a.re, b.re, s.re);
+    s.re = fmx(-a.im, b.im, s.re);
+    s.im = fmx(a.re, b.im, s.im);
+    s.im = fmx(a.im, b.re, s.im);
+}
+// s += conj(a) b
+template <typename R> __device__ __forceinline__ void mac_cn(cplx<R> &s, cplx<R> a, cplx<R> b)
+{
+    s.re = fmx(a.re, b.re, s.re);
+    s.re = fmx(a.im, b.im, s.re);
+    s.im = fmx(a.re, b.im, s.im);
+    s.im = fmx(-a.im, b.re, s.im);
+}
+// s += a conj(b)
+template <typename R> __device__ __forceinline__ void mac_nc(cplx<R> &s, cplx<R> a, cplx<R> b)
+{
+    s.re = fmx(a.re, b.re, s.re);
+    s.re = fmx(a.im, b.im, s.re);
+    s.im = fmx(a.im, b.re, s.im);
+    s.im = fmx(-a.re, b.im, s.im);
+}
+// s += conj(a) conj(b)
+template <typename R> __device__ __forceinline__ void mac_cc(cplx<R> &s, cplx<R> a, cplx<R> b)
+{
+    s.re = fmx(a.re, b.re, s.re);
+    s.re = fmx(-a.im, b.im, s.re);
+    s.im = fmx(-a.re, b.im, s.im);
+    s.im = fmx(-a.im, b.re, s.im);
+}
+
 // C = A * B
 template <typename R> __device__ __forceinline__ mat3<R> mul(const mat3<R> &a, const mat3<R> &b)
 {
@@ -57,9 +119,10 @@ template <typename R> __device__ __forceinline__ mat3<R> mul(const mat3<R> &a, c
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            cplx<R> s = a.m[3 * r] * b.m[k];
-            s = s + a.m[3 * r + 1] * b.m[3 + k];
-            s = s + a.m[3 * r + 2] * b.m[6 + k];
+            cplx<R> s = {R(0), R(0)};
+            mac(s, a.m[3 * r], b.m[k]);
+            mac(s, a.m[3 * r + 1], b.m[3 + k]);
+            mac(s, a.m[3 * r + 2], b.m[6 + k]);
             c.m[3 * r + k] = s;
         }
     return c;
@@ -73,9 +136,10 @@ template <typename R> __device__ __forceinline__ mat3<R> mul_nd(const mat3<R> &a
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            cplx<R> s = mulc(a.m[3 * r], b.m[3 * k]);
-            s = s + mulc(a.m[3 * r + 1], b.m[3 * k + 1]);
-            s = s + mulc(a.m[3 * r + 2], b.m[3 * k + 2]);
+            cplx<R> s = {R(0), R(0)};
+            mac_nc(s, a.m[3 * r], b.m[3 * k]);
+            mac_nc(s, a.m[3 * r + 1], b.m[3 * k + 1]);
+            mac_nc(s, a.m[3 * r + 2], b.m[3 * k + 2]);
             c.m[3 * r + k] = s;
         }
     return c;
@@ -89,9 +153,10 @@ template <typename R> __device__ __forceinline__ mat3<R> mul_dn(const mat3<R> &a
     for (int r = 0; r < 3; ++r)
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
-            cplx<R> s = cmul(a.m[r], b.m[k]);
-            s = s + cmul(a.m[3 + r], b.m[3 + k]);
-            s = s + cmul(a.m[6 + r], b.m[6 + k]);
+            cplx<R> s = {R(0), R(0)};
+            mac_cn(s, a.m[r], b.m[k]);
+            mac_cn(s, a.m[3 + r], b.m[3 + k]);
+            mac_cn(s, a.m[6 + r], b.m[6 + k]);
             c.m[3 * r + k] = s;
         }
     return c;
@@ -106,9 +171,10 @@ template <typename R> __device__ __forceinline__ mat3<R> mul_dd(const mat3<R> &a
 #pragma unroll
         for (int k = 0; k < 3; ++k) {
             // sum_j conj(a[j][r]) conj(b[k][j])
-            cplx<R> s = conjg(a.m[r] * b.m[3 * k]);
-            s = s + conjg(a.m[3 + r] * b.m[3 * k + 1]);
-            s = s + conjg(a.m[6 + r] * b.m[3 * k + 2]);
+            cplx<R> s = {R(0), R(0)};
+            mac_cc(s, a.m[r], b.m[3 * k]);
+            mac_cc(s, a.m[3 + r], b.m[3 * k + 1]);
+            mac_cc(s, a.m[6 + r], b.m[3 * k + 2]);
             c.m[3 * r + k] = s;
         }
     return c;
@@ -170,25 +236,58 @@ template <typename R> __device__ __forceinline__ void two_re_trace_gen(const mat
     out[7] = s3 * (w.m[0].re + w.m[4].re - R(2) * w.m[8].re);
 }
 
-template <typename R> struct exp_eps;
-template <> struct exp_eps<double> { static constexpr double v = 1e-17; };
-template <> struct exp_eps<float> { static constexpr float v = 1e-8f; };
+// Series length and scaling threshold of the exponential: Q is scaled so
+// that its spectral radius is <= 1/4; the series then converges to below
+// one unit in the last place after `terms` terms ((1/4)^13 / 13! = 2e-18
+// for FP64, (1/4)^8 / 8! = 4e-10 for FP32).
+template <typename R> struct exp_cfg;
+template <> struct exp_cfg<double> {
+    static constexpr int terms = 12;
+};
+template <> struct exp_cfg<float> {
+    static constexpr int terms = 7;
+};
 
-// exp(i Q) for a traceless Hermitian Q by Cayley-Hamilton: Q^3 = c1 Q + c0 I
+// exp(iQ) for traceless Hermitian Q by Cayley-Hamilton:
+//   Q^3 = c0 I + c1 Q,
 // with c1 = tr(Q^2)/2, c0 = det Q, so exp(iQ) = f0 I + f1 Q + f2 Q^2 where the
 // f's are the power series sum_n i^n (a_n, b_n, c_n)/n! of the recurrence
 // (a,b,c)_{n+1} = (c0 c_n, a_n + c1 c_n, b_n).  Q is first scaled by 2^-s so
-// that its spectral radius is <= 1 (|lambda|_max^2 <= 4 c1 / 3 for traceless
-// Hermitian 3x3), and the result is squared s times.
+// that its spectral radius is <= 1/4 (|lambda|_max^2 <= 4 c1 / 3 for a
+// traceless Hermitian 3x3), and the result is squared s times.
 template <typename R> __device__ __forceinline__ mat3<R> exp_i_hermitian(mat3<R> q)
 {
-    // Q^2 (Hermitian): compute all entries from rows of Q and columns of Q
-    mat3<R> q2 = mul(q, q);
+    // Q^2 is Hermitian: real diagonal, upper triangle, lower = conjugate
+    mat3<R> q2;
+    {
+        const R d0 = q.m[0].re, d1 = q.m[4].re, d2 = q.m[8].re;
+        const R a01 = q.m[1].re * q.m[1].re + q.m[1].im * q.m[1].im;
+        const R a02 = q.m[2].re * q.m[2].re + q.m[2].im * q.m[2].im;
+        const R a12 = q.m[5].re * q.m[5].re + q.m[5].im * q.m[5].im;
+        q2.m[0] = {fmx(d0, d0, a01 + a02), R(0)};
+        q2.m[4] = {fmx(d1, d1, a01 + a12), R(0)};
+        q2.m[8] = {fmx(d2, d2, a02 + a12), R(0)};
+        // (Q^2)_01 = q00 q01 + q01 q11 + q02 q21, q21 = conj(q12)
+        cplx<R> t = {(d0 + d1) * q.m[1].re, (d0 + d1) * q.m[1].im};
+        mac_nc(t, q.m[2], q.m[5]);
+        q2.m[1] = t;
+        // (Q^2)_02 = q00 q02 + q01 q12 + q02 q22
+        t = {(d0 + d2) * q.m[2].re, (d0 + d2) * q.m[2].im};
+        mac(t, q.m[1], q.m[5]);
+        q2.m[2] = t;
+        // (Q^2)_12 = q10 q02 + q11 q12 + q12 q22, q10 = conj(q01)
+        t = {(d1 + d2) * q.m[5].re, (d1 + d2) * q.m[5].im};
+        mac_cn(t, q.m[1], q.m[2]);
+        q2.m[5] = t;
+        q2.m[3] = conjg(q2.m[1]);
+        q2.m[6] = conjg(q2.m[2]);
+        q2.m[7] = conjg(q2.m[5]);
+    }
     R c1 = R(0.5) * (q2.m[0].re + q2.m[4].re + q2.m[8].re);
     int s = 0;
     R rho2 = R(4) / R(3) * c1;
     R scale = R(1);
-    while (rho2 > R(1) && s < 64) { // bounded: a non-finite input cannot loop forever
+    while (rho2 > R(0.0625) && s < 64) { // bounded: a non-finite input cannot loop forever
         rho2 *= R(0.25);
         scale *= R(0.5);
         ++s;
@@ -200,16 +299,26 @@ template <typename R> __device__ __forceinline__ mat3<R> exp_i_hermitian(mat3<R>
         for (int k = 0; k < 9; ++k) q2.m[k] = (scale * scale) * q2.m[k];
         c1 *= scale * scale;
     }
-    // det Q (real for Hermitian Q)
-    cplx<R> d = q.m[0] * (q.m[4] * q.m[8] - q.m[5] * q.m[7]) - q.m[1] * (q.m[3] * q.m[8] - q.m[5] * q.m[6]) +
-                q.m[2] * (q.m[3] * q.m[7] - q.m[4] * q.m[6]);
-    R c0 = d.re;
-    // series
-    R a = 1, b = 0, c = 0;            // (a,b,c)_n / n!
+    // det Q, real for Hermitian Q:
+    //   q00 q11 q22 + 2 Re(q01 q12 q20) - q00 |q12|^2 - q11 |q02|^2 - q22 |q01|^2
+    R c0;
+    {
+        const R d0 = q.m[0].re, d1 = q.m[4].re, d2 = q.m[8].re;
+        cplx<R> t = {R(0), R(0)};
+        mac(t, q.m[1], q.m[5]); // q01 q12
+        R tri = t.re * q.m[2].re + t.im * q.m[2].im; // Re(q01 q12 conj(q02))
+        const R a01 = q.m[1].re * q.m[1].re + q.m[1].im * q.m[1].im;
+        const R a02 = q.m[2].re * q.m[2].re + q.m[2].im * q.m[2].im;
+        const R a12 = q.m[5].re * q.m[5].re + q.m[5].im * q.m[5].im;
+        c0 = d0 * d1 * d2 + R(2) * tri - d0 * a12 - d1 * a02 - d2 * a01;
+    }
+    // series, fully unrolled with compile-time reciprocals
+    R a = 1, b = 0, c = 0; // (a,b,c)_n / n!
     R f0r = 1, f0i = 0, f1r = 0, f1i = 0, f2r = 0, f2i = 0;
-    for (int n = 0; n < 40; ++n) {
-        R na = c0 * c, nb = a + c1 * c, nc = b;
-        R inv = R(1) / R(n + 1);
+#pragma unroll
+    for (int n = 0; n < exp_cfg<R>::terms; ++n) {
+        const R inv = R(1) / R(n + 1);
+        const R na = c0 * c, nb = fmx(c1, c, a), nc = b;
         a = na * inv;
         b = nb * inv;
         c = nc * inv;
@@ -220,13 +329,16 @@ template <typename R> __device__ __forceinline__ mat3<R> exp_i_hermitian(mat3<R>
         case 2: f0r -= a; f1r -= b; f2r -= c; break;
         default: f0i -= a; f1i -= b; f2i -= c; break;
         }
-        R mag = fabs(a) + fabs(b) + fabs(c);
-        if (mag < exp_eps<R>::v && n >= 2) break;
     }
-    cplx<R> f0 = {f0r, f0i}, f1 = {f1r, f1i}, f2 = {f2r, f2i};
+    const cplx<R> f0 = {f0r, f0i}, f1 = {f1r, f1i}, f2 = {f2r, f2i};
     mat3<R> x;
 #pragma unroll
-    for (int k = 0; k < 9; ++k) x.m[k] = f1 * q.m[k] + f2 * q2.m[k];
+    for (int k = 0; k < 9; ++k) {
+        cplx<R> t = {R(0), R(0)};
+        mac(t, f1, q.m[k]);
+        mac(t, f2, q2.m[k]);
+        x.m[k] = t;
+    }
     x.m[0] = x.m[0] + f0;
     x.m[4] = x.m[4] + f0;
     x.m[8] = x.m[8] + f0;

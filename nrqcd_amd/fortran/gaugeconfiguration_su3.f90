@@ -38,6 +38,7 @@ module gaugeconfiguration_su3
      procedure, public :: Upload
      procedure, public :: Download
      procedure, public :: Reunitarise
+     procedure, public :: SetReunitarisationInterval
      procedure, private :: Update_Leapfrog
      generic, public :: Update => Update_Leapfrog
      procedure, private :: Assign
@@ -245,4 +246,13 @@ contains
     class(GaugeConfiguration), intent(inout) :: GaugeConf
     call nrq_check(nrq_gauge_reunitarise(GaugeConf%handle), 'GaugeConfiguration%Reunitarise')
   end subroutine Reunitarise
+
+  !> Reunitarise after every Interval leapfrog steps (0: never).
+  subroutine SetReunitarisationInterval(GaugeConf, Interval)
+    class(GaugeConfiguration), intent(inout) :: GaugeConf
+    integer(int64), intent(in) :: Interval
+    call GaugeConf%Allocate
+    call nrq_check(nrq_gauge_set_reunitarise_interval(GaugeConf%handle, Interval), &
+         'GaugeConfiguration%SetReunitarisationInterval')
+  end subroutine SetReunitarisationInterval
 end module gaugeconfiguration_su3

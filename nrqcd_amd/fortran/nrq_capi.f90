@@ -138,6 +138,13 @@ module nrq_capi
        type(c_ptr), value :: h
      end function nrq_gauge_reunitarise
 
+     integer(c_int) function nrq_gauge_set_reunitarise_interval(h, interval) &
+          bind(C, name='nrq_gauge_set_reunitarise_interval')
+       import :: c_int, c_ptr, c_int64_t
+       type(c_ptr), value :: h
+       integer(c_int64_t), value :: interval
+     end function nrq_gauge_set_reunitarise_interval
+
      integer(c_int) function nrq_cn_init(mass, c, sw, tol) bind(C, name='nrq_cn_init')
        import :: c_int, c_double, c_double_complex
        real(c_double), value :: mass, sw, tol

@@ -168,3 +168,39 @@ def test_fp32_storage_tracks_fp64(nrq, oracle, lattice):
     assert np.max(np.abs(Eg - E)) < 1e-4
     assert g.GetEnergy() == pytest.approx(oracle.energy(U, E), rel=1e-4)
     assert e0 == pytest.approx(oracle.energy(*oracle.box_init(6, 0.5, 1.0, 1.0)), rel=1e-5)
+
+
+def test_reunitarisation_interval(nrq, oracle, lattice):
+    """FP32 storage with automatic FP64 reunitarisation every 10 steps: the
+    interval counts across Update calls and gives bit-identical results to
+    explicit Reunitarise calls at the same points; the links stay unitary and
+    track the FP64 oracle."""
+    ext = (8, 6, 10)
+    _say("reunitarisation interval")
+    lattice(ext, SPACING_DT005)
+    oracle.lattice_init(ext, SPACING_DT005)
+    U, E = oracle.box_init(7, 0.5, 1.0, 1.0)
+    auto = nrq.GaugeConfiguration(nrq.NRQ_PRECISION_FP32)
+    auto.upload(U, E)
+    auto.SetReunitarisationInterval(10)
+    manual = auto.copy()
+    manual.SetReunitarisationInterval(0)
+    for n in (7, 8, 10):  # 25 steps: reunitarised after 10 and 20
+        auto.Update(1.0, n)
+    manual.Update(1.0, 10)
+    manual.Reunitarise()
+    manual.Update(1.0, 10)
+    manual.Reunitarise()
+    manual.Update(1.0, 5)
+    Ua, Ea = auto.download()
+    Um, Em = manual.download()
+    assert np.array_equal(Ua, Um) and np.array_equal(Ea, Em)
+    # unitarity of the stored FP32 links
+    M = np.moveaxis(Ua.reshape(3, 3, 3, -1), (0, 1), (-2, -1))
+    dev = np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max()
+    assert dev < 1e-5
+    oracle.leapfrog(U, E, 1.0, 25)
+    assert np.max(np.abs(Ua - U)) < 1e-4
+    assert np.max(np.abs(Ea - E)) < 1e-4
+    with pytest.raises(nrq.NRQError):
+        auto.SetReunitarisationInterval(-1)
