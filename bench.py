@@ -2,13 +2,14 @@
 
 Workload (one GPU): 64^3 lattice, spacings (a0, a1, a2, a3) = (0.05, 1, 1, 1),
 transverse-polarised box initial condition with Qs = 0.5, amplitude 1,
-coupling 1, ranlxd seed 2; FP64 links and E; heavy quark point source at
-lattice index 1, Ma = 2, c = (1, 0, 0, 0), CN solved by GMRES(30) to an FP64
-true relative residual of 1e-10.
+coupling 1, ranlxd seed 2; FP64 links and E; heavy quark and antiquark point
+sources at lattice index 1, Ma = 2, c = (1, 0, 0, 0), every CN column solve
+to an FP64 true relative residual of 1e-10.
 
 A "step" is one leapfrog step of the whole lattice (3V link-updates).  The
 timed region runs `--steps` leapfrog steps back to back on the library's
-stream with the inputs resident in HBM; CN steps are timed separately.
+stream with the inputs resident in HBM; CN steps are timed separately for
+each solver: the plain FP64 GMRES(30) (PETSc's algorithm) and the fast ones.
 
 Prints one JSON line.  Options: --gpus N (only 1), --steps K, --warmup W,
 --size L, --cn-steps C, --no-cpu (skip the CPU baseline).
@@ -27,6 +28,19 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "link-updates/s & CN steps/s at FP64 rel.res 1e-10: learned+mixed vs plain"
 HBM_PEAK_GBS = 8000.0
+# HBM bytes per fused leapfrog launch measured with rocprofv3 (FETCH_SIZE and
+# WRITE_SIZE, each calibrated on the copy kernel), kept with the profile
+TRAFFIC_FILE = ROOT / "profiles" / "leapfrog_fused_64cubed_fp64_traffic.txt"
+
+
+def _measured_traffic(size: int):
+    """HBM bytes per fused launch from the committed rocprof summary, or None."""
+    if size != 64 or not TRAFFIC_FILE.exists():
+        return None
+    for line in TRAFFIC_FILE.read_text().splitlines():
+        if line.startswith("traffic_bytes_per_launch="):
+            return float(line.split("=", 1)[1])
+    return None
 
 
 def _cpu_baseline(size: int, seconds: float) -> dict:
@@ -99,26 +113,31 @@ def main() -> None:
     # ---- CN ----------------------------------------------------------------
     nrq.nrqcd_init(2.0, [1, 0, 0, 0], 1.0, 1e-10)
     cn = {}
-    for name, kind in (("plain_fp64", nrq.NRQ_SOLVER_GMRES_FP64), ("gmres_mixed", nrq.NRQ_SOLVER_GMRES_MIXED),
-                       ("chebyshev_fp64", nrq.NRQ_SOLVER_CHEBYSHEV_FP64),
-                       ("chebyshev_mixed", nrq.NRQ_SOLVER_CHEBYSHEV_MIXED)):
+    for name, kind in (("plain_fp64_gmres", nrq.NRQ_SOLVER_GMRES_FP64),
+                       ("mixed_gmres", nrq.NRQ_SOLVER_GMRES_MIXED),
+                       ("fp64_chebyshev", nrq.NRQ_SOLVER_CHEBYSHEV_FP64),
+                       ("mixed_chebyshev", nrq.NRQ_SOLVER_CHEBYSHEV_MIXED)):
         nrq.nrqcd_set_solver(kind, 30, 10000)
         h = nrq.NRQCDField()
         h.InitSinglePoint(1, 1)
         h.Update(g, 2.0, [1, 0, 0, 0])  # warm-up (workspace allocation)
         nrq.synchronize()
         its = 0
+        solves = 0
         worst = 0.0
         nrq.event_record(2)
         for _ in range(args.cn_steps):
             st = h.Update(g, 2.0, [1, 0, 0, 0])
             its += st.iterations
+            solves += st.solves
             worst = max(worst, st.max_true_relres)
         nrq.event_record(3)
         cn_ms = nrq.event_elapsed_ms(2, 3)
         nq, na = h.norms()
         cn[name] = {"cn_steps_per_s": args.cn_steps / (cn_ms * 1e-3), "ms_per_step": cn_ms / args.cn_steps,
-                    "krylov_iterations_per_step": its / args.cn_steps, "worst_true_relres": worst, "norms": [nq, na]}
+                    "column_solves_per_s": solves / (cn_ms * 1e-3),
+                    "iterations_per_column_solve": its / max(solves, 1), "worst_true_relres": worst,
+                    "norms": [nq, na]}
         del h
     nrq.nrqcd_finalize()
     copy_gbs = nrq.bandwidth_probe(1 << 30, 20)
@@ -144,12 +163,13 @@ def main() -> None:
             "workload": f"{L}^3 SU(3) leapfrog (FP64) + NRQCD CN point source, box IC Qs=0.5 seed 2",
             "lattice": [L, L, L],
             "spacings": [0.05, 1.0, 1.0, 1.0],
-            "cn_solver": "GMRES(30) FP64, true rel.res 1e-10",
+            "cn_solvers": "GMRES(30) FP64 (plain), FP32-GMRES/FP64-IR, Chebyshev FP64, FP32-Chebyshev/FP64-IR",
             "mass": 2.0,
         },
-        "cn_steps_per_s": cn["plain_fp64"]["cn_steps_per_s"],
+        "cn_steps_per_s": cn["mixed_chebyshev"]["cn_steps_per_s"],
+        "cn_steps_per_s_plain": cn["plain_fp64_gmres"]["cn_steps_per_s"],
         "cn": cn,
-        "cn_speedup_vs_plain": {k: v["cn_steps_per_s"] / cn["plain_fp64"]["cn_steps_per_s"] for k, v in cn.items()},
+        "cn_speedup_vs_plain": {k: v["cn_steps_per_s"] / cn["plain_fp64_gmres"]["cn_steps_per_s"] for k, v in cn.items()},
         "energy_drift_rel": (obs1["energy"] - obs0["energy"]) / obs0["energy"],
         "gauss_max_drift": obs1["gauss_max"] - obs0["gauss_max"],
         "wall_s_leapfrog": wall,
@@ -160,7 +180,7 @@ def main() -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": None,
+            "traffic": _measured_traffic(L),
             "kernel": "k_fused<double> (E update + next link update per (site, dir))",
             "launch_shape": shape,
             "bytes_per_launch": bytes_per_site * V,
