@@ -129,6 +129,14 @@ int nrq_gauge_get_efield(const nrq_gauge *h, int dir, int64_t idx, double e[8]);
  * times, gaugeconfiguration_su3.f90:2082-2111 (links then E for
  * stepwidth > 0, E then links otherwise). */
 int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps);
+/* nrq_gauge_leapfrog for n configurations of one storage precision on the
+ * current lattice (an ensemble): each pass over all of them is one launch,
+ * and the launch sequence of nsteps steps is captured into a hipGraph and
+ * replayed on later calls with the same buffers (NRQ_GRAPH=0 in the
+ * environment launches directly).  Each configuration's reunitarisation
+ * interval is honoured.  No reference counterpart: the reference evolves
+ * one configuration per run (src/main.f90:2572-2620 loops over runs). */
+int nrq_gauge_leapfrog_batch(nrq_gauge *const *h, int n, double stepwidth, int64_t nsteps);
 /* GaugeConfiguration%GetEnergy(), gaugeconfiguration_su3.f90:1769-1817 */
 int nrq_gauge_energy(const nrq_gauge *h, double *energy);
 /* GaugeConfiguration%GetDeviationFromGaussLaw(), gaugeconfiguration_su3.f90:
@@ -222,6 +230,14 @@ int nrq_heavy_download(const nrq_heavy *h, double *quark, double *antiq);
  * stats may be NULL. */
 int nrq_heavy_update(nrq_heavy *h, const nrq_gauge *g, double mass, const double c[8],
                      double stepwidth, nrq_solve_stats *stats);
+/* nrq_heavy_update for n heavy fields on one gauge configuration, heavy field
+ * k with mass[k] (same Wilson coefficients and step width): all their live
+ * colour fields (at most 48) are right-hand sides of one batched solve that
+ * streams the links once per chunk of fields.  Results equal n separate
+ * nrq_heavy_update calls to the solver tolerance.  No reference counterpart:
+ * the reference updates one NRQCDField per call (nrqcd.f90:1053). */
+int nrq_heavy_update_batch(nrq_heavy *const *h, const double *mass, int n, const nrq_gauge *g, const double c[8],
+                           double stepwidth, nrq_solve_stats *stats);
 /* GetNorm_Quark / GetNorm_AntiQ, nrqcd.f90:547-611 */
 int nrq_heavy_norms(const nrq_heavy *h, double *norm_quark, double *norm_antiq);
 /* GetMesoncorrelator_{O1_1S0, O1_3S1, O8_1S0, O8_3S1}_ZeroMomentum,

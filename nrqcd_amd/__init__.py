@@ -115,6 +115,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_gauge_get_link": (I, [P, I, I64, P]),
         "nrq_gauge_get_efield": (I, [P, I, I64, P]),
         "nrq_gauge_leapfrog": (I, [P, D, I64]),
+        "nrq_gauge_leapfrog_batch": (I, [P, I, D, I64]),
         "nrq_gauge_energy": (I, [P, P]),
         "nrq_gauge_gauss_ref": (I, [P, P]),
         "nrq_gauge_gauss_max": (I, [P, P]),
@@ -135,6 +136,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_heavy_upload": (I, [P, P, P]),
         "nrq_heavy_download": (I, [P, P, P]),
         "nrq_heavy_update": (I, [P, P, D, P, D, P]),
+        "nrq_heavy_update_batch": (I, [P, P, I, P, P, D, P]),
         "nrq_heavy_norms": (I, [P, P, P]),
         "nrq_heavy_correlators": (I, [P, P]),
     }
@@ -365,6 +367,28 @@ class GaugeConfiguration:
 # ---------------------------------------------------------------------------
 # nrqcd
 # ---------------------------------------------------------------------------
+def leapfrog_batch(configs, StepWidth: float = 1.0, nsteps: int = 1) -> None:
+    """Advance an ensemble of GaugeConfiguration objects (one storage
+    precision) by ``nsteps`` leapfrog steps, one launch per pass over all of
+    them, replayed from a captured hipGraph."""
+    hs = (ctypes.c_void_p * len(configs))(*[c.handle.value for c in configs])
+    _check(library().nrq_gauge_leapfrog_batch(ctypes.cast(hs, ctypes.c_void_p), len(configs), StepWidth, nsteps))
+
+
+def update_batch(fields, GaugeConf, masses, WilsonCoeffs, StepWidth: float = 1.0) -> "SolveStats":
+    """One CN step of several NRQCDField objects with their own masses on one
+    gauge configuration, as one batched multi-right-hand-side solve."""
+    if len(fields) != len(masses):
+        raise ValueError("one mass per field")
+    hs = (ctypes.c_void_p * len(fields))(*[f._h.value for f in fields])
+    ms = np.ascontiguousarray(masses, dtype=np.float64)
+    c = _coeffs(WilsonCoeffs)
+    st = SolveStats()
+    _check(library().nrq_heavy_update_batch(ctypes.cast(hs, ctypes.c_void_p), _ptr(ms), len(fields), GaugeConf.handle,
+                                            _ptr(c), StepWidth, ctypes.byref(st)))
+    return st
+
+
 def _coeffs(WilsonCoeffs) -> np.ndarray:
     c = np.zeros(4, np.complex128)
     c[: len(WilsonCoeffs)] = WilsonCoeffs

@@ -27,7 +27,7 @@ namespace nrq {
 
 const void *gauge_links(const nrq_gauge *h);
 
-constexpr int NFIELDS = NSLOTS;
+constexpr int NFIELDS = NFIELDS_PER_HEAVY;
 
 struct cn_config {
     bool initialised = false;
@@ -355,53 +355,75 @@ extern "C" int nrq_heavy_download(const nrq_heavy *h, double *quark, double *ant
     return NRQ_OK;
 }
 
-extern "C" int nrq_heavy_update(nrq_heavy *h, const nrq_gauge *gf, double mass, const double c[8], double stepwidth,
-                                nrq_solve_stats *stats)
+// One CN step of the live fields of n heavy fields on one gauge
+// configuration as a single batched solve; heavy field k has mass[k].
+static int heavy_update_n(nrq_heavy *const *hs, const double *mass, int n, const nrq_gauge *gf, const double c[8],
+                          double stepwidth, nrq_solve_stats *stats)
 {
     if (!g_cn.initialised) return fail(NRQ_ERR_STATE, "nrqcd: not initialised");
-    NEED_HEAVY(h);
+    if (!hs || !mass || n < 1) return fail(NRQ_ERR_ARG, "heavy: no fields");
     if (!gf || !gf->u[0]) return fail(NRQ_ERR_ARG, "heavy: null gauge configuration");
-    if (gf->vol != h->vol) return fail(NRQ_ERR_ARG, "heavy: gauge and heavy field lattices differ");
     if (!c) return fail(NRQ_ERR_ARG, "heavy: null Wilson coefficients");
-    if (!std::isfinite(mass) || mass == 0.0 || !std::isfinite(stepwidth)) return fail(NRQ_ERR_ARG, "heavy: bad mass or step width");
+    if (!std::isfinite(stepwidth)) return fail(NRQ_ERR_ARG, "heavy: bad step width");
     for (int k = 1; k < 4; ++k)
         if (c[2 * k] != 0.0 || c[2 * k + 1] != 0.0)
             return fail(NRQ_ERR_UNSUPPORTED, "heavy: only the kinetic term c1 is implemented (c%d != 0)", k + 1);
-    if (int r = krylov_reserve(h->vol, g_cn.solver.restart)) return r;
-    nrq_solve_stats st = {0, 0, 0, 0.0};
     const double a0 = lat().g.a[0];
     const std::complex<double> c1(c[0], c[1]);
     const std::complex<double> mi(0.0, -(stepwidth / 2.0 * a0));
-    const std::complex<double> alpha_q = mi * c1 / mass;
-    const std::complex<double> alpha_a = mi * std::conj(c1) / (-mass);
     // The live fields are the right-hand sides (a field that is exactly zero
     // stays zero, as KSPSolve returns x = 0 for b = 0).  rhs = S psi with
     // S = 1 + alpha K (SystMat_herm before the in-place Hermitian
     // transpose), then S^+ psi' = rhs with S^+ = 1 + conj(alpha) K.
-    int fields[NFIELDS];
-    cplx<double> alpha[NFIELDS], beta[NFIELDS];
+    cplx<double> *fields[NSLOTS];
+    cplx<double> alpha[NSLOTS], beta[NSLOTS];
     int R = 0;
-    for (int f = 0; f < NFIELDS; ++f) {
-        if (!(h->live >> f & 1u)) continue;
-        const std::complex<double> a = f < 12 ? alpha_q : alpha_a;
-        fields[R] = f;
-        alpha[R] = {a.real(), a.imag()};
-        beta[R] = {a.real(), -a.imag()};
-        ++R;
+    for (int k = 0; k < n; ++k) {
+        const nrq_heavy *h = hs[k];
+        NEED_HEAVY(h);
+        if (gf->vol != h->vol) return fail(NRQ_ERR_ARG, "heavy: gauge and heavy field lattices differ");
+        if (!std::isfinite(mass[k]) || mass[k] == 0.0) return fail(NRQ_ERR_ARG, "heavy: bad mass %g", mass[k]);
+        for (int j = 0; j < k; ++j)
+            if (hs[j] == h) return fail(NRQ_ERR_ARG, "heavy: field %d listed twice", k);
+        const std::complex<double> alpha_q = mi * c1 / mass[k];
+        const std::complex<double> alpha_a = mi * std::conj(c1) / (-mass[k]);
+        for (int f = 0; f < NFIELDS; ++f) {
+            if (!(h->live >> f & 1u)) continue;
+            if (R == NSLOTS) return fail(NRQ_ERR_ARG, "heavy: more than %d live fields in one batch", NSLOTS);
+            const std::complex<double> a = f < 12 ? alpha_q : alpha_a;
+            fields[R] = (cplx<double> *)h->psi + (int64_t)f * 3 * h->vol;
+            alpha[R] = {a.real(), a.imag()};
+            beta[R] = {a.real(), -a.imag()};
+            ++R;
+        }
     }
+    nrq_solve_stats st = {0, 0, 0, 0.0};
     st.solves = R;
-    const int status = krylov_cn_step(gf, (cplx<double> *)h->psi, fields, R, alpha, beta, g_cn.solver, &st);
+    if (int r = krylov_reserve(hs[0]->vol, g_cn.solver.restart, R)) return r;
+    const int status = krylov_cn_step(gf, fields, R, alpha, beta, g_cn.solver, &st);
     if (status != NRQ_OK && status != NRQ_ERR_NOCONV) return status;
     NRQ_HIP(hipStreamSynchronize(stream()));
     if (stats) *stats = st;
     return status;
 }
 
+extern "C" int nrq_heavy_update(nrq_heavy *h, const nrq_gauge *gf, double mass, const double c[8], double stepwidth,
+                                nrq_solve_stats *stats)
+{
+    return heavy_update_n(&h, &mass, 1, gf, c, stepwidth, stats);
+}
+
+extern "C" int nrq_heavy_update_batch(nrq_heavy *const *h, const double *mass, int n, const nrq_gauge *gf,
+                                      const double c[8], double stepwidth, nrq_solve_stats *stats)
+{
+    return heavy_update_n(h, mass, n, gf, c, stepwidth, stats);
+}
+
 extern "C" int nrq_heavy_norms(const nrq_heavy *h, double *nq, double *na)
 {
     NEED_HEAVY(h);
     if (!nq || !na) return fail(NRQ_ERR_ARG, "null argument");
-    if (int r = krylov_reserve(h->vol, g_cn.initialised ? g_cn.solver.restart : 30)) return r;
+    if (int r = krylov_reserve(h->vol, g_cn.initialised ? g_cn.solver.restart : 30, 0)) return r;
     double n2[NFIELDS];
     if (int e = krylov_norms2((const cplx<double> *)h->psi, NFIELDS, n2)) return e;
     double q = 0, a = 0;
@@ -416,7 +438,7 @@ extern "C" int nrq_heavy_correlators(const nrq_heavy *h, double out[8])
 {
     NEED_HEAVY(h);
     if (!out) return fail(NRQ_ERR_ARG, "null argument");
-    if (int r = krylov_reserve(h->vol, g_cn.initialised ? g_cn.solver.restart : 30)) return r;
+    if (int r = krylov_reserve(h->vol, g_cn.initialised ? g_cn.solver.restart : 30, 0)) return r;
     const int nb = 256;
     double *part = krylov_reduce_scratch();
     double *res = part + 8 * nb;

@@ -131,10 +131,21 @@ __device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, con
     for (int a = 0; a < 8; ++a) e[a] -= t[a];
 }
 
+// Buffers of the configurations of one launch (blockIdx.y selects one): all
+// share precision and lattice; passed to the kernels by value.
+constexpr int LF_MAXB = 32;
+template <typename R> struct lf_batch {
+    const cplx<R> *uin[LF_MAXB];
+    cplx<R> *uout[LF_MAXB];
+    R *ef[LF_MAXB];
+};
+
 template <typename R, int BS, bool NT>
-__global__ __launch_bounds__(BS) void k_links(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
-                                              const R *__restrict__ ef, geom g, R dt, lf_map m)
+__global__ __launch_bounds__(BS) void k_links(lf_batch<R> bt, geom g, R dt, lf_map m)
 {
+    const cplx<R> *__restrict__ uin = bt.uin[blockIdx.y];
+    cplx<R> *__restrict__ uout = bt.uout[blockIdx.y];
+    const R *__restrict__ ef = bt.ef[blockIdx.y];
     lsite p;
     int i;
     if (!lf_site(m, g, p, i)) return;
@@ -148,9 +159,10 @@ __global__ __launch_bounds__(BS) void k_links(const cplx<R> *__restrict__ uin, c
 }
 
 template <typename R, int BS, bool NT, bool REC>
-__global__ __launch_bounds__(BS) void k_efield(const cplx<R> *__restrict__ u, R *__restrict__ ef, geom g, R dt,
-                                               lf_map m)
+__global__ __launch_bounds__(BS) void k_efield(lf_batch<R> bt, geom g, R dt, lf_map m)
 {
+    const cplx<R> *__restrict__ u = bt.uin[blockIdx.y];
+    R *__restrict__ ef = bt.ef[blockIdx.y];
     lsite p;
     int i;
     if (!lf_site(m, g, p, i)) return;
@@ -170,9 +182,11 @@ __global__ __launch_bounds__(BS) void k_efield(const cplx<R> *__restrict__ u, R 
 // HBM round trip of E and U.  Neighbour links are read from uin, new links
 // go to uout (ping-pong), E is updated in place (each thread owns its E).
 template <typename R, int BS, bool NT, bool REC>
-__global__ __launch_bounds__(BS) void k_fused(const cplx<R> *__restrict__ uin, cplx<R> *__restrict__ uout,
-                                              R *__restrict__ ef, geom g, R dt, lf_map m)
+__global__ __launch_bounds__(BS) void k_fused(lf_batch<R> bt, geom g, R dt, lf_map m)
 {
+    const cplx<R> *__restrict__ uin = bt.uin[blockIdx.y];
+    cplx<R> *__restrict__ uout = bt.uout[blockIdx.y];
+    R *__restrict__ ef = bt.ef[blockIdx.y];
     lsite p;
     int i;
     if (!lf_site(m, g, p, i)) return;
@@ -501,43 +515,47 @@ static lf_map make_map(int variant, const geom &g)
     return make_lf_map(g, v.bs, v.xcd, v.mode, 3);
 }
 
-template <typename R>
-static void launch_links(int variant, const cplx<R> *uin, cplx<R> *uout, const R *ef, const geom &g, R dt)
+template <typename R> static void launch_links(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
 {
     const lf_map m = make_map(variant, g);
-    const dim3 grd((unsigned)(m.ngroup * m.nsb));
+    const dim3 grd((unsigned)(m.ngroup * m.nsb), (unsigned)nb);
     switch (m.bs) {
-    case 64: LF_NT(hipLaunchKernelGGL((k_links<R, 64, NTV>), grd, dim3(64), 0, stream(), uin, uout, ef, g, dt, m)); break;
-    case 128: LF_NT(hipLaunchKernelGGL((k_links<R, 128, NTV>), grd, dim3(128), 0, stream(), uin, uout, ef, g, dt, m)); break;
-    case 512: LF_NT(hipLaunchKernelGGL((k_links<R, 512, NTV>), grd, dim3(512), 0, stream(), uin, uout, ef, g, dt, m)); break;
-    default: LF_NT(hipLaunchKernelGGL((k_links<R, 256, NTV>), grd, dim3(256), 0, stream(), uin, uout, ef, g, dt, m)); break;
+    case 64: LF_NT(hipLaunchKernelGGL((k_links<R, 64, NTV>), grd, dim3(64), 0, stream(), bt, g, dt, m)); break;
+    case 128: LF_NT(hipLaunchKernelGGL((k_links<R, 128, NTV>), grd, dim3(128), 0, stream(), bt, g, dt, m)); break;
+    default: LF_NT(hipLaunchKernelGGL((k_links<R, 256, NTV>), grd, dim3(256), 0, stream(), bt, g, dt, m)); break;
     }
 }
 
-template <typename R>
-static void launch_efield(int variant, const cplx<R> *u, R *ef, const geom &g, R dt)
+template <typename R> static void launch_efield(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
 {
     const lf_map m = make_map(variant, g);
-    const dim3 grd((unsigned)(m.ngroup * m.nsb));
+    const dim3 grd((unsigned)(m.ngroup * m.nsb), (unsigned)nb);
     switch (m.bs) {
-    case 64: LF_NT(hipLaunchKernelGGL((k_efield<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), u, ef, g, dt, m)); break;
-    case 128: LF_NT(hipLaunchKernelGGL((k_efield<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), u, ef, g, dt, m)); break;
-    case 512: LF_NT(hipLaunchKernelGGL((k_efield<R, 512, NTV, RECV>), grd, dim3(512), 0, stream(), u, ef, g, dt, m)); break;
-    default: LF_NT(hipLaunchKernelGGL((k_efield<R, 256, NTV, RECV>), grd, dim3(256), 0, stream(), u, ef, g, dt, m)); break;
+    case 64: LF_NT(hipLaunchKernelGGL((k_efield<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), bt, g, dt, m)); break;
+    case 128: LF_NT(hipLaunchKernelGGL((k_efield<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), bt, g, dt, m)); break;
+    default: LF_NT(hipLaunchKernelGGL((k_efield<R, 256, NTV, RECV>), grd, dim3(256), 0, stream(), bt, g, dt, m)); break;
     }
 }
 
-template <typename R>
-static void launch_fused(int variant, const cplx<R> *uin, cplx<R> *uout, R *ef, const geom &g, R dt)
+template <typename R> static void launch_fused(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
 {
     const lf_map m = make_map(variant, g);
-    const dim3 grd((unsigned)(m.ngroup * m.nsb));
+    const dim3 grd((unsigned)(m.ngroup * m.nsb), (unsigned)nb);
     switch (m.bs) {
-    case 64: LF_NT(hipLaunchKernelGGL((k_fused<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), uin, uout, ef, g, dt, m)); break;
-    case 128: LF_NT(hipLaunchKernelGGL((k_fused<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), uin, uout, ef, g, dt, m)); break;
-    case 512: LF_NT(hipLaunchKernelGGL((k_fused<R, 512, NTV, RECV>), grd, dim3(512), 0, stream(), uin, uout, ef, g, dt, m)); break;
-    default: LF_NT(hipLaunchKernelGGL((k_fused<R, 256, NTV, RECV>), grd, dim3(256), 0, stream(), uin, uout, ef, g, dt, m)); break;
+    case 64: LF_NT(hipLaunchKernelGGL((k_fused<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), bt, g, dt, m)); break;
+    case 128: LF_NT(hipLaunchKernelGGL((k_fused<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), bt, g, dt, m)); break;
+    default: LF_NT(hipLaunchKernelGGL((k_fused<R, 256, NTV, RECV>), grd, dim3(256), 0, stream(), bt, g, dt, m)); break;
     }
+}
+
+// one-configuration table
+template <typename R> static lf_batch<R> one(const cplx<R> *uin, cplx<R> *uout, R *ef)
+{
+    lf_batch<R> b = {};
+    b.uin[0] = uin;
+    b.uout[0] = uout;
+    b.ef[0] = ef;
+    return b;
 }
 
 // Times every launch shape on scratch copies of the configuration (the
@@ -565,9 +583,9 @@ template <typename R> static int tune_leapfrog(const nrq_gauge *h, R dt, int *be
     *best = LF_DEFAULT;
     for (int v = 0; v < LF_NVARIANTS; ++v) {
         if (!variant_ok(v, g)) continue;
-        launch_fused<R>(v, u0, u1, ef, g, dt); // warm-up
+        launch_fused<R>(v, one<R>(u0, u1, ef), 1, g, dt); // warm-up
         NRQ_HIP(hipEventRecord(e0, stream()));
-        for (int k = 0; k < reps; ++k) launch_fused<R>(v, (k & 1) ? u1 : u0, (k & 1) ? u0 : u1, ef, g, dt);
+        for (int k = 0; k < reps; ++k) launch_fused<R>(v, (k & 1) ? one<R>(u1, u0, ef) : one<R>(u0, u1, ef), 1, g, dt);
         NRQ_HIP(hipEventRecord(e1, stream()));
         NRQ_HIP(hipEventSynchronize(e1));
         float ms = 0;
@@ -607,30 +625,97 @@ template <typename R> static int leapfrog_variant(const nrq_gauge *h, R dt, int 
     return NRQ_OK;
 }
 
-template <typename R> static int leapfrog_t(nrq_gauge *h, double sw, int64_t n)
+// Leapfrog of nb <= LF_MAXB configurations (same precision, current lattice)
+// by n steps: one launch per pass covers all of them (blockIdx.y selects the
+// configuration).  With `graph`, the launch sequence of the n steps is
+// captured into a hipGraph once and replayed on later calls with the same
+// buffers, parities, step count and width.
+struct lf_graph {
+    std::vector<const void *> key;
+    int64_t n;
+    double sw;
+    hipGraphExec_t exec;
+};
+static std::vector<lf_graph> g_lf_graphs;
+constexpr size_t LF_GRAPH_CACHE = 8;
+
+template <typename R> static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, double sw, int64_t n, bool graph)
 {
     const geom g = lat().g;
     int v = 0;
-    if (int r = leapfrog_variant<R>(h, (R)sw, &v)) return r;
-    cplx<R> *u0 = (cplx<R> *)h->u[0], *u1 = (cplx<R> *)h->u[1];
-    R *ef = (R *)h->e;
+    if (int r = leapfrog_variant<R>(hs[0], (R)sw, &v)) return r;
     const R dt = (R)sw;
-    auto cur = [&](int c) { return c ? u1 : u0; };
-    if (sw > 0) {
-        launch_links<R>(v, cur(h->cur), cur(h->cur), (const R *)ef, g, dt);
-        for (int64_t k = 1; k < n; ++k) {
-            launch_fused<R>(v, cur(h->cur), cur(h->cur ^ 1), ef, g, dt);
-            h->cur ^= 1;
+    auto table = [&](const std::vector<int> &cur, bool swap) {
+        lf_batch<R> b = {};
+        for (int i = 0; i < nb; ++i) {
+            b.uin[i] = (const cplx<R> *)hs[i]->u[cur[i]];
+            b.uout[i] = (cplx<R> *)hs[i]->u[swap ? cur[i] ^ 1 : cur[i]];
+            b.ef[i] = (R *)hs[i]->e;
         }
-        launch_efield<R>(v, (const cplx<R> *)cur(h->cur), ef, g, dt);
+        return b;
+    };
+    std::vector<int> cur(nb);
+    for (int i = 0; i < nb; ++i) cur[i] = hs[i]->cur;
+    const int64_t flips = sw > 0 ? n - 1 : n;
+    auto issue = [&]() {
+        std::vector<int> c = cur;
+        if (sw > 0) {
+            launch_links<R>(v, table(c, false), nb, g, dt);
+            for (int64_t k = 1; k < n; ++k) {
+                launch_fused<R>(v, table(c, true), nb, g, dt);
+                for (int &x : c) x ^= 1;
+            }
+            launch_efield<R>(v, table(c, false), nb, g, dt);
+        } else {
+            for (int64_t k = 0; k < n; ++k) {
+                launch_fused<R>(v, table(c, true), nb, g, dt);
+                for (int &x : c) x ^= 1;
+            }
+        }
+    };
+    if (!graph || n < 2) {
+        issue();
+        NRQ_HIP(hipGetLastError());
     } else {
-        for (int64_t k = 0; k < n; ++k) {
-            launch_fused<R>(v, cur(h->cur), cur(h->cur ^ 1), ef, g, dt);
-            h->cur ^= 1;
+        std::vector<const void *> key = {(const void *)(intptr_t)g.n0, (const void *)(intptr_t)g.n1,
+                                         (const void *)(intptr_t)g.n2, (const void *)(intptr_t)v,
+                                         (const void *)(intptr_t)sizeof(R)};
+        for (int i = 0; i < nb; ++i) {
+            key.push_back(hs[i]->u[cur[i]]);
+            key.push_back(hs[i]->u[cur[i] ^ 1]);
+            key.push_back(hs[i]->e);
         }
+        hipGraphExec_t exec = nullptr;
+        for (const lf_graph &c : g_lf_graphs)
+            if (c.n == n && c.sw == sw && c.key == key) exec = c.exec;
+        if (!exec) {
+            hipGraph_t gr = nullptr;
+            NRQ_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
+            issue();
+            const hipError_t le = hipGetLastError();
+            const hipError_t ce = hipStreamEndCapture(stream(), &gr);
+            NRQ_HIP(le);
+            NRQ_HIP(ce);
+            const hipError_t ie = hipGraphInstantiate(&exec, gr, nullptr, nullptr, 0);
+            (void)hipGraphDestroy(gr);
+            NRQ_HIP(ie);
+            if (g_lf_graphs.size() >= LF_GRAPH_CACHE) {
+                (void)hipGraphExecDestroy(g_lf_graphs.front().exec);
+                g_lf_graphs.erase(g_lf_graphs.begin());
+            }
+            g_lf_graphs.push_back({key, n, sw, exec});
+        }
+        NRQ_HIP(hipGraphLaunch(exec, stream()));
     }
-    NRQ_HIP(hipGetLastError());
+    for (int i = 0; i < nb; ++i)
+        if (flips & 1) hs[i]->cur ^= 1;
     return NRQ_OK;
+}
+
+template <typename R> static int leapfrog_t(nrq_gauge *h, double sw, int64_t n)
+{
+    nrq_gauge *hs[1] = {h};
+    return leapfrog_batch_t<R>(hs, 1, sw, n, false);
 }
 
 int leapfrog_choice(const nrq_gauge *h, int *bs, int *xcd, int *mode, int *nt, int *rec)
@@ -848,6 +933,40 @@ extern "C" int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps
         nsteps -= n;
         if (h->reunit_interval > 0 && h->steps_since_reunit >= h->reunit_interval)
             if (int r = nrq_gauge_reunitarise(h)) return r;
+    }
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_leapfrog_batch(nrq_gauge *const *h, int n, double stepwidth, int64_t nsteps)
+{
+    if (!h || n < 1) return fail(NRQ_ERR_ARG, "leapfrog batch: no configurations");
+    if (nsteps < 0 || !std::isfinite(stepwidth)) return fail(NRQ_ERR_ARG, "leapfrog: bad step count or width");
+    for (int i = 0; i < n; ++i) {
+        NEED_GAUGE(h[i]);
+        if (h[i]->precision != h[0]->precision) return fail(NRQ_ERR_ARG, "leapfrog batch: mixed storage precisions");
+        for (int j = 0; j < i; ++j)
+            if (h[j] == h[i]) return fail(NRQ_ERR_ARG, "leapfrog batch: configuration %d listed twice", i);
+    }
+    const char *env = getenv("NRQ_GRAPH");
+    const bool graph = !(env && env[0] == '0');
+    while (nsteps > 0) {
+        // stop at the next reunitarisation point of any configuration
+        int64_t m = nsteps;
+        for (int i = 0; i < n; ++i)
+            if (h[i]->reunit_interval > 0)
+                m = std::min(m, std::max<int64_t>(1, h[i]->reunit_interval - h[i]->steps_since_reunit));
+        for (int i0 = 0; i0 < n; i0 += LF_MAXB) {
+            const int nb = std::min(LF_MAXB, n - i0);
+            if (int r = h[0]->precision == 64 ? leapfrog_batch_t<double>(h + i0, nb, stepwidth, m, graph)
+                                              : leapfrog_batch_t<float>(h + i0, nb, stepwidth, m, graph))
+                return r;
+        }
+        nsteps -= m;
+        for (int i = 0; i < n; ++i) {
+            h[i]->steps_since_reunit += m;
+            if (h[i]->reunit_interval > 0 && h[i]->steps_since_reunit >= h[i]->reunit_interval)
+                if (int r = nrq_gauge_reunitarise(h[i])) return r;
+        }
     }
     return NRQ_OK;
 }

@@ -12,7 +12,8 @@
 
 namespace nrq {
 
-constexpr int NSLOTS = 24; // colour-vector fields per NRQCDField (and max batch)
+constexpr int NFIELDS_PER_HEAVY = 24; // colour-vector fields per NRQCDField
+constexpr int NSLOTS = 48;            // right-hand sides of one batched solve
 
 enum krylov_kind {
     KRYLOV_GMRES_FP64 = 0,  // GMRES(m), classical Gram-Schmidt, FP64, zero guess
@@ -30,14 +31,17 @@ struct krylov_params {
     bool extrapolate = false;  // first-order initial guess instead of zero
 };
 
-// One CN step of R fields of the 24-field storage psi ([field][colour][site]
-// complex doubles): for q < R and f = fields[q],
-//   y_q = (1 + alpha_q K) psi_f,  solve (1 + beta_q K) x_q = y_q,  psi_f = x_q
+// One CN step of R colour-vector fields (3V complex doubles each, colour
+// major) at fields[q]: for q < R,
+//   y_q = (1 + alpha_q K) psi_q,  solve (1 + beta_q K) x_q = y_q,  psi_q = x_q
 // to an FP64 true relative residual <= p.tol.  Statistics accumulate into st.
-int krylov_cn_step(const nrq_gauge *gf, cplx<double> *psi, const int *fields, int R, const cplx<double> *alpha,
+// The workspace must hold R slots (krylov_reserve).
+int krylov_cn_step(const nrq_gauge *gf, cplx<double> *const *fields, int R, const cplx<double> *alpha,
                    const cplx<double> *beta, const krylov_params &p, nrq_solve_stats *st);
 
-int krylov_reserve(int64_t vol, int m);
+// workspace for lattice volume vol, GMRES restart m and at least `slots`
+// right-hand sides (0: per-slot scalars only)
+int krylov_reserve(int64_t vol, int m, int slots);
 void krylov_release();
 // squared norms of nf consecutive FP64 fields into out[nf] (host)
 int krylov_norms2(const cplx<double> *x, int nf, double *out);

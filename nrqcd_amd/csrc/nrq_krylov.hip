@@ -549,7 +549,8 @@ struct workspace {
     int64_t vol = 0;
     int m = 0;
     int64_t nsb = 0;                   // site blocks of the stencil launches
-    void *basis = nullptr;             // (m+1) x NSLOTS x 3V complex double (or float)
+    int rcap = 0;                      // slots the field buffers hold
+    void *basis = nullptr;             // (m+1) x rcap x 3V complex double (or float)
     cplx<double> *y = nullptr, *x = nullptr, *t = nullptr, *res = nullptr;
     cplx<float> *y32 = nullptr, *x32 = nullptr, *t32 = nullptr, *res32 = nullptr;
     cplx<float> *links32 = nullptr;    // FP32 copy of the links for the inner solves
@@ -557,14 +558,13 @@ struct workspace {
     cplx<double> *cheb = nullptr;      // Chebyshev coefficients [iteration][slot][2]
     double *rpart = nullptr, *spart = nullptr, *nrm = nullptr;
     int *active = nullptr, *nused = nullptr;
-    cplx<double> *psi = nullptr;       // heavy-field storage of the current CN step
-    int fields[NSLOTS] = {};           // its field of each slot
+    cplx<double> *fptr[NSLOTS] = {};   // heavy field of each slot in the current CN step
     cplx<double> *hbuf = nullptr;      // pinned host
     double *nbuf = nullptr;            // pinned host (NRM doubles)
 
-    // nrm layout: [0, 32) working norms, [32, 64) ||y||^2, [64, 96) saved
-    // refinement ||r||^2, [96, 128) GMRES restart norms
-    static constexpr int NRM = 128;
+    // nrm layout, NSLOTS each: working norms, ||y||^2, saved refinement
+    // ||r||^2, GMRES restart norms
+    static constexpr int NRM_Y = NSLOTS, NRM_SAVED = 2 * NSLOTS, NRM_RESTART = 3 * NSLOTS, NRM = 4 * NSLOTS;
 
     void release()
     {
@@ -577,11 +577,15 @@ struct workspace {
         *this = workspace();
     }
 
-    int reserve(int64_t v, int mm)
+    // Buffers for lattice volume v, GMRES restart mm and at least r slots
+    // (r = 0: the per-slot scalars only; field buffers are kept).
+    int reserve(int64_t v, int mm, int r)
     {
-        if (v == vol && mm == m) return NRQ_OK;
+        r = std::max(r, rcap);
+        if (v == vol && mm == m && r == rcap) return NRQ_OK;
         release();
-        const size_t f64 = (size_t)3 * (size_t)v * sizeof(cplx<double>) * NSLOTS;
+        const int rc = std::max(r, 1);
+        const size_t f64 = (size_t)3 * (size_t)v * sizeof(cplx<double>) * rc;
         const size_t f32 = f64 / 2;
         int64_t sb = 0;
         for (int md = 0; md <= 2; ++md) sb = std::max(sb, make_lf_map(lat().g, SBS, 8, md, 1).nsb);
@@ -603,17 +607,18 @@ struct workspace {
         if (!ok) {
             (void)hipGetLastError();
             release();
-            return fail(NRQ_ERR_ALLOC, "CN: workspace allocation failed (V=%lld, m=%d)", (long long)v, mm);
+            return fail(NRQ_ERR_ALLOC, "CN: workspace allocation failed (V=%lld, m=%d, %d slots)", (long long)v, mm, rc);
         }
         vol = v;
         m = mm;
+        rcap = rc;
         nsb = sb;
         return NRQ_OK;
     }
 };
 static workspace g_ws;
 
-int krylov_reserve(int64_t vol, int m) { return g_ws.reserve(vol, m); }
+int krylov_reserve(int64_t vol, int m, int slots) { return g_ws.reserve(vol, m, slots); }
 void krylov_release() { g_ws.release(); }
 double *krylov_reduce_scratch() { return g_ws.rpart; }
 double *krylov_host_scratch() { return g_ws.nbuf; }
@@ -689,7 +694,7 @@ template <typename T> static slots<T> ws_slots(const cplx<T> *base, int R) { ret
 static slots<double> psi_slots(int R)
 {
     slots<double> t = {};
-    for (int r = 0; r < R; ++r) t.p[r] = g_ws.psi + (int64_t)g_ws.fields[r] * 3 * g_ws.vol;
+    for (int r = 0; r < R; ++r) t.p[r] = g_ws.fptr[r];
     return t;
 }
 
@@ -772,7 +777,7 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
         SN(R, std::vector<zc>(m)), Y(R, std::vector<zc>(m));
     std::vector<std::vector<double>> CS(R, std::vector<double>(m));
     std::vector<int> jused(R, 0), cycle(R);
-    double *restart_n2 = g_ws.nrm + 96;
+    double *restart_n2 = g_ws.nrm + workspace::NRM_RESTART;
     while (true) {
         bool any = false;
         for (int r = 0; r < R; ++r) any = any || act[r];
@@ -1047,7 +1052,7 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
         return k;
     };
     auto bn_from_host = [&]() {
-        for (int r = 0; r < R; ++r) bn[r] = std::sqrt(std::max(g_ws.nbuf[32 + r], 0.0));
+        for (int r = 0; r < R; ++r) bn[r] = std::sqrt(std::max(g_ws.nbuf[workspace::NRM_Y + r], 0.0));
     };
     *in_psi = false;
     if (p.kind == KRYLOV_CHEB_FP64) {
@@ -1166,7 +1171,7 @@ static int gmres_mixed(const cplx<RL> *u, int R, std::vector<double> &bn, const 
         if (!any) break;
         ++cycles;
         // ||r||^2 of this refinement step at nrm[64..): the inner solve reuses nrm[0..R)
-        double *rnorm2 = g_ws.nrm + 64;
+        double *rnorm2 = g_ws.nrm + workspace::NRM_SAVED;
         NRQ_HIP(hipMemcpyAsync(rnorm2, g_ws.nrm, sizeof(double) * R, hipMemcpyDeviceToDevice, stream()));
         // normalised FP32 correction problem A d = r / ||r||
         if (int e = upload_active(act)) return e;
@@ -1189,7 +1194,7 @@ static int gmres_mixed(const cplx<RL> *u, int R, std::vector<double> &bn, const 
 }
 
 template <typename RL>
-static int cn_step_t(const cplx<RL> *u, cplx<double> *psi, int R, const std::vector<zc> &alpha, const std::vector<zc> &beta,
+static int cn_step_t(const cplx<RL> *u, int R, const std::vector<zc> &alpha, const std::vector<zc> &beta,
                      const krylov_params &p, nrq_solve_stats *st)
 {
     const int64_t n = 3 * g_ws.vol;
@@ -1206,7 +1211,7 @@ static int cn_step_t(const cplx<RL> *u, cplx<double> *psi, int R, const std::vec
     NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, tpsi, ty, tx,
                                        g_ws.cpar, g_ws.cpar + NSLOTS, gs, R, lat().g, mp, g_ws.spart));
     NRQ_HIP(hipGetLastError());
-    if (int e = fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm + 32)) return e;
+    if (int e = fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm + workspace::NRM_Y)) return e;
     cplx<double> *xsol = g_ws.x;
     bool in_psi = false;
     int status = NRQ_OK;
@@ -1229,7 +1234,7 @@ static int cn_step_t(const cplx<RL> *u, cplx<double> *psi, int R, const std::vec
         }
     } else {
         if (int e = nrm_to_host()) return e;
-        for (int r = 0; r < R; ++r) bn[r] = std::sqrt(std::max(g_ws.nbuf[32 + r], 0.0));
+        for (int r = 0; r < R; ++r) bn[r] = std::sqrt(std::max(g_ws.nbuf[workspace::NRM_Y + r], 0.0));
         if (p.kind == KRYLOV_GMRES_FP64)
             status = gmres_core<double, RL>(u, R, g_ws.y, g_ws.x, (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, true,
                                             bn.data(), act, p.tol, p.restart, p.maxit, its, rel.data(), cycles);
@@ -1259,30 +1264,29 @@ static int cn_step_t(const cplx<RL> *u, cplx<double> *psi, int R, const std::vec
     return NRQ_OK;
 }
 
-int krylov_cn_step(const nrq_gauge *gf, cplx<double> *psi, const int *fields, int R, const cplx<double> *alpha,
+int krylov_cn_step(const nrq_gauge *gf, cplx<double> *const *fields, int R, const cplx<double> *alpha,
                    const cplx<double> *beta, const krylov_params &p, nrq_solve_stats *st)
 {
     if (R <= 0) return NRQ_OK;
     if (R > NSLOTS) return fail(NRQ_ERR_ARG, "CN: %d fields exceed the batch of %d", R, NSLOTS);
-    if (g_ws.vol != lat().g.vol) return fail(NRQ_ERR_STATE, "CN: workspace not reserved for this lattice");
+    if (g_ws.vol != lat().g.vol || g_ws.rcap < R) return fail(NRQ_ERR_STATE, "CN: workspace not reserved for this solve");
     std::vector<int> act(R, 1);
     std::vector<cplx<double>> par(2 * NSLOTS, cplx<double>{0.0, 0.0});
     std::vector<zc> a(R), b(R);
     for (int r = 0; r < R; ++r) {
-        if (fields[r] < 0 || fields[r] >= NSLOTS) return fail(NRQ_ERR_ARG, "CN: field %d out of range", fields[r]);
-        g_ws.fields[r] = fields[r];
+        if (!fields[r]) return fail(NRQ_ERR_ARG, "CN: null field %d", r);
+        g_ws.fptr[r] = fields[r];
         a[r] = zc(alpha[r].re, alpha[r].im);
         b[r] = zc(beta[r].re, beta[r].im);
         par[r] = alpha[r];
         const zc g = a[r] - b[r];
         par[NSLOTS + r] = {g.real(), g.imag()};
     }
-    g_ws.psi = psi;
     if (int e = upload_active(act)) return e;
     NRQ_HIP(hipMemcpyAsync(g_ws.cpar, par.data(), sizeof(cplx<double>) * par.size(), hipMemcpyHostToDevice, stream()));
     NRQ_HIP(hipMemcpyAsync(g_ws.beta, beta, sizeof(cplx<double>) * R, hipMemcpyHostToDevice, stream()));
-    if (gf->precision == 64) return cn_step_t<double>((const cplx<double> *)gauge_links(gf), psi, R, a, b, p, st);
-    return cn_step_t<float>((const cplx<float> *)gauge_links(gf), psi, R, a, b, p, st);
+    if (gf->precision == 64) return cn_step_t<double>((const cplx<double> *)gauge_links(gf), R, a, b, p, st);
+    return cn_step_t<float>((const cplx<float> *)gauge_links(gf), R, a, b, p, st);
 }
 
 } // namespace nrq

@@ -10,7 +10,7 @@ module gaugeconfiguration_su3
   use nrq_capi
   implicit none
   private
-  public :: GaugeConfiguration
+  public :: GaugeConfiguration, UpdateEnsemble
 
   character(len=22), parameter, public :: modulename = 'gaugeconfiguration_su3'
   integer(int8), parameter, public :: nSUN = 3_int8, nGen = 8_int8
@@ -225,6 +225,27 @@ contains
     if (present(nsteps)) n = nsteps
     call nrq_check(nrq_gauge_leapfrog(GaugeConf%handle, sw, n), 'GaugeConfiguration%Update')
   end subroutine Update_Leapfrog
+
+  !> Update for an ensemble of configurations of one storage precision: one
+  !! launch per pass over all members, replayed from a captured hipGraph.
+  subroutine UpdateEnsemble(GaugeConfs, StepWidth_, nsteps)
+    type(GaugeConfiguration), intent(inout) :: GaugeConfs(:)
+    real(fp), intent(in), optional :: StepWidth_
+    integer(int64), intent(in), optional :: nsteps
+    type(c_ptr), allocatable :: h(:)
+    real(fp) :: sw
+    integer(int64) :: n
+    integer :: k
+    sw = 1._fp
+    if (present(StepWidth_)) sw = StepWidth_
+    n = 1
+    if (present(nsteps)) n = nsteps
+    allocate(h(size(GaugeConfs)))
+    do k = 1, size(GaugeConfs)
+       h(k) = GaugeConfs(k)%handle
+    end do
+    call nrq_check(nrq_gauge_leapfrog_batch(h, int(size(GaugeConfs), c_int), sw, n), 'UpdateEnsemble')
+  end subroutine UpdateEnsemble
 
   !> Copies links(3,3,3,V) / efield(8,3,V) in the reference layout to the device.
   subroutine Upload(GaugeConf, links, efield)

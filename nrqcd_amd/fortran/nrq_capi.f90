@@ -121,6 +121,13 @@ module nrq_capi
        real(c_double), value :: sw
        integer(c_int64_t), value :: n
      end function nrq_gauge_leapfrog
+     integer(c_int) function nrq_gauge_leapfrog_batch(h, n, sw, nsteps) bind(C, name='nrq_gauge_leapfrog_batch')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), intent(in) :: h(*)
+       integer(c_int), value :: n
+       real(c_double), value :: sw
+       integer(c_int64_t), value :: nsteps
+     end function nrq_gauge_leapfrog_batch
      integer(c_int) function nrq_gauge_observables(h, out) bind(C, name='nrq_gauge_observables')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h
@@ -192,6 +199,16 @@ module nrq_capi
        complex(c_double_complex), intent(in) :: c(4)
        type(nrq_solve_stats), intent(out) :: stats
      end function nrq_heavy_update
+     integer(c_int) function nrq_heavy_update_batch(h, mass, n, g, c, sw, stats) bind(C, name='nrq_heavy_update_batch')
+       import :: c_int, c_ptr, c_double, c_double_complex, nrq_solve_stats
+       type(c_ptr), intent(in) :: h(*)
+       real(c_double), intent(in) :: mass(*)
+       integer(c_int), value :: n
+       type(c_ptr), value :: g
+       real(c_double), value :: sw
+       complex(c_double_complex), intent(in) :: c(4)
+       type(nrq_solve_stats), intent(out) :: stats
+     end function nrq_heavy_update_batch
      integer(c_int) function nrq_heavy_norms(h, nq, na) bind(C, name='nrq_heavy_norms')
        import :: c_int, c_ptr, c_double
        type(c_ptr), value :: h

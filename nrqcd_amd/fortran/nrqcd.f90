@@ -10,7 +10,7 @@ module nrqcd
   use gaugeconfiguration_su3, only: SU3GaugeConfiguration => GaugeConfiguration
   implicit none
   private
-  public :: NRQCDField, InitModule, FinalizeModule
+  public :: NRQCDField, InitModule, FinalizeModule, UpdateBatch
 
   character(len=5), parameter, public :: modulename = 'nrqcd'
   integer(int8), parameter, public :: nDof = 6_int8
@@ -122,6 +122,30 @@ contains
          'NRQCDField%Update')
     TotalKrylovIterations = TotalKrylovIterations + st%iterations
   end subroutine Update_CrankNicholson
+
+  !> Update for several heavy fields with their own masses on one gauge
+  !! configuration, as one batched multi-right-hand-side solve.
+  subroutine UpdateBatch(HeavyFields, GaugeConf, Masses, WilsonCoeffs, StepWidth)
+    type(NRQCDField), intent(inout) :: HeavyFields(:)
+    type(SU3GaugeConfiguration), intent(in) :: GaugeConf
+    real(fp), intent(in) :: Masses(:)
+    complex(fp), intent(in) :: WilsonCoeffs(nWilsonCoefficients)
+    real(fp), intent(in), optional :: StepWidth
+    type(c_ptr), allocatable :: h(:)
+    real(fp) :: sw
+    type(nrq_solve_stats) :: st
+    integer :: k
+    sw = 1._fp
+    if (present(StepWidth)) sw = StepWidth
+    if (size(Masses) /= size(HeavyFields)) error stop 'UpdateBatch: one mass per field'
+    allocate(h(size(HeavyFields)))
+    do k = 1, size(HeavyFields)
+       h(k) = HeavyFields(k)%handle
+    end do
+    call nrq_check(nrq_heavy_update_batch(h, Masses, int(size(HeavyFields), c_int), GaugeConf%CHandle(), &
+         WilsonCoeffs, sw, st), 'UpdateBatch')
+    TotalKrylovIterations = TotalKrylovIterations + st%iterations
+  end subroutine UpdateBatch
 
   !> GetNorm_Quark, nrqcd.f90:547-557
   real(fp) function GetNorm_Quark(HeavyField)
