@@ -1,0 +1,176 @@
+"""Full-size workloads beside the single-configuration benchmark.
+
+    python -m nrqcd_amd.workloads mixed      # 128^3 FP32 storage vs FP64, CN solvers at 128^3
+    python -m nrqcd_amd.workloads ensemble   # 32 x 48^3 batched ensemble with 4 masses each
+
+Each prints one JSON object.  Inputs are synthetic (the seeded box initial
+condition of gaugeconfiguration_su3.f90:906-951); all timing uses HIP events
+on the library's stream.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import sys
+import time
+
+import numpy as np
+
+from . import (GaugeConfiguration, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
+               NRQ_SOLVER_CHEBYSHEV_MIXED, NRQ_SOLVER_GMRES_FP64, NRQ_SOLVER_GMRES_MIXED, event_elapsed_ms,
+               event_record, lattice_finalize, lattice_init, leapfrog_batch, nrqcd_finalize, nrqcd_init,
+               nrqcd_set_solver, random_init, synchronize, update_batch)
+
+SPACINGS = (0.05, 1.0, 1.0, 1.0)
+C1 = [1, 0, 0, 0]
+
+
+def _timed(fn) -> float:
+    synchronize()
+    event_record(0)
+    fn()
+    event_record(1)
+    return event_elapsed_ms(0, 1)
+
+
+def mixed(size: int = 128, steps: int = 500, seed: int = 3, cn_steps: int = 5, log_every: int = 50) -> dict:
+    """FP32 link/E storage with FP64 reunitarisation every 10 steps against the
+    all-FP64 run from the same seed (observables in FP64 sums), then CN steps/s
+    of every solver at this size on the FP64 configuration."""
+    lattice_init((size,) * 3, SPACINGS)
+    runs = {}
+    for name, prec in (("fp64", NRQ_PRECISION_FP64), ("fp32", NRQ_PRECISION_FP32)):
+        random_init(seed)
+        g = GaugeConfiguration(prec)
+        g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+        if prec == NRQ_PRECISION_FP32:
+            g.SetReunitarisationInterval(10)
+        log = [dict(step=0, **g.observables())]
+        ms = 0.0
+        for k in range(0, steps, log_every):
+            n = min(log_every, steps - k)
+            ms += _timed(lambda: g.Update(1.0, n))
+            log.append(dict(step=k + n, **g.observables()))
+        runs[name] = {"g": g, "log": log, "ms_per_step": ms / steps}
+    err = {}
+    for key in ("energy", "gauss_max", "plaquette"):
+        a = np.array([o[key] for o in runs["fp64"]["log"]])
+        b = np.array([o[key] for o in runs["fp32"]["log"]])
+        err[key] = {"max_abs": float(np.max(np.abs(a - b))),
+                    "max_rel": float(np.max(np.abs(a - b) / np.maximum(np.abs(a), 1e-300)))}
+    e = np.array([o["energy"] for o in runs["fp64"]["log"]])
+    out = {
+        "workload": f"{size}^3 box IC Qs=0.5 seed {seed}, {steps} leapfrog steps",
+        "leapfrog_us_per_step": {k: 1e3 * v["ms_per_step"] for k, v in runs.items()},
+        "link_updates_per_s": {k: 3 * size ** 3 / (1e-3 * v["ms_per_step"]) for k, v in runs.items()},
+        "fp32_vs_fp64_error": err,
+        "fp64_energy_drift_rel": float((e[-1] - e[0]) / e[0]),
+        "log": {k: v["log"] for k, v in runs.items()},
+    }
+    g64 = runs["fp64"]["g"]
+    del runs
+    nrqcd_init(2.0, C1, 1.0, 1e-10)
+    cn = {}
+    for name, kind in (("plain_fp64_gmres", NRQ_SOLVER_GMRES_FP64), ("mixed_gmres", NRQ_SOLVER_GMRES_MIXED),
+                       ("fp64_chebyshev", NRQ_SOLVER_CHEBYSHEV_FP64), ("mixed_chebyshev", NRQ_SOLVER_CHEBYSHEV_MIXED)):
+        nrqcd_set_solver(kind, 30, 10000)
+        h = NRQCDField()
+        h.InitSinglePoint(1, 1)
+        h.Update(g64, 2.0, C1)
+        stats = []
+        ms = _timed(lambda: stats.extend(h.Update(g64, 2.0, C1) for _ in range(cn_steps)))
+        cn[name] = {"cn_steps_per_s": cn_steps / (1e-3 * ms), "ms_per_step": ms / cn_steps,
+                    "worst_true_relres": max(s.max_true_relres for s in stats),
+                    "iterations_per_column_solve": sum(s.iterations for s in stats) / sum(s.solves for s in stats)}
+        del h
+    nrqcd_finalize()
+    plain = cn["plain_fp64_gmres"]["cn_steps_per_s"]
+    out["cn"] = cn
+    out["cn_speedup_vs_plain"] = {k: v["cn_steps_per_s"] / plain for k, v in cn.items()}
+    del g64
+    lattice_finalize()
+    return out
+
+
+def ensemble(members: int = 32, size: int = 48, steps: int = 500, cn_steps: int = 10, precision: int = 32,
+             first_seed: int = 100) -> dict:
+    """`members` configurations (seeds first_seed.., Qs uniform in [0.3, 0.7]
+    from a separate generator) advanced by one batched launch per pass, then
+    CN steps of 4 heavy fields per configuration (Ma = 1, 2, 5, 10 at random
+    sites) as one multi-right-hand-side solve per configuration."""
+    lattice_init((size,) * 3, SPACINGS)
+    vol = size ** 3
+    rng = np.random.default_rng(1000)  # Qs and source sites, apart from the ranlxd stream of each seed
+    qs = rng.uniform(0.3, 0.7, members)
+    configs = []
+    t0 = time.perf_counter()
+    for k in range(members):
+        random_init(first_seed + k)
+        g = GaugeConfiguration(NRQ_PRECISION_FP32 if precision == 32 else NRQ_PRECISION_FP64)
+        g.TransversePolarisedOccupiedInit_Box(float(qs[k]), 1.0, 1.0)
+        if precision == 32:
+            g.SetReunitarisationInterval(10)
+        configs.append(g)
+    init_s = time.perf_counter() - t0
+    leapfrog_batch(configs, 1.0, 10)  # tuning and graph capture outside the timed region
+    ms = _timed(lambda: leapfrog_batch(configs, 1.0, steps))
+    e = [c.GetEnergy() for c in configs]
+    out = {
+        "workload": f"{members} x {size}^3 ensemble, FP{precision} storage, seeds {first_seed}..{first_seed + members - 1}",
+        "initial_conditions_s": init_s,
+        "leapfrog_ms_per_step": ms / steps,
+        "link_updates_per_s": 3 * vol * members * steps / (1e-3 * ms),
+        "energies": e,
+    }
+    masses = [1.0, 2.0, 5.0, 10.0]
+    nrqcd_init(2.0, C1, 1.0, 1e-10)
+    nrqcd_set_solver(NRQ_SOLVER_CHEBYSHEV_MIXED, 30, 10000)
+    fields = []
+    for k in range(members):
+        sites = rng.integers(1, vol + 1, size=(4, 2))
+        fs = []
+        for (iq, ia) in sites:
+            h = NRQCDField()
+            h.InitSinglePoint(int(iq), int(ia))
+            fs.append(h)
+        fields.append(fs)
+    for k in range(members):
+        update_batch(fields[k], configs[k], masses, C1)  # workspace
+    stats = []
+
+    def cn_round():
+        for _ in range(cn_steps):
+            for k in range(members):
+                stats.append(update_batch(fields[k], configs[k], masses, C1))
+
+    ms = _timed(cn_round)
+    out["cn"] = {"solver": "FP32 Chebyshev corrections, FP64 outer residual 1e-10",
+                 "configuration_cn_steps_per_s": members * cn_steps / (1e-3 * ms),
+                 "column_solves_per_s": sum(s.solves for s in stats) / (1e-3 * ms),
+                 "worst_true_relres": max(s.max_true_relres for s in stats),
+                 "norms_member0": [f.norms() for f in fields[0]]}
+    del fields
+    nrqcd_finalize()
+    del configs
+    lattice_finalize()
+    return out
+
+
+def main(argv=None) -> None:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("workload", choices=("mixed", "ensemble"))
+    ap.add_argument("--size", type=int)
+    ap.add_argument("--steps", type=int)
+    ap.add_argument("--members", type=int, default=32)
+    ap.add_argument("--cn-steps", type=int)
+    a = ap.parse_args(argv)
+    kw = {k: v for k, v in (("size", a.size), ("steps", a.steps), ("cn_steps", a.cn_steps)) if v is not None}
+    if a.workload == "mixed":
+        out = mixed(**kw)
+    else:
+        out = ensemble(members=a.members, **kw)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    sys.exit(main())
