@@ -195,6 +195,8 @@ typedef struct {
                                         FP64, first-order guess, GMRES fallback */
 #define NRQ_SOLVER_CHEBYSHEV_MIXED 3 /* FP32 Chebyshev corrections inside FP64
                                         iterative refinement */
+#define NRQ_SOLVER_FGMRES_CNN 4      /* FGMRES(m) in FP64, right-preconditioned by
+                                        the learned network (nrq_cn_set_preconditioner) */
 
 /* nrqcd::InitModule(Mass, WilsonCoeffs(4), StepWidth, kspTol),
  * nrqcd.f90:167-192 with KSPSetTolerances(rtol=kspTol), nrqcd.f90:803-804.
@@ -243,6 +245,42 @@ int nrq_heavy_norms(const nrq_heavy *h, double *norm_quark, double *norm_antiq);
 /* GetMesoncorrelator_{O1_1S0, O1_3S1, O8_1S0, O8_3S1}_ZeroMomentum,
  * nrqcd.f90:1484-1661; out = 4 complex numbers in that order */
 int nrq_heavy_correlators(const nrq_heavy *h, double out[8]);
+
+/* ---- learned preconditioner ------------------------------------------
+ * No reference counterpart (the reference uses PETSc's ILU(0)/block-Jacobi,
+ * nrqcd.f90:786-804).  A 4-layer 3-D convolutional network N acting on three
+ * colour-vector fields at a time (18 real channels per site), 32 channels
+ * wide, 3^3 periodic kernels, ReLU, no biases, BF16 weights and activations
+ * with FP32 accumulation on matrix cores.  FGMRES uses z = v + N(v) as its
+ * right preconditioner.  The lattice x extent must be a multiple of 16. */
+typedef struct nrq_cnn nrq_cnn;
+#define NRQ_CNN_NWEIGHTS (4 * 27 * 32 * 32) /* [layer][tap][co][ci], tap = 9(dz+1)+3(dy+1)+(dx+1);
+                                               channels past 18 of layer 1 inputs and
+                                               layer 4 outputs are zero */
+/* He-initialised weights (the last layer scaled by 0.1) from `seed` */
+int nrq_cnn_create(nrq_cnn **net, uint64_t seed);
+int nrq_cnn_destroy(nrq_cnn *net);
+int nrq_cnn_get_weights(const nrq_cnn *net, float *w);
+int nrq_cnn_set_weights(nrq_cnn *net, const float *w);
+/* out = N(in) on the current lattice; in, out: [site][18] floats (host) */
+int nrq_cnn_forward(nrq_cnn *net, const float *in, float *out);
+/* loss = sum (N(in) - target)^2 / norm and its gradient with respect to the
+ * NRQ_CNN_NWEIGHTS weights (host arrays, [site][18] layouts) */
+int nrq_cnn_loss_grad(nrq_cnn *net, const float *in, const float *target, double norm, double *loss, float *grad);
+/* Adds training pairs from `nsolves` plain FP64 GMRES solves (true relative
+ * residual 1e-12) of CN systems on g at `mass`: each solve takes 12 random
+ * Gaussian fields psi (6 quark, 6 antiquark), forms y = S psi and solves
+ * S^+ x = y; every three fields give one pair r = y/|y|, target
+ * (x - y)/|y| (the error of the zero guess minus the residual).  All pairs of
+ * one network come from one lattice size. */
+int nrq_cnn_harvest(nrq_cnn *net, const nrq_gauge *g, double mass, int nsolves, uint64_t seed);
+int nrq_cnn_dataset_size(const nrq_cnn *net, int64_t *count);
+/* `epochs` passes of Adam (learning rate lr, one pair per step, order
+ * shuffled from seed); mean_loss (may be NULL) = mean relative loss of the
+ * last pass */
+int nrq_cnn_train(nrq_cnn *net, int epochs, double lr, uint64_t seed, double *mean_loss);
+/* network used by NRQ_SOLVER_FGMRES_CNN (NULL: none) */
+int nrq_cn_set_preconditioner(nrq_cnn *net);
 
 #ifdef __cplusplus
 }

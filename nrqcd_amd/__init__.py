@@ -37,6 +37,8 @@ NRQ_SOLVER_GMRES_FP64 = 0
 NRQ_SOLVER_GMRES_MIXED = 1
 NRQ_SOLVER_CHEBYSHEV_FP64 = 2
 NRQ_SOLVER_CHEBYSHEV_MIXED = 3
+NRQ_SOLVER_FGMRES_CNN = 4
+NRQ_CNN_NWEIGHTS = 4 * 27 * 32 * 32
 
 _ERRORS = {
     -1: "NRQ_ERR_ARG",
@@ -137,6 +139,16 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_heavy_download": (I, [P, P, P]),
         "nrq_heavy_update": (I, [P, P, D, P, D, P]),
         "nrq_heavy_update_batch": (I, [P, P, I, P, P, D, P]),
+        "nrq_cnn_create": (I, [P, ctypes.c_uint64]),
+        "nrq_cnn_destroy": (I, [P]),
+        "nrq_cnn_get_weights": (I, [P, P]),
+        "nrq_cnn_set_weights": (I, [P, P]),
+        "nrq_cnn_forward": (I, [P, P, P]),
+        "nrq_cnn_loss_grad": (I, [P, P, P, D, P, P]),
+        "nrq_cnn_harvest": (I, [P, P, D, I, ctypes.c_uint64]),
+        "nrq_cnn_dataset_size": (I, [P, P]),
+        "nrq_cnn_train": (I, [P, I, D, ctypes.c_uint64, P]),
+        "nrq_cn_set_preconditioner": (I, [P]),
         "nrq_heavy_norms": (I, [P, P, P]),
         "nrq_heavy_correlators": (I, [P, P]),
     }
@@ -482,3 +494,68 @@ class NRQCDField:
     @property
     def handle(self) -> ctypes.c_void_p:
         return self._h
+
+
+class PreconditionerNet:
+    """The learned CN preconditioner (``nrq_cnn``): a 4-layer 3-D CNN on 18
+    real channels per site, trained on the GPU with the library's own
+    forward/backward kernels and used by ``NRQ_SOLVER_FGMRES_CNN``."""
+
+    def __init__(self, seed: int = 1):
+        self._h = ctypes.c_void_p()
+        _check(library().nrq_cnn_create(ctypes.byref(self._h), seed))
+
+    def __del__(self):
+        if getattr(self, "_h", None) and _lib is not None:
+            library().nrq_cnn_destroy(self._h)
+            self._h = ctypes.c_void_p()
+
+    @property
+    def weights(self) -> np.ndarray:
+        w = np.zeros(NRQ_CNN_NWEIGHTS, np.float32)
+        _check(library().nrq_cnn_get_weights(self._h, _ptr(w)))
+        return w
+
+    @weights.setter
+    def weights(self, w) -> None:
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        if w.size != NRQ_CNN_NWEIGHTS:
+            raise ValueError("expected %d weights" % NRQ_CNN_NWEIGHTS)
+        _check(library().nrq_cnn_set_weights(self._h, _ptr(w)))
+
+    def forward(self, inp: np.ndarray) -> np.ndarray:
+        """N(inp) for inp of shape (V, 18) on the current lattice."""
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        out = np.zeros_like(inp)
+        _check(library().nrq_cnn_forward(self._h, _ptr(inp), _ptr(out)))
+        return out
+
+    def loss_grad(self, inp: np.ndarray, target: np.ndarray, norm: float = 1.0):
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        loss = ctypes.c_double()
+        grad = np.zeros(NRQ_CNN_NWEIGHTS, np.float32)
+        _check(library().nrq_cnn_loss_grad(self._h, _ptr(inp), _ptr(target), norm, ctypes.byref(loss), _ptr(grad)))
+        return loss.value, grad
+
+    def harvest(self, GaugeConf, mass: float, nsolves: int, seed: int = 1) -> None:
+        _check(library().nrq_cnn_harvest(self._h, GaugeConf.handle, mass, nsolves, seed))
+
+    def dataset_size(self) -> int:
+        n = ctypes.c_int64()
+        _check(library().nrq_cnn_dataset_size(self._h, ctypes.byref(n)))
+        return n.value
+
+    def train(self, epochs: int, lr: float = 1e-3, seed: int = 1) -> float:
+        loss = ctypes.c_double()
+        _check(library().nrq_cnn_train(self._h, epochs, lr, seed, ctypes.byref(loss)))
+        return loss.value
+
+    @property
+    def handle(self) -> ctypes.c_void_p:
+        return self._h
+
+
+def nrqcd_set_preconditioner(net) -> None:
+    """Network used by NRQ_SOLVER_FGMRES_CNN (None: none)."""
+    _check(library().nrq_cn_set_preconditioner(net.handle if net is not None else None))

@@ -201,12 +201,16 @@ extern "C" int nrq_cn_finalize(void)
 extern "C" int nrq_cn_set_solver(int kind, int restart, int64_t maxit)
 {
     if (!g_cn.initialised) return fail(NRQ_ERR_STATE, "nrqcd: not initialised");
-    if (kind < NRQ_SOLVER_GMRES_FP64 || kind > NRQ_SOLVER_CHEBYSHEV_MIXED)
+    if (kind < NRQ_SOLVER_GMRES_FP64 || kind > NRQ_SOLVER_FGMRES_CNN)
         return fail(NRQ_ERR_UNSUPPORTED, "nrqcd: solver kind %d not available", kind);
     if (restart < 1 || restart > 200 || maxit < 1) return fail(NRQ_ERR_ARG, "nrqcd: bad restart or maxit");
-    static const krylov_kind kinds[] = {KRYLOV_GMRES_FP64, KRYLOV_GMRES_MIXED, KRYLOV_CHEB_FP64, KRYLOV_CHEB_MIXED};
+    static const krylov_kind kinds[] = {KRYLOV_GMRES_FP64, KRYLOV_GMRES_MIXED, KRYLOV_CHEB_FP64, KRYLOV_CHEB_MIXED,
+                                        KRYLOV_FGMRES_CNN};
     g_cn.solver.kind = kinds[kind];
-    g_cn.solver.extrapolate = kind != NRQ_SOLVER_GMRES_FP64;
+    // the fast solvers start from the first-order guess; plain GMRES and
+    // FGMRES start from zero (PETSc's default), so that FGMRES differs from
+    // GMRES only by the network
+    g_cn.solver.extrapolate = kind != NRQ_SOLVER_GMRES_FP64 && kind != NRQ_SOLVER_FGMRES_CNN;
     g_cn.solver.restart = restart;
     g_cn.solver.maxit = maxit;
     return NRQ_OK;

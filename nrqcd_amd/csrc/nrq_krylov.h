@@ -20,6 +20,7 @@ enum krylov_kind {
     KRYLOV_GMRES_MIXED = 1, // FP32 GMRES(m) corrections inside FP64 iterative refinement
     KRYLOV_CHEB_FP64 = 2,   // Chebyshev iteration on the known spectral segment, FP64
     KRYLOV_CHEB_MIXED = 3,  // FP32 Chebyshev corrections inside FP64 iterative refinement
+    KRYLOV_FGMRES_CNN = 4,  // FGMRES(m) in FP64, right-preconditioned by the learned network
 };
 
 struct krylov_params {
@@ -43,6 +44,8 @@ int krylov_cn_step(const nrq_gauge *gf, cplx<double> *const *fields, int R, cons
 // right-hand sides (0: per-slot scalars only)
 int krylov_reserve(int64_t vol, int m, int slots);
 void krylov_release();
+// right-hand sides y of the last krylov_cn_step, slot-major (3V complex each)
+const cplx<double> *krylov_last_rhs();
 // squared norms of nf consecutive FP64 fields into out[nf] (host)
 int krylov_norms2(const cplx<double> *x, int nf, double *out);
 // partial sums for the correlator kernel: device scratch of >= 8 x 256 doubles

@@ -9,6 +9,8 @@
 //    (PETSc's defaults).
 //  * KRYLOV_GMRES_MIXED: FP64 iterative refinement around FP32 GMRES(m)
 //    corrections.
+//  * KRYLOV_FGMRES_CNN: flexible GMRES(m) in FP64 whose right preconditioner
+//    is the learned network of nrq_cnn.hip (z = v + N(v)).
 //  * KRYLOV_CHEB_FP64 / KRYLOV_CHEB_MIXED: Chebyshev iteration on the known
 //    spectral segment of A_r, in FP64 or as FP32 corrections inside FP64
 //    iterative refinement.  Step counts are planned from a priori bounds, so
@@ -17,6 +19,7 @@
 // In every case ||y - A x|| <= tol ||y|| with the FP64 operator decides
 // convergence: precision and solver change the work, never the answer.
 #include "nrq_krylov.h"
+#include "nrq_cnn.h"
 #include "nrq_map.h"
 #include <algorithm>
 #include <cmath>
@@ -559,6 +562,8 @@ struct workspace {
     double *rpart = nullptr, *spart = nullptr, *nrm = nullptr;
     int *active = nullptr, *nused = nullptr;
     cplx<double> *fptr[NSLOTS] = {};   // heavy field of each slot in the current CN step
+    cplx<double> *zbasis = nullptr;    // FGMRES: preconditioned directions, m x rcap x 3V
+    int zm = 0;
     cplx<double> *hbuf = nullptr;      // pinned host
     double *nbuf = nullptr;            // pinned host (NRM doubles)
 
@@ -574,6 +579,7 @@ struct workspace {
             if (q) (void)hipFree(q);
         if (hbuf) (void)hipHostFree(hbuf);
         if (nbuf) (void)hipHostFree(nbuf);
+        if (zbasis) (void)hipFree(zbasis);
         *this = workspace();
     }
 
@@ -619,6 +625,21 @@ struct workspace {
 static workspace g_ws;
 
 int krylov_reserve(int64_t vol, int m, int slots) { return g_ws.reserve(vol, m, slots); }
+const cplx<double> *krylov_last_rhs() { return g_ws.y; }
+
+static int reserve_zbasis()
+{
+    if (g_ws.zbasis && g_ws.zm == g_ws.m) return NRQ_OK;
+    if (g_ws.zbasis) (void)hipFree(g_ws.zbasis);
+    g_ws.zbasis = nullptr;
+    const size_t b = (size_t)g_ws.m * g_ws.rcap * 3 * (size_t)g_ws.vol * sizeof(cplx<double>);
+    if (hipMalloc((void **)&g_ws.zbasis, b) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(NRQ_ERR_ALLOC, "CN: FGMRES basis allocation failed");
+    }
+    g_ws.zm = g_ws.m;
+    return NRQ_OK;
+}
 void krylov_release() { g_ws.release(); }
 double *krylov_reduce_scratch() { return g_ws.rpart; }
 double *krylov_host_scratch() { return g_ws.nbuf; }
@@ -749,8 +770,10 @@ int krylov_norms2(const cplx<double> *x, int nf, double *out)
 template <typename T, typename RL>
 static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, cplx<T> *basis, cplx<T> *t,
                       cplx<T> *res, bool zero_guess, const double *bnorm, std::vector<int> act, double tol, int m,
-                      int64_t maxit, std::vector<int64_t> &its, double *rel_out, int64_t &cycles)
+                      int64_t maxit, std::vector<int64_t> &its, double *rel_out, int64_t &cycles,
+                      nrq_cnn *net = nullptr, cplx<T> *zbasis = nullptr)
 {
+    // with net (FGMRES, T = double): z_j = M(v_j), w = A z_j, x += Z y
     typedef std::complex<double> zc;
     const int64_t n = 3 * g_ws.vol;
     const size_t fbytes = sizeof(cplx<T>) * (size_t)n;
@@ -802,7 +825,18 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
             if (int e = upload_active(cycle)) return e;
             cplx<T> *vj = basis + (size_t)j * R * n;
             cplx<T> *w = basis + (size_t)(j + 1) * R * n;
-            if (int e = apply_t<RL, T>(links, vj, w, R)) return e;
+            if (net) {
+                cplx<T> *zj = zbasis + (size_t)j * R * n;
+                cplx<double> *vp[NSLOTS], *zp[NSLOTS];
+                for (int r = 0; r < R; ++r) {
+                    vp[r] = (cplx<double> *)(vj + (size_t)r * n);
+                    zp[r] = (cplx<double> *)(zj + (size_t)r * n);
+                }
+                if (int e = cnn_precondition(net, vp, zp, R)) return e;
+                if (int e = apply_t<RL, T>(links, zj, w, R)) return e;
+            } else if (int e = apply_t<RL, T>(links, vj, w, R)) {
+                return e;
+            }
             const int nv = j + 1;
             hipLaunchKernelGGL(k_mdot<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<T> *)basis, nv,
                                (const cplx<T> *)w, n, R, g_ws.active, g_ws.cpart);
@@ -862,8 +896,8 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
         if (int e = upload_active(act)) return e;
         NRQ_HIP(hipMemcpyAsync(g_ws.ydev, g_ws.hbuf, sizeof(cplx<double>) * R * (m + 1), hipMemcpyHostToDevice, stream()));
         if (int e = upload_ints(g_ws.nused, nu)) return e;
-        hipLaunchKernelGGL(k_xupdate<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), x, (const cplx<T> *)basis, R, n,
-                           g_ws.ydev, m + 1, g_ws.nused, g_ws.active);
+        hipLaunchKernelGGL(k_xupdate<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), x,
+                           (const cplx<T> *)(net ? zbasis : basis), R, n, g_ws.ydev, m + 1, g_ws.nused, g_ws.active);
         // true residual in precision T
         if (int e = apply_t<RL, T>(links, x, t, R)) return e;
         hipLaunchKernelGGL(k_residual<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), b, (const cplx<T> *)t, res, n,
@@ -1238,7 +1272,14 @@ static int cn_step_t(const cplx<RL> *u, int R, const std::vector<zc> &alpha, con
         if (p.kind == KRYLOV_GMRES_FP64)
             status = gmres_core<double, RL>(u, R, g_ws.y, g_ws.x, (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res, true,
                                             bn.data(), act, p.tol, p.restart, p.maxit, its, rel.data(), cycles);
-        else
+        else if (p.kind == KRYLOV_FGMRES_CNN) {
+            nrq_cnn *net = cn_preconditioner();
+            if (!net) return fail(NRQ_ERR_STATE, "CN: FGMRES needs a network (nrq_cn_set_preconditioner)");
+            if (int e = reserve_zbasis()) return e;
+            status = gmres_core<double, RL>(u, R, g_ws.y, g_ws.x, (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res,
+                                            !p.extrapolate, bn.data(), act, p.tol, p.restart, p.maxit, its, rel.data(),
+                                            cycles, net, g_ws.zbasis);
+        } else
             status = gmres_mixed<RL>(u, R, bn, p, act, rel, its, cycles);
     }
     if (status) return status;

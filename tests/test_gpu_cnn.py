@@ -1,0 +1,148 @@
+"""GPU parity and effect of the learned CN preconditioner (nrq_cnn.hip).
+
+The checker is the numpy restatement oracle/cnn_oracle.py, which rounds to
+bfloat16 where the GPU does; the GPU accumulates in FP32 (the oracle in
+float64), so single hidden activations can round to the neighbouring bfloat16
+value: the bars are relative L2 errors of 1e-2 (forward) and 3e-2 (gradient).
+"""
+import numpy as np
+import pytest
+
+from oracle import cnn_oracle as co
+
+pytestmark = pytest.mark.gpu
+
+SP = (0.05, 1.0, 1.0, 1.0)
+C1 = [1.0, 0.0, 0.0, 0.0]
+
+
+def _say(msg):
+    print(msg, flush=True)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(a - b) / np.linalg.norm(b))
+
+
+def test_forward_matches_oracle(nrq, lattice):
+    ext = (16, 4, 3)
+    _say("cnn forward 16x4x3")
+    lattice(ext, SP)
+    rng = np.random.default_rng(4)
+    v = ext[0] * ext[1] * ext[2]
+    inp = rng.normal(size=(v, co.CIN)).astype(np.float32)
+    w = co.random_weights(rng).astype(np.float32)
+    net = nrq.PreconditionerNet(1)
+    net.weights = w
+    assert np.array_equal(net.weights, w)
+    out = net.forward(inp)
+    ref, _, _ = co.forward(inp.reshape(ext[2], ext[1], ext[0], co.CIN), w)
+    assert _rel(out, ref.reshape(v, co.CIN)) < 1e-2
+
+
+def test_loss_gradient_matches_oracle(nrq, lattice):
+    ext = (16, 3, 2)
+    _say("cnn loss/gradient 16x3x2")
+    lattice(ext, SP)
+    rng = np.random.default_rng(5)
+    v = ext[0] * ext[1] * ext[2]
+    inp = rng.normal(size=(v, co.CIN)).astype(np.float32)
+    tgt = rng.normal(size=(v, co.CIN)).astype(np.float32)
+    w = co.random_weights(rng).astype(np.float32)
+    net = nrq.PreconditionerNet(1)
+    net.weights = w
+    loss, grad = net.loss_grad(inp, tgt, 3.0)
+    shp = (ext[2], ext[1], ext[0], co.CIN)
+    rloss, rgrad = co.loss_grad(inp.reshape(shp), tgt.reshape(shp), w, 3.0)
+    assert loss == pytest.approx(rloss, rel=1e-3)
+    assert _rel(grad, rgrad) < 3e-2
+    # per layer too, so that an error in one layer cannot hide behind the others
+    g4, r4 = grad.reshape(4, -1), rgrad.reshape(4, -1)
+    for l in range(4):
+        assert _rel(g4[l], r4[l]) < 5e-2, l
+
+
+def _evolved(nrq, seed=7, steps=40):
+    nrq.random_init(seed)
+    g = nrq.GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    g.Update(1.0, steps)
+    return g
+
+
+def test_fgmres_identity_network_matches_gmres_and_oracle(nrq, oracle, lattice):
+    """With all weights zero the preconditioner is the identity: FGMRES takes
+    the same iterations as GMRES and both reproduce the oracle's CN step."""
+    ext = (16, 6, 5)
+    _say("FGMRES identity network")
+    lattice(ext, SP)
+    nrq.nrqcd_init(2.0, C1, 1.0, 1e-12)
+    try:
+        oracle.lattice_init(ext, SP)
+        U, E = oracle.box_init(8, 0.5, 1.0, 1.0)
+        g = nrq.GaugeConfiguration()
+        g.upload(U, E)
+        net = nrq.PreconditionerNet(1)
+        net.weights = np.zeros(nrq.NRQ_CNN_NWEIGHTS, np.float32)
+        nrq.nrqcd_set_preconditioner(net)
+        its = {}
+        res = {}
+        for name, kind in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64), ("fgmres", nrq.NRQ_SOLVER_FGMRES_CNN)):
+            nrq.nrqcd_set_solver(kind, 20, 10000)
+            h = nrq.NRQCDField()
+            h.InitSinglePoint(1, 17)
+            st = h.Update(g, 2.0, C1)
+            assert st.max_true_relres <= 1e-12
+            its[name] = st.iterations
+            res[name] = h.download()
+        Q, A = oracle.point_prop(1), oracle.point_prop(17)
+        oracle.cn_update(U, Q, A, 2.0, C1, 1.0, 1e-13)
+        for name in res:
+            assert np.max(np.abs(res[name][0] - Q)) < 1e-10 * np.max(np.abs(Q)), name
+            assert np.max(np.abs(res[name][1] - A)) < 1e-10 * np.max(np.abs(A)), name
+        assert its["fgmres"] == its["gmres"]
+        nrq.nrqcd_set_preconditioner(None)
+    finally:
+        nrq.nrqcd_finalize()
+
+
+def test_training_and_gain(nrq, oracle, lattice):
+    """Train on pairs from plain GMRES solves at Ma = 1, 2, 5 on an evolved
+    configuration; the loss falls, and on held-out masses 3 and 8 FGMRES with
+    the trained network reaches the same FP64 answer in no more iterations
+    than GMRES without it.  The iteration counts are printed."""
+    ext = (16, 16, 8)
+    _say("CNN training 16x16x8")
+    lattice(ext, SP)
+    nrq.nrqcd_init(2.0, C1, 1.0, 1e-10)
+    try:
+        g = _evolved(nrq)
+        net = nrq.PreconditionerNet(3)
+        for m in (1.0, 2.0, 5.0):
+            net.harvest(g, m, 4, seed=int(10 * m))
+        assert net.dataset_size() == 3 * 4 * 4
+        first = net.train(1, 1e-3, seed=1)
+        last = net.train(15, 1e-3, seed=2)
+        _say(f"training loss: after 1 epoch {first:.4g}, after 16 epochs {last:.4g}")
+        assert last < first
+        nrq.nrqcd_set_preconditioner(net)
+        for m in (3.0, 8.0):
+            out = {}
+            for name, kind in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64), ("fgmres_cnn", nrq.NRQ_SOLVER_FGMRES_CNN)):
+                nrq.nrqcd_set_solver(kind, 20, 10000)
+                h = nrq.NRQCDField()
+                h.InitSinglePoint(100, 900)
+                its = 0
+                for _ in range(3):
+                    st = h.Update(g, m, C1)
+                    assert st.max_true_relres <= 1e-10
+                    its += st.iterations
+                out[name] = (its, h.download())
+            _say(f"Ma={m}: iterations gmres {out['gmres'][0]}, fgmres+cnn {out['fgmres_cnn'][0]}")
+            assert out["fgmres_cnn"][0] <= out["gmres"][0]
+            for k in range(2):
+                a, b = out["fgmres_cnn"][1][k], out["gmres"][1][k]
+                assert np.max(np.abs(a - b)) < 1e-8 * np.max(np.abs(b))
+        nrq.nrqcd_set_preconditioner(None)
+    finally:
+        nrq.nrqcd_finalize()
