@@ -268,12 +268,13 @@ int nrq_cnn_forward(nrq_cnn *net, const float *in, float *out);
  * NRQ_CNN_NWEIGHTS weights (host arrays, [site][18] layouts) */
 int nrq_cnn_loss_grad(nrq_cnn *net, const float *in, const float *target, double norm, double *loss, float *grad);
 /* Adds training pairs from `nsolves` plain FP64 GMRES solves (true relative
- * residual 1e-12) of CN systems on g at `mass`: each solve takes 12 random
+ * residual 1e-12) of CN systems on g at `mass` and `stepwidth` (c1 = 1):
+ * each solve takes 12 random
  * Gaussian fields psi (6 quark, 6 antiquark), forms y = S psi and solves
  * S^+ x = y; every three fields give one pair r = y/|y|, target
  * (x - y)/|y| (the error of the zero guess minus the residual).  All pairs of
  * one network come from one lattice size. */
-int nrq_cnn_harvest(nrq_cnn *net, const nrq_gauge *g, double mass, int nsolves, uint64_t seed);
+int nrq_cnn_harvest(nrq_cnn *net, const nrq_gauge *g, double mass, double stepwidth, int nsolves, uint64_t seed);
 int nrq_cnn_dataset_size(const nrq_cnn *net, int64_t *count);
 /* `epochs` passes of Adam (learning rate lr, one pair per step, order
  * shuffled from seed); mean_loss (may be NULL) = mean relative loss of the

@@ -145,7 +145,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_cnn_set_weights": (I, [P, P]),
         "nrq_cnn_forward": (I, [P, P, P]),
         "nrq_cnn_loss_grad": (I, [P, P, P, D, P, P]),
-        "nrq_cnn_harvest": (I, [P, P, D, I, ctypes.c_uint64]),
+        "nrq_cnn_harvest": (I, [P, P, D, D, I, ctypes.c_uint64]),
         "nrq_cnn_dataset_size": (I, [P, P]),
         "nrq_cnn_train": (I, [P, I, D, ctypes.c_uint64, P]),
         "nrq_cn_set_preconditioner": (I, [P]),
@@ -538,8 +538,9 @@ class PreconditionerNet:
         _check(library().nrq_cnn_loss_grad(self._h, _ptr(inp), _ptr(target), norm, ctypes.byref(loss), _ptr(grad)))
         return loss.value, grad
 
-    def harvest(self, GaugeConf, mass: float, nsolves: int, seed: int = 1) -> None:
-        _check(library().nrq_cnn_harvest(self._h, GaugeConf.handle, mass, nsolves, seed))
+    def harvest(self, GaugeConf, mass: float, nsolves: int, seed: int = 1, StepWidth: float = 1.0) -> None:
+        """Training pairs from plain GMRES solves of CN systems at this mass."""
+        _check(library().nrq_cnn_harvest(self._h, GaugeConf.handle, mass, StepWidth, nsolves, seed))
 
     def dataset_size(self) -> int:
         n = ctypes.c_int64()

@@ -591,12 +591,12 @@ extern "C" int nrq_cnn_loss_grad(nrq_cnn *n, const float *in, const float *targe
     return e;
 }
 
-extern "C" int nrq_cnn_harvest(nrq_cnn *n, const nrq_gauge *gf, double mass, int nsolves, uint64_t seed)
+extern "C" int nrq_cnn_harvest(nrq_cnn *n, const nrq_gauge *gf, double mass, double stepwidth, int nsolves, uint64_t seed)
 {
     NEED_NET(n);
     if (!lat().initialised) return fail(NRQ_ERR_STATE, "lattice: not initialised");
     if (!gf || !gf->u[0]) return fail(NRQ_ERR_ARG, "cnn: null gauge configuration");
-    if (!(mass > 0) || nsolves < 1) return fail(NRQ_ERR_ARG, "cnn: bad mass or solve count");
+    if (!(mass > 0) || nsolves < 1 || !(stepwidth > 0)) return fail(NRQ_ERR_ARG, "cnn: bad mass, step width or solve count");
     const geom g = lat().g;
     if (g.n0 % 16) return fail(NRQ_ERR_UNSUPPORTED, "cnn: the x extent %d must be a multiple of 16", g.n0);
     if (!n->data.empty() && n->data_g.vol != g.vol) return fail(NRQ_ERR_STATE, "cnn: training set holds another lattice");
@@ -605,8 +605,8 @@ extern "C" int nrq_cnn_harvest(nrq_cnn *n, const nrq_gauge *gf, double mass, int
     const int64_t fs = 3 * g.vol;
     cplx<double> *psi = nullptr;
     NRQ_HIP(hipMalloc(&psi, sizeof(cplx<double>) * (size_t)fs * R));
-    // CN coefficients of the quark (mass) and antiquark (-mass), c1 = 1, StepWidth 1
-    const std::complex<double> aq(0.0, -(g.a[0] / 2.0) / mass), aa(0.0, (g.a[0] / 2.0) / mass);
+    // CN coefficients of the quark (mass) and antiquark (-mass), c1 = 1
+    const std::complex<double> aq(0.0, -(stepwidth * g.a[0] / 2.0) / mass), aa(0.0, (stepwidth * g.a[0] / 2.0) / mass);
     cplx<double> alpha[R], beta[R];
     cplx<double> *fields[R];
     for (int r = 0; r < R; ++r) {

@@ -2,6 +2,7 @@
 
     python -m nrqcd_amd.workloads mixed      # 128^3 FP32 storage vs FP64, CN solvers at 128^3
     python -m nrqcd_amd.workloads ensemble   # 32 x 48^3 batched ensemble with 4 masses each
+    python -m nrqcd_amd.workloads learned    # FGMRES + trained CNN vs GMRES at several step widths
 
 Each prints one JSON object.  Inputs are synthetic (the seeded box initial
 condition of gaugeconfiguration_su3.f90:906-951); all timing uses HIP events
@@ -17,9 +18,10 @@ import time
 import numpy as np
 
 from . import (GaugeConfiguration, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
-               NRQ_SOLVER_CHEBYSHEV_MIXED, NRQ_SOLVER_GMRES_FP64, NRQ_SOLVER_GMRES_MIXED, event_elapsed_ms,
-               event_record, lattice_finalize, lattice_init, leapfrog_batch, nrqcd_finalize, nrqcd_init,
-               nrqcd_set_solver, random_init, synchronize, update_batch)
+               NRQ_SOLVER_CHEBYSHEV_MIXED, NRQ_SOLVER_FGMRES_CNN, NRQ_SOLVER_GMRES_FP64, NRQ_SOLVER_GMRES_MIXED,
+               PreconditionerNet, event_elapsed_ms, event_record, lattice_finalize, lattice_init, leapfrog_batch,
+               nrqcd_finalize, nrqcd_init, nrqcd_set_preconditioner, nrqcd_set_solver, random_init, synchronize,
+               update_batch)
 
 SPACINGS = (0.05, 1.0, 1.0, 1.0)
 C1 = [1, 0, 0, 0]
@@ -156,9 +158,56 @@ def ensemble(members: int = 32, size: int = 48, steps: int = 500, cn_steps: int 
     return out
 
 
+def learned(size: int = 32, stepwidths=(1.0, 10.0, 40.0), seed: int = 7, evolve: int = 40, solves: int = 6,
+            epochs: int = 30, lr: float = 1e-3, cn_steps: int = 3) -> dict:
+    """Train the preconditioner network on pairs from plain GMRES solves at
+    Ma = 1, 2, 5 and compare FGMRES(20) + network with GMRES(20) on held-out
+    masses 3 and 8 (point sources, zero initial guess, FP64 true residual
+    1e-10), for each step width on an evolved configuration."""
+    lattice_init((size,) * 3, SPACINGS)
+    nrqcd_init(2.0, C1, 1.0, 1e-10)
+    random_init(seed)
+    g = GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    g.Update(1.0, evolve)
+    rows = []
+    for sw in stepwidths:
+        net = PreconditionerNet(3)
+        t0 = time.perf_counter()
+        for m in (1.0, 2.0, 5.0):
+            net.harvest(g, m, solves, seed=int(10 * m), StepWidth=sw)
+        loss0 = net.train(1, lr, seed=1)
+        loss1 = net.train(epochs - 1, lr, seed=2)
+        train_s = time.perf_counter() - t0
+        nrqcd_set_preconditioner(net)
+        for m in (3.0, 8.0):
+            row = {"stepwidth": sw, "mass": m, "beta_kmax": sw * SPACINGS[0] / 2 / m * 12,
+                   "pairs": net.dataset_size(), "loss_first_epoch": loss0, "loss_last_epoch": loss1,
+                   "training_s": train_s}
+            for name, kind in (("gmres", NRQ_SOLVER_GMRES_FP64), ("fgmres_cnn", NRQ_SOLVER_FGMRES_CNN)):
+                nrqcd_set_solver(kind, 20, 10000)
+                h = NRQCDField()
+                h.InitSinglePoint(100, 900)
+                h.Update(g, m, C1, sw)
+                stats = []
+                ms = _timed(lambda: stats.extend(h.Update(g, m, C1, sw) for _ in range(cn_steps)))
+                row[name] = {"iterations_per_column_solve": sum(s.iterations for s in stats) / sum(s.solves for s in stats),
+                             "ms_per_cn_step": ms / cn_steps,
+                             "worst_true_relres": max(s.max_true_relres for s in stats)}
+                del h
+            rows.append(row)
+        nrqcd_set_preconditioner(None)
+        del net
+    nrqcd_finalize()
+    del g
+    lattice_finalize()
+    return {"workload": f"{size}^3 evolved box IC (seed {seed}, {evolve} steps), train at Ma 1,2,5, test at 3,8",
+            "rows": rows}
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=("mixed", "ensemble"))
+    ap.add_argument("workload", choices=("mixed", "ensemble", "learned"))
     ap.add_argument("--size", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--members", type=int, default=32)
@@ -167,6 +216,8 @@ def main(argv=None) -> None:
     kw = {k: v for k, v in (("size", a.size), ("steps", a.steps), ("cn_steps", a.cn_steps)) if v is not None}
     if a.workload == "mixed":
         out = mixed(**kw)
+    elif a.workload == "learned":
+        out = learned(**{k: v for k, v in kw.items() if k in ("size", "cn_steps")})
     else:
         out = ensemble(members=a.members, **kw)
     print(json.dumps(out), flush=True)
