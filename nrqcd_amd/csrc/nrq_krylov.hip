@@ -552,6 +552,7 @@ struct workspace {
     int64_t vol = 0;
     int m = 0;
     int64_t nsb = 0;                   // site blocks of the stencil launches
+    int ext[3] = {0, 0, 0};            // lattice extents the buffers were sized for
     int rcap = 0;                      // slots the field buffers hold
     void *basis = nullptr;             // (m+1) x rcap x 3V complex double (or float)
     cplx<double> *y = nullptr, *x = nullptr, *t = nullptr, *res = nullptr;
@@ -588,13 +589,16 @@ struct workspace {
     int reserve(int64_t v, int mm, int r)
     {
         r = std::max(r, rcap);
-        if (v == vol && mm == m && r == rcap) return NRQ_OK;
+        const geom &lg = lat().g;
+        // the partial-sum buffer depends on the extents, not only on V
+        const bool same = lg.n0 == ext[0] && lg.n1 == ext[1] && lg.n2 == ext[2];
+        if (v == vol && mm == m && r == rcap && same) return NRQ_OK;
         release();
         const int rc = std::max(r, 1);
         const size_t f64 = (size_t)3 * (size_t)v * sizeof(cplx<double>) * rc;
         const size_t f32 = f64 / 2;
         int64_t sb = 0;
-        for (int md = 0; md <= 2; ++md) sb = std::max(sb, make_lf_map(lat().g, SBS, 8, md, 1).nsb);
+        for (int md = 0; md <= 2; ++md) sb = std::max(sb, make_lf_map(lg, SBS, 8, md, 1).nsb);
         auto dm = [](void **p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
         bool ok = dm(&basis, f64 * (size_t)(mm + 1)) && dm((void **)&y, f64) && dm((void **)&x, f64) &&
                   dm((void **)&t, f64) && dm((void **)&res, f64) && dm((void **)&y32, f32) &&
@@ -619,6 +623,9 @@ struct workspace {
         m = mm;
         rcap = rc;
         nsb = sb;
+        ext[0] = lg.n0;
+        ext[1] = lg.n1;
+        ext[2] = lg.n2;
         return NRQ_OK;
     }
 };
@@ -685,18 +692,19 @@ static lf_map cn_map(int nf, int nc)
     return make_lf_map(lat().g, SBS, shape().xcd, shape().mode, (nf + nc - 1) / nc);
 }
 
-// Runs `body` with the compile-time chunk width NCV equal to nc (1, 2 or 4).
-#define NC_DISPATCH(nc, body)                                                                                            \
+// Runs the statements after `nc` with the compile-time chunk width NCV equal
+// to nc (1, 2 or 4).
+#define NC_DISPATCH(nc, ...)                                                                                             \
     do {                                                                                                                 \
         if ((nc) == 1) {                                                                                                 \
             constexpr int NCV = 1;                                                                                       \
-            body;                                                                                                        \
+            __VA_ARGS__;                                                                                                 \
         } else if ((nc) == 2) {                                                                                          \
             constexpr int NCV = 2;                                                                                       \
-            body;                                                                                                        \
+            __VA_ARGS__;                                                                                                 \
         } else {                                                                                                         \
             constexpr int NCV = 4;                                                                                       \
-            body;                                                                                                        \
+            __VA_ARGS__;                                                                                                 \
         }                                                                                                                \
     } while (0)
 
@@ -735,14 +743,57 @@ static int nrm_to_host()
     return NRQ_OK;
 }
 
+// out_r = in_r + beta_r K in_r for the active slots
 template <typename RL, typename T>
-static int apply_t(const cplx<RL> *u, const cplx<T> *in, cplx<T> *out, int nf)
+static void launch_stencil(const cplx<RL> *u, const slots<T> &in, const slots<T> &out, int nf)
 {
     const int nc = nc_of<T>();
     const lf_map mp = cn_map(nf, nc);
-    const slots<T> ti = ws_slots(in, nf), to_ = ws_slots(out, nf);
-    NC_DISPATCH(nc, hipLaunchKernelGGL((k_stencil<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, ti, to_,
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_stencil<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, in, out,
                                        g_ws.beta, g_ws.active, nf, lat().g, mp));
+}
+
+// One Chebyshev step (k_cheb) for the active slots of R
+template <typename RL, typename T>
+static void launch_cheb(const cplx<RL> *links, const slots<T> &xk, const slots<T> &xnext, const slots<T> &b,
+                        const cplx<double> *cf, int R, int mode, int fin, const slots<double> &fi, const slots<double> &fo)
+{
+    const int nc = nc_of<T>();
+    const lf_map mp = cn_map(R, nc);
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cheb<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), links, xk, xnext,
+                                       xnext, b, g_ws.beta, cf, g_ws.active, R, lat().g, mp, mode, fin, fi, fo));
+}
+
+// FP64 true residual r = y - A x of the R slots x (r stored as TR when
+// WRITE); partial ||r||^2 to g_ws.spart.  Returns the partials per slot.
+template <typename RL, typename TR, bool WRITE>
+static int64_t launch_resid(const cplx<RL> *u, int R, const slots<double> &x, const slots<TR> &res)
+{
+    const slots<double> ty = ws_slots(g_ws.y, R);
+    const int nc = shape().nc64;
+    const lf_map mp = cn_map(R, nc);
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, TR, WRITE>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, x, ty,
+                                       res, g_ws.beta, R, lat().g, mp, g_ws.spart));
+    return mp.nsb;
+}
+
+// Right-hand sides y and guesses x of a CN step (k_cn_rhs); partial ||y||^2
+// to g_ws.spart.  Returns the partials per slot.
+template <typename RL>
+static int64_t launch_rhs(const cplx<RL> *u, int R, const slots<double> &psi, const slots<double> &y,
+                          const slots<double> &x, int guess)
+{
+    const int nc = shape().nc64;
+    const lf_map mp = cn_map(R, nc);
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, psi, y, x,
+                                       g_ws.cpar, g_ws.cpar + NSLOTS, guess, R, lat().g, mp, g_ws.spart));
+    return mp.nsb;
+}
+
+template <typename RL, typename T>
+static int apply_t(const cplx<RL> *u, const cplx<T> *in, cplx<T> *out, int nf)
+{
+    launch_stencil<RL, T>(u, ws_slots(in, nf), ws_slots(out, nf), nf);
     NRQ_HIP(hipGetLastError());
     return NRQ_OK;
 }
@@ -994,9 +1045,6 @@ static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, 
     if (zero_start && k < 2) k = 2;
     if (k > CHEB_MAX) k = CHEB_MAX;
     if (int e = cheb_table(R, beta, k, kmax)) return e;
-    const geom g = lat().g;
-    const int nc = nc_of<T>();
-    const lf_map mp = cn_map(R, nc);
     const slots<T> tb = ws_slots(b, R);
     const slots<double> none = {};
     cplx<T> *cur = x, *other = xalt;
@@ -1008,8 +1056,7 @@ static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, 
         const int fin = (fin_in && j == k - 1) ? 1 : 0;
         const slots<double> fi = fin ? *fin_in : none, fo = fin ? *fin_out : none;
         const cplx<double> *cf = g_ws.cheb + (size_t)2 * NSLOTS * j;
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cheb<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), links, txk, tnext,
-                                           tnext, tb, g_ws.beta, cf, g_ws.active, R, g, mp, mode, fin, fi, fo));
+        launch_cheb<RL, T>(links, txk, tnext, tb, cf, R, mode, fin, fi, fo);
         if (mode != 2) std::swap(cur, other);
     }
     NRQ_HIP(hipGetLastError());
@@ -1022,23 +1069,13 @@ static int cheb_run(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, 
 template <typename RL>
 static int cn_resid(const cplx<RL> *u, int R, const slots<double> &x, cplx<float> *res32)
 {
-    const int nc = shape().nc64;
-    const lf_map mp = cn_map(R, nc);
-    const geom g = lat().g;
-    const slots<double> ty = ws_slots(g_ws.y, R);
-    const cplx<double> *bt = g_ws.beta;
-    double *sp = g_ws.spart;
-    if (res32) {
-        const slots<float> tr = ws_slots(res32, R);
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, true>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u,
-                                           x, ty, tr, bt, R, g, mp, sp));
-    } else {
-        const slots<float> tr = {};
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, float, false>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u,
-                                           x, ty, tr, bt, R, g, mp, sp));
-    }
+    int64_t nsb;
+    if (res32)
+        nsb = launch_resid<RL, float, true>(u, R, x, ws_slots(res32, R));
+    else
+        nsb = launch_resid<RL, float, false>(u, R, x, slots<float>{});
     NRQ_HIP(hipGetLastError());
-    return fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm);
+    return fold_slots(g_ws.spart, nsb, R, nullptr, g_ws.nrm);
 }
 
 // Per-slot convergence check of the residual norms in g_ws.nbuf; returns
@@ -1184,15 +1221,12 @@ static int gmres_mixed(const cplx<RL> *u, int R, std::vector<double> &bn, const 
         l32 = (const cplx<float> *)u;
     }
     std::vector<double> one(R, 1.0);
-    const int nc = shape().nc64;
-    const lf_map mp = cn_map(R, nc);
-    const slots<double> tx = ws_slots(g_ws.x, R), ty = ws_slots(g_ws.y, R), tres = ws_slots(g_ws.res, R);
+    const slots<double> tx = ws_slots(g_ws.x, R), tres = ws_slots(g_ws.res, R);
     for (int outer = 0; outer < 40; ++outer) {
         // FP64 true residual r = y - A x
-        NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, double, true>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u,
-                                           tx, ty, tres, g_ws.beta, R, lat().g, mp, g_ws.spart));
+        const int64_t nsb = launch_resid<RL, double, true>(u, R, tx, tres);
         NRQ_HIP(hipGetLastError());
-        if (int e = fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm)) return e;
+        if (int e = fold_slots(g_ws.spart, nsb, R, nullptr, g_ws.nrm)) return e;
         if (int e = nrm_to_host()) return e;
         bool any = false;
         for (int r = 0; r < R; ++r) {
@@ -1238,14 +1272,10 @@ static int cn_step_t(const cplx<RL> *u, int R, const std::vector<zc> &alpha, con
     std::vector<int> act(R, 1);
     const bool guess = p.extrapolate && p.kind != KRYLOV_GMRES_FP64;
     // right-hand sides and guesses; ||y||^2 to nrm[32..)
-    const int nc = shape().nc64;
-    const lf_map mp = cn_map(R, nc);
-    const int gs = guess ? 1 : 0;
     const slots<double> tpsi = psi_slots(R), ty = ws_slots(g_ws.y, R), tx = ws_slots(g_ws.x, R);
-    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, tpsi, ty, tx,
-                                       g_ws.cpar, g_ws.cpar + NSLOTS, gs, R, lat().g, mp, g_ws.spart));
+    const int64_t nsb = launch_rhs<RL>(u, R, tpsi, ty, tx, guess ? 1 : 0);
     NRQ_HIP(hipGetLastError());
-    if (int e = fold_slots(g_ws.spart, mp.nsb, R, nullptr, g_ws.nrm + workspace::NRM_Y)) return e;
+    if (int e = fold_slots(g_ws.spart, nsb, R, nullptr, g_ws.nrm + workspace::NRM_Y)) return e;
     cplx<double> *xsol = g_ws.x;
     bool in_psi = false;
     int status = NRQ_OK;

@@ -27,14 +27,22 @@ struct lf_map {
     int nxb;     // mode 2: tiles along x
 };
 
-__device__ __forceinline__ unsigned lf_logical_block(const lf_map &m)
+// Logical block of hardware block blockIdx.x among `total` when XCD x
+// (= blockIdx.x mod xcd) is to walk the x-th contiguous range of logical
+// blocks.
+__device__ __forceinline__ unsigned xcd_block(unsigned total, int xcdi)
 {
     const unsigned b = blockIdx.x;
-    if (m.xcd <= 1) return b;
-    const unsigned total = (unsigned)m.ngroup * (unsigned)m.nsb, xcd = (unsigned)m.xcd;
+    if (xcdi <= 1) return b;
+    const unsigned xcd = (unsigned)xcdi;
     const unsigned q = total / xcd, r = total % xcd;
     const unsigned x = b % xcd, k = b / xcd;
     return x * q + (x < r ? x : r) + k;
+}
+
+__device__ __forceinline__ unsigned lf_logical_block(const lf_map &m)
+{
+    return xcd_block((unsigned)m.ngroup * (unsigned)m.nsb, m.xcd);
 }
 
 // Site of this thread, the work item (direction / chunk) of its block and the
