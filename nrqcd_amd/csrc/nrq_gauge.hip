@@ -46,9 +46,9 @@ __device__ __forceinline__ mat3<R> load_nb(const cplx<R> *__restrict__ u, int64_
 {
     if (!REC) return load_link(u, vol, dir, s);
     mat3<R> m;
-    const cplx<R> *p = u + (int64_t)dir * 9 * vol + s;
+    const cplx<R> *p = u + (int64_t)dir * 9 * vol;
 #pragma unroll
-    for (int e = 0; e < 6; ++e) m.m[e] = p[(int64_t)e * vol];
+    for (int e = 0; e < 6; ++e) m.m[e] = *at32(p + (int64_t)e * vol, (uint32_t)s);
     // row3_c = conj(row1_{c+1} row2_{c+2} - row1_{c+2} row2_{c+1})
 #pragma unroll
     for (int c = 0; c < 3; ++c) {
@@ -102,9 +102,10 @@ __device__ __forceinline__ mat3<R> evolve_link(const mat3<R> &u, const R *e, R d
 // 2164-2198: E^a_i(x) -= 2 Im Tr[T^a U_i(x) sum_{k!=i} dt (a0/a_k)^2 (S^U+S^D)]
 // with S^U = U_k(x+i) U_i(x+k)^+ U_k(x)^+ and
 //      S^D = U_k(x+i-k)^+ U_i(x-k)^+ U_k(x-k).
+// staple_sum returns sum_{k!=i} dt (a0/a_k)^2 (S^U+S^D); apply_force does the
+// rest.
 template <bool REC, typename R>
-__device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, const geom &g, int i, const lsite &p,
-                                              const mat3<R> &ui, R *e, R dt)
+__device__ __forceinline__ mat3<R> staple_sum(const cplx<R> *__restrict__ u, const geom &g, int i, const lsite &p, R dt)
 {
     mat3<R> sum;
     set_zero(sum);
@@ -124,11 +125,42 @@ __device__ __forceinline__ void evolve_efield(const cplx<R> *__restrict__ u, con
 #pragma unroll
         for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * (su.m[m] + sd.m[m]);
     }
+    return sum;
+}
+
+template <typename R> __device__ __forceinline__ void apply_force(const mat3<R> &ui, const mat3<R> &sum, R *e)
+{
     mat3<R> w = mul(ui, sum);
     R t[8];
     two_im_trace_gen(w, t);
 #pragma unroll
     for (int a = 0; a < 8; ++a) e[a] -= t[a];
+}
+
+// E_i(x) and U_i(x) of the E update, read before the staples or, with LATE,
+// after them (with 32-bit offsets), so that they hold no registers while the
+// neighbour links stream in.  Both orders give the same arithmetic; which one
+// schedules better depends on the neighbour format, so the two-row variants
+// use LATE.
+template <bool REC, bool NT, typename R>
+__device__ __forceinline__ void efield_update(const cplx<R> *__restrict__ u, R *__restrict__ ef, const geom &g, int i,
+                                              const lsite &p, R dt, R *e, mat3<R> &ui)
+{
+    const int64_t s = p.s;
+    if (REC) {
+        const mat3<R> sum = staple_sum<REC>(u, g, i, p, dt);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(at32(ef + ((int64_t)i * 8 + a) * g.vol, (uint32_t)s));
+        const cplx<R> *b = u + (int64_t)i * 9 * g.vol;
+#pragma unroll
+        for (int k = 0; k < 9; ++k) ui.m[k] = *at32(b + (int64_t)k * g.vol, (uint32_t)s);
+        apply_force(ui, sum, e);
+    } else {
+#pragma unroll
+        for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s);
+        ui = load_link(u, g.vol, i, s);
+        apply_force(ui, staple_sum<REC>(u, g, i, p, dt), e);
+    }
 }
 
 // Buffers of the configurations of one launch (blockIdx.y selects one): all
@@ -168,10 +200,8 @@ __global__ __launch_bounds__(BS) void k_efield(lf_batch<R> bt, geom g, R dt, lf_
     if (!lf_site(m, g, p, i)) return;
     const int64_t s = p.s;
     R e[8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s);
-    mat3<R> ui = load_link(u, g.vol, i, s);
-    evolve_efield<REC>(u, g, i, p, ui, e, dt);
+    mat3<R> ui;
+    efield_update<REC, NT>(u, ef, g, i, p, dt, e, ui);
 #pragma unroll
     for (int a = 0; a < 8; ++a) st_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s, e[a]);
 }
@@ -192,10 +222,8 @@ __global__ __launch_bounds__(BS) void k_fused(lf_batch<R> bt, geom g, R dt, lf_m
     if (!lf_site(m, g, p, i)) return;
     const int64_t s = p.s;
     R e[8];
-#pragma unroll
-    for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s);
-    mat3<R> ui = load_link(uin, g.vol, i, s);
-    evolve_efield<REC>(uin, g, i, p, ui, e, dt);
+    mat3<R> ui;
+    efield_update<REC, NT>(uin, ef, g, i, p, dt, e, ui);
 #pragma unroll
     for (int a = 0; a < 8; ++a) st_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s, e[a]);
     store_link_out<NT>(uout, g.vol, i, s, evolve_link(ui, e, dt));

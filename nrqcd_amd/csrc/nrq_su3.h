@@ -63,6 +63,19 @@ template <typename R> __device__ __forceinline__ cplx<R> load_nt(const cplx<R> *
     return {w.x, w.y};
 }
 
+// Element i of an array whose base p is the same for the whole wavefront:
+// the scalar base plus a 32-bit byte offset (i < 3 V <= 2^27.6 and 16-byte
+// elements keep the offset below 2^32), so no 64-bit address arithmetic
+// runs in the vector registers.
+template <typename V> __device__ __forceinline__ const V *at32(const V *p, uint32_t i)
+{
+    return (const V *)((const char *)p + i * (uint32_t)sizeof(V));
+}
+template <typename V> __device__ __forceinline__ V *at32(V *p, uint32_t i)
+{
+    return (V *)((char *)p + i * (uint32_t)sizeof(V));
+}
+
 template <typename R> struct mat3 {
     cplx<R> m[9];
 };
