@@ -33,13 +33,23 @@ HBM_PEAK_GBS = 8000.0
 TRAFFIC_FILE = ROOT / "profiles" / "leapfrog_fused_64cubed_fp64_traffic.txt"
 
 
-def _measured_traffic(size: int):
-    """HBM bytes per fused launch from the committed rocprof summary, or None."""
+TILINGS = ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles")
+
+
+def _shape_key(shape: dict) -> str:
+    return (f"bs={shape['threads_per_block']} xcd={shape['xcd_remap']} tiling={TILINGS.index(shape['tiling'])} "
+            f"streaming={int(shape['streaming'])} two_row={int(shape['two_row_neighbours'])}")
+
+
+def _measured_traffic(size: int, shape: dict):
+    """HBM bytes per fused launch of this launch shape from the committed
+    rocprof summary, or None if that shape was not measured."""
     if size != 64 or not TRAFFIC_FILE.exists():
         return None
+    key = f"traffic_bytes_per_launch[{_shape_key(shape)}]="
     for line in TRAFFIC_FILE.read_text().splitlines():
-        if line.startswith("traffic_bytes_per_launch="):
-            return float(line.split("=", 1)[1])
+        if line.startswith(key):
+            return float(line[len(key):])
     return None
 
 
@@ -180,7 +190,7 @@ def main() -> None:
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": _measured_traffic(L),
+            "traffic": _measured_traffic(L, shape),
             "kernel": "k_fused<double> (E update + next link update per (site, dir))",
             "launch_shape": shape,
             "bytes_per_launch": bytes_per_site * V,
