@@ -87,6 +87,8 @@ int nrq_lattice_negative_index(int64_t idx, int64_t *out);
 int nrq_lattice_momentum(int64_t idx, double p[6]);
 /* lattice::GetNorm2Momentum_G, lattice.f90:1048-1058 */
 int nrq_lattice_norm2_momentum(int64_t idx, double *out);
+/* lattice::GetMaxNorm2Momentum, lattice.f90:1101-1105 (largest |p| above) */
+int nrq_lattice_max_norm2_momentum(double *out);
 
 /* ---- random (random.f90 / ranlxd_generator.f90) --------------------- */
 
@@ -169,11 +171,27 @@ int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval);
  * new links) and two_row (1: neighbour links read as two rows, the third
  * rebuilt as the conjugate cross product, exact for SU(3)).  The first
  * leapfrog call per (precision, size) times every shape on scratch copies;
- * NRQ_AUTOTUNE=0 in the environment disables it and fixes the first shape.
+ * NRQ_AUTOTUNE=0 in the environment disables it and fixes the default shape
+ * (256 threads, consecutive sites, plain loads), NRQ_LF_SHAPE=n pins shape n.
  * All shapes compute the same update, equal up to rounding.  No reference
  * counterpart. */
 int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling, int *streaming,
                            int *two_row);
+/* GaugeConfiguration%CoulombGaugefixing(Tolerance_, alpha_, MaxIterations_),
+ * gaugeconfiguration_su3.f90:2269-2497 (the reference's defaults are 1e-8,
+ * 0.08 and 100): Davies' Fourier-accelerated gauge fixing of the links.  E is
+ * not transformed, as in the reference.  Stops when max_x ||Delta(x)||_F / 9
+ * <= tol or after max_iterations gauge transformations; iterations (may be
+ * null) receives the transformations applied, final_deviation (may be null)
+ * the last max_x ||Delta(x)||_F / 9. */
+int nrq_gauge_coulomb_gaugefix(nrq_gauge *h, double tol, double alpha, int64_t max_iterations, int64_t *iterations,
+                               double *final_deviation);
+/* GaugeConfiguration%GetTransverseAACorrelator / GetTransverseEECorrelator,
+ * gaugeconfiguration_su3.f90:1825-1972: V doubles each, indexed by momentum
+ * index (lattice index - 1); either pointer may be null.  The driver's gluon
+ * distribution is sqrt(aa ee) per momentum next to nrq_lattice_norm2_momentum
+ * (main.f90:3093-3189). */
+int nrq_gauge_transverse_correlators(const nrq_gauge *h, double *aa, double *ee);
 
 /* ---- nrqcd ------------------------------------------------------------ */
 

@@ -105,6 +105,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_lattice_negative_index": (I, [I64, P]),
         "nrq_lattice_momentum": (I, [I64, P]),
         "nrq_lattice_norm2_momentum": (I, [I64, P]),
+        "nrq_lattice_max_norm2_momentum": (I, [P]),
         "nrq_random_init": (I, [I64]),
         "nrq_random_uniform": (I, [P, I64]),
         "nrq_gauge_create": (I, [I, P]),
@@ -127,6 +128,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_gauge_reunitarise": (I, [P]),
         "nrq_gauge_set_reunitarise_interval": (I, [P, I64]),
         "nrq_gauge_launch_shape": (I, [P, P, P, P, P, P]),
+        "nrq_gauge_coulomb_gaugefix": (I, [P, D, D, I64, P, P]),
+        "nrq_gauge_transverse_correlators": (I, [P, P, P]),
         "nrq_cn_init": (I, [D, P, D, D]),
         "nrq_cn_finalize": (I, []),
         "nrq_cn_set_solver": (I, [I, I, I64]),
@@ -250,6 +253,19 @@ def GetNorm2Momentum_G(index: int) -> float:
     return out.value
 
 
+def GetMaxNorm2Momentum() -> float:
+    out = ctypes.c_double()
+    _check(library().nrq_lattice_max_norm2_momentum(ctypes.byref(out)))
+    return out.value
+
+
+def momentum_norms() -> np.ndarray:
+    """|p| of every momentum index (lattice order), as written next to the
+    gluon distribution (main.f90:3093-3189)."""
+    n = lattice_size()
+    return np.array([GetNorm2Momentum_G(k) for k in range(1, n + 1)])
+
+
 def random_init(seed: int) -> None:
     """``random::InitModule(CustomSeed)`` -> ranlxd level 2."""
     _check(library().nrq_random_init(seed))
@@ -370,6 +386,36 @@ class GaugeConfiguration:
         out = np.zeros((3, 3), np.complex128, order="F")
         _check(library().nrq_gauge_plaquette(self._h, i, j, LatticeIndex, _ptr(out)))
         return out
+
+    # -- gluon occupation --------------------------------------------------
+    def CoulombGaugefixing(self, Tolerance_: float = 1e-8, alpha_: float = 0.08, MaxIterations_: int = 100):
+        """Davies' Fourier-accelerated Coulomb gauge fixing of the links (E is
+        not transformed, as in the reference).  Returns (gauge transformations
+        applied, final max_x ||Delta(x)||_F / 9)."""
+        it = ctypes.c_int64()
+        dev = ctypes.c_double()
+        _check(library().nrq_gauge_coulomb_gaugefix(self._h, Tolerance_, alpha_, MaxIterations_, ctypes.byref(it),
+                                                     ctypes.byref(dev)))
+        return it.value, dev.value
+
+    def transverse_correlators(self):
+        """(AA, EE) per momentum index (lattice order)."""
+        n = lattice_size()
+        aa, ee = np.zeros(n), np.zeros(n)
+        _check(library().nrq_gauge_transverse_correlators(self._h, _ptr(aa), _ptr(ee)))
+        return aa, ee
+
+    def GetTransverseAACorrelator(self) -> np.ndarray:
+        n = lattice_size()
+        aa = np.zeros(n)
+        _check(library().nrq_gauge_transverse_correlators(self._h, _ptr(aa), None))
+        return aa
+
+    def GetTransverseEECorrelator(self) -> np.ndarray:
+        n = lattice_size()
+        ee = np.zeros(n)
+        _check(library().nrq_gauge_transverse_correlators(self._h, None, _ptr(ee)))
+        return ee
 
     @property
     def handle(self) -> ctypes.c_void_p:

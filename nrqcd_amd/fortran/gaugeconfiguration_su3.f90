@@ -39,6 +39,9 @@ module gaugeconfiguration_su3
      procedure, public :: Download
      procedure, public :: Reunitarise
      procedure, public :: SetReunitarisationInterval
+     procedure, public :: CoulombGaugefixing
+     procedure, public :: GetTransverseAACorrelator
+     procedure, public :: GetTransverseEECorrelator
      procedure, private :: Update_Leapfrog
      generic, public :: Update => Update_Leapfrog
      procedure, private :: Assign
@@ -276,4 +279,61 @@ contains
     call nrq_check(nrq_gauge_set_reunitarise_interval(GaugeConf%handle, Interval), &
          'GaugeConfiguration%SetReunitarisationInterval')
   end subroutine SetReunitarisationInterval
+
+  !> GaugeConfiguration%CoulombGaugefixing(Tolerance_, alpha_, MaxIterations_),
+  !! gaugeconfiguration_su3.f90:2269-2311, with the reference's defaults, and
+  !! CoulombGaugefixing_Links :2318-2497 (Davies' Fourier-accelerated steepest
+  !! descent on the links; E is not transformed).  Prints the final loop index
+  !! as the reference does (transformations applied + 1).
+  subroutine CoulombGaugefixing(GaugeConf, Tolerance_, alpha_, MaxIterations_)
+    use, intrinsic :: iso_fortran_env, only: output_unit
+    class(GaugeConfiguration), intent(inout) :: GaugeConf
+    real(fp), intent(in), optional :: Tolerance_
+    real(fp), intent(in), optional :: alpha_
+    integer(int64), intent(in), optional :: MaxIterations_
+    real(fp) :: Tolerance, alpha, deviation
+    integer(int64) :: MaxIterations, done
+    if (present(Tolerance_)) then
+       Tolerance = Tolerance_
+    else
+       Tolerance = 1.E-8
+    end if
+    if (present(alpha_)) then
+       alpha = alpha_
+    else
+       alpha = 0.08_fp
+    end if
+    if (present(MaxIterations_)) then
+       MaxIterations = MaxIterations_
+    else
+       MaxIterations = 100
+    end if
+    call nrq_check(nrq_gauge_coulomb_gaugefix(GaugeConf%handle, Tolerance, alpha, MaxIterations, done, deviation), &
+         'GaugeConfiguration%CoulombGaugefixing')
+    write(output_unit, *) deviation
+    write(output_unit, *) done + 1, 'iterations in Coulomb gauge fixing performed'
+    call flush(output_unit)
+  end subroutine CoulombGaugefixing
+
+  !> GaugeConfiguration%GetTransverseAACorrelator(Correlator),
+  !! gaugeconfiguration_su3.f90:1825-1856: one value per momentum index
+  subroutine GetTransverseAACorrelator(GaugeConf, Correlator)
+    use lattice, only: GetLocalLatticeSize
+    class(GaugeConfiguration), intent(in) :: GaugeConf
+    real(fp), allocatable, target, intent(out) :: Correlator(:)
+    allocate(Correlator(GetLocalLatticeSize()))
+    call nrq_check(nrq_gauge_transverse_correlators(GaugeConf%handle, c_loc(Correlator), c_null_ptr), &
+         'GaugeConfiguration%GetTransverseAACorrelator')
+  end subroutine GetTransverseAACorrelator
+
+  !> GaugeConfiguration%GetTransverseEECorrelator(Correlator),
+  !! gaugeconfiguration_su3.f90:1863-1894: one value per momentum index
+  subroutine GetTransverseEECorrelator(GaugeConf, Correlator)
+    use lattice, only: GetLocalLatticeSize
+    class(GaugeConfiguration), intent(in) :: GaugeConf
+    real(fp), allocatable, target, intent(out) :: Correlator(:)
+    allocate(Correlator(GetLocalLatticeSize()))
+    call nrq_check(nrq_gauge_transverse_correlators(GaugeConf%handle, c_null_ptr, c_loc(Correlator)), &
+         'GaugeConfiguration%GetTransverseEECorrelator')
+  end subroutine GetTransverseEECorrelator
 end module gaugeconfiguration_su3

@@ -10,7 +10,7 @@ module lattice
   private
   public :: InitModule, FinalizeModule, IsModuleInitialised, GetLatticeIndex, GetLatticePosition, &
        GetNeib_G, GetNeib_M, GetLatticeIndex_M, GetMemoryIndex, GetNegativeLatticeIndex, &
-       GetMomentum_G, GetMomentum_M, GetNorm2Momentum_G, GetNorm2Momentum_M, GetLatticeSize, &
+       GetMomentum_G, GetMomentum_M, GetNorm2Momentum_G, GetNorm2Momentum_M, GetMaxNorm2Momentum, GetLatticeSize, &
        GetMemorySize, GetLocalLatticeSize, GetLatticeExtension, GetLatticeSpacing, GetVolume, &
        GetProc_G, GetProc_M
 
@@ -95,6 +95,11 @@ contains
     integer(int64), intent(in) :: LatticeIndex
     GetNorm2Momentum_G = norm2(abs(GetMomentum_G(LatticeIndex)))
   end function GetNorm2Momentum_G
+
+  !> lattice::GetMaxNorm2Momentum, lattice.f90:1101-1105
+  real(fp) function GetMaxNorm2Momentum()
+    call nrq_check(nrq_lattice_max_norm2_momentum(GetMaxNorm2Momentum), 'lattice::GetMaxNorm2Momentum')
+  end function GetMaxNorm2Momentum
 
   real(fp) function GetNorm2Momentum_M(MemoryIndex)
     integer(int64), intent(in) :: MemoryIndex

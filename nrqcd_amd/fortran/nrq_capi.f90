@@ -151,6 +151,24 @@ module nrq_capi
        type(c_ptr), value :: h
        integer(c_int64_t), value :: interval
      end function nrq_gauge_set_reunitarise_interval
+     integer(c_int) function nrq_gauge_coulomb_gaugefix(h, tol, alpha, maxit, iterations, dev) &
+          bind(C, name='nrq_gauge_coulomb_gaugefix')
+       import :: c_int, c_ptr, c_int64_t, c_double
+       type(c_ptr), value :: h
+       real(c_double), value :: tol, alpha
+       integer(c_int64_t), value :: maxit
+       integer(c_int64_t), intent(out) :: iterations
+       real(c_double), intent(out) :: dev
+     end function nrq_gauge_coulomb_gaugefix
+     integer(c_int) function nrq_gauge_transverse_correlators(h, aa, ee) &
+          bind(C, name='nrq_gauge_transverse_correlators')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, aa, ee
+     end function nrq_gauge_transverse_correlators
+     integer(c_int) function nrq_lattice_max_norm2_momentum(out) bind(C, name='nrq_lattice_max_norm2_momentum')
+       import :: c_int, c_double
+       real(c_double), intent(out) :: out
+     end function nrq_lattice_max_norm2_momentum
 
      integer(c_int) function nrq_cn_init(mass, c, sw, tol) bind(C, name='nrq_cn_init')
        import :: c_int, c_double, c_double_complex

@@ -265,7 +265,7 @@ class RefModules:
             ("ref_ranlxd_init", None, [I64, I64]), ("ref_ranlxd", None, [P, I64]), ("ref_ranlxd_get", None, [P]),
             ("ref_su3_generators", None, [P]), ("ref_su2_generators", None, [P]), ("ref_group_exp", None, [P, P]),
             ("ref_algebra_matrix", None, [P, P]), ("ref_algebra_coordinate", D, [I, P]),
-            ("ref_trace_with_generator", None, [I, P, P]), ("ref_expah", None, [P, P]),
+            ("ref_trace_with_generator", None, [I, P, P]), ("ref_expah", None, [P, P]), ("ref_group_log", None, [P, P]),
             ("ref_kron", None, [P, I, I, P, I, I, P]),
         ]:
             f = getattr(L, name)
@@ -325,6 +325,12 @@ class RefModules:
         aa = np.asfortranarray(A, np.complex128)
         out = np.zeros((3, 3), np.complex128, order="F")
         self.lib.ref_expah(_ptr(aa), _ptr(out))
+        return out
+
+    def group_log(self, U):
+        uu = np.asfortranarray(U, np.complex128)
+        out = np.zeros(8)
+        self.lib.ref_group_log(_ptr(uu), _ptr(out))
         return out
 
     def kron(self, A, B):

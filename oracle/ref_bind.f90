@@ -115,6 +115,16 @@ contains
     res = ExpAH(aa)
   end subroutine ref_expah
 
+  ! su3.f90:318-334 (ArgLogU, matrixoperations.f90:474-495)
+  subroutine ref_group_log(u, res) bind(C, name='ref_group_log')
+    use su3, only: GetGroupLog
+    complex(c_double_complex), intent(in) :: u(3, 3)
+    real(c_double), intent(out) :: res(8)
+    complex(fp) :: uu(3, 3)
+    uu = u
+    res = GetGroupLog(uu)
+  end subroutine ref_group_log
+
   ! matrixoperations.f90:81-104
   subroutine ref_kron(a, ra, ca, b, rb, cb, ab) bind(C, name='ref_kron')
     use matrixoperations, only: GetKronProd
