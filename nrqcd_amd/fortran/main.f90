@@ -1,5 +1,9 @@
 !> nrqcd_run -- Fortran driver over the GPU library, with the reference's
 !! positional command line (main.f90:3301-3340) for the hot-path modes:
+!!   1  MeasureGluondistribution           (main.f90:3020-3190)
+!!        nrqcd_run 1 Nx Ny Nz a0 ax ay az CoMTime Seed Qs Amp g
+!!                  GaugefixingMaxIterations GaugefixingTolerance GaugefixingCoefficient FileGluondist
+!!        -> FileGluondist: |p| and sqrt(AA EE) per momentum '(2(SP,E13.6,1X))'
 !!   2  MeasureEnergyAndGaussLawDeviation   (main.f90:2572-2718)
 !!        nrqcd_run 2 Nx Ny Nz a0 ax ay az CoMTime TimeRange Seed Qs Amp g
 !!        -> energy_gauss.txt: time, energy, Gauss deviation '(3(SP,E13.6,1X))'
@@ -15,7 +19,7 @@ program nrqcd_run
   use, intrinsic :: ieee_arithmetic
   use precision, only: fp
   use mpiinterface, only: InitModule_MPI => InitModule, ThisProc, MPIstop
-  use lattice, only: InitModule_Lattice => InitModule, GetLatticeSpacing
+  use lattice, only: InitModule_Lattice => InitModule, GetLatticeSpacing, GetNorm2Momentum_G
   use random, only: InitModule_Random => InitModule
   use gaugeconfiguration_su3, only: GaugeConfiguration
   use nrqcd, only: NRQCDField, InitModule_NRQCD => InitModule, FinalizeModule_NRQCD => FinalizeModule, &
@@ -30,6 +34,8 @@ program nrqcd_run
   read(arg, *) mode
   call InitModule_MPI()
   select case (mode)
+  case (1)
+     call MeasureGluondistribution()
   case (2)
      call MeasureEnergyAndGaussLawDeviation()
   case (3)
@@ -43,7 +49,7 @@ program nrqcd_run
 contains
 
   subroutine usage()
-    write(output_unit, '(A)') 'usage: nrqcd_run 2|3|32 <positional parameters of the reference mode>'
+    write(output_unit, '(A)') 'usage: nrqcd_run 1|2|3|32 <positional parameters of the reference mode>'
     stop 2
   end subroutine usage
 
@@ -58,6 +64,56 @@ contains
     call get_command_argument(k, arg)
     read(arg, *) rarg
   end function rarg
+
+  !> Mode 1: gluon distribution of the box initial condition evolved to
+  !! CoMTime: Coulomb gauge fixing of a copy, transverse AA and EE
+  !! correlators, and per momentum |p| and sqrt(AA EE)
+  subroutine MeasureGluondistribution()
+    integer(int64) :: ext(3), seed, TimeSteps, GFMaxIterations, is
+    real(fp) :: spacings(0:3), CoMTime, qs, amp, g, GFTolerance, GFCoefficient
+    character(len=256) :: fname
+    type(GaugeConfiguration) :: GaugeConf, GaugeConf_gaugefixed
+    real(fp), allocatable :: aa(:), ee(:)
+    integer :: k, fid
+
+    if (command_argument_count() < 17) call usage()
+    do k = 1, 3
+       ext(k) = iarg(1 + k)
+    end do
+    do k = 0, 3
+       spacings(k) = rarg(5 + k)
+    end do
+    CoMTime = rarg(9)
+    seed = iarg(10)
+    qs = rarg(11)
+    amp = rarg(12)
+    g = rarg(13)
+    GFMaxIterations = iarg(14)
+    GFTolerance = rarg(15)
+    GFCoefficient = rarg(16)
+    call get_command_argument(17, fname)
+    TimeSteps = ceiling(abs(CoMTime) / spacings(0))
+
+    call InitModule_Lattice(ext, spacings)
+    call InitModule_Random(seed + ThisProc())
+    call GaugeConf%TransversePolarisedOccupiedInit_Box(qs, amp, g)
+    if (TimeSteps > 0) call GaugeConf%Update(sign(1._fp, CoMTime), TimeSteps)
+
+    GaugeConf_gaugefixed = GaugeConf
+    call GaugeConf_gaugefixed%CoulombGaugefixing(Tolerance_=GFTolerance, alpha_=GFCoefficient, &
+         MaxIterations_=GFMaxIterations)
+    call GaugeConf_gaugefixed%GetTransverseAACorrelator(aa)
+    call GaugeConf_gaugefixed%GetTransverseEECorrelator(ee)
+
+    open(newunit=fid, file=trim(fname), status='replace', form='formatted', action='write')
+    do is = 1, size(aa)
+       write(fid, '(2(SP,E13.6,1X))') GetNorm2Momentum_G(is), sqrt(aa(is) * ee(is))
+    end do
+    close(fid)
+    call GaugeConf_gaugefixed%Deallocate
+    call GaugeConf%Deallocate
+    write(output_unit, *) 'Simulation completed'
+  end subroutine MeasureGluondistribution
 
   !> Mode 2: energy and Gauss-law deviation along the evolution
   subroutine MeasureEnergyAndGaussLawDeviation()

@@ -76,3 +76,27 @@ def test_mode3_correlator_grid(oracle, tmp_path):
         assert corr[srow, 2] == pytest.approx(c.imag, rel=1e-8, abs=1e-9)
         assert norm[srow, 1] == pytest.approx(nq, rel=1e-8)
         assert norm[srow, 2] == pytest.approx(na, rel=1e-8)
+
+
+def test_mode1_gluon_distribution(oracle, tmp_path):
+    """main.f90:3020-3190: box IC evolved to CoMTime, Coulomb gauge fixing of a
+    copy, |p| and sqrt(AA EE) per momentum, against the oracle's replay."""
+    from oracle import gaugefix_oracle as gf
+
+    ext, sp, seed = (8, 6, 5), (0.05, 1.0, 1.0, 1.0), 4
+    comtime, maxit, tol, alpha = 0.25, 500, 1e-10, 0.08
+    # mode Nx Ny Nz a0 ax ay az CoMTime Seed Qs Amp g GFMaxIt GFTol GFCoeff File
+    _run([1, *ext, *sp, comtime, seed, 0.5, 1.0, 1.0, maxit, tol, alpha, "gluon.txt"], tmp_path)
+    rows = np.loadtxt(tmp_path / "gluon.txt")
+    oracle.lattice_init(ext, sp)
+    lat = gf.Lattice(ext, sp, oracle.generators())
+    assert rows.shape == (lat.vol, 2)
+    U, E = oracle.box_init(seed, 0.5, 1.0, 1.0)
+    oracle.leapfrog(U, E, 1.0, int(np.ceil(comtime / sp[0])))
+    V, _, dev = gf.coulomb_gauge_fix(lat, U, tol, alpha, maxit)
+    assert dev <= tol
+    aa, ee = gf.transverse_correlators(lat, V, E)
+    f = np.sqrt(aa * ee)
+    # E13.6 keeps 6 significant digits: relative rounding up to 5e-6
+    assert np.allclose(rows[:, 0], lat.pnorm, rtol=5e-6, atol=1e-12)
+    assert np.allclose(rows[:, 1], f, rtol=5e-6, atol=1e-6 * f.max())
