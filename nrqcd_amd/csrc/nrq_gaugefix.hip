@@ -400,11 +400,8 @@ static double max_momentum(const geom &g)
 struct gf_buffers {
     cplx<double> *fld = nullptr;
     double *partial = nullptr, *dmax = nullptr, *hmax = nullptr;
-    hipfftHandle plan = 0;
-    bool have_plan = false;
     ~gf_buffers()
     {
-        if (have_plan) hipfftDestroy(plan);
         if (fld) (void)hipFree(fld);
         if (partial) (void)hipFree(partial);
         if (dmax) (void)hipFree(dmax);
@@ -412,14 +409,33 @@ struct gf_buffers {
     }
 };
 
-static int gf_plan(gf_buffers &b, const geom &g, int batch)
+// Batched in-place Z2Z plans, kept per (extents, batch) for the life of the
+// process: plan creation costs milliseconds, a measurement repeated along a
+// run should not pay it every time.
+struct gf_plan_entry {
+    int n0, n1, n2, batch;
+    hipfftHandle plan;
+};
+static std::vector<gf_plan_entry> g_gf_plans;
+
+static int gf_plan(const geom &g, int batch, hipfftHandle *out)
 {
+    for (const gf_plan_entry &e : g_gf_plans)
+        if (e.n0 == g.n0 && e.n1 == g.n1 && e.n2 == g.n2 && e.batch == batch) {
+            *out = e.plan;
+            return hipfftSetStream(e.plan, stream()) == HIPFFT_SUCCESS ? NRQ_OK : fail(NRQ_ERR_HIP, "hipFFT set stream failed");
+        }
     int dims[3] = {g.n2, g.n1, g.n0}; // row-major, x fastest
-    hipfftResult fr = hipfftPlanMany(&b.plan, 3, dims, nullptr, 1, (int)g.vol, nullptr, 1, (int)g.vol, HIPFFT_Z2Z, batch);
+    hipfftHandle plan = 0;
+    hipfftResult fr = hipfftPlanMany(&plan, 3, dims, nullptr, 1, (int)g.vol, nullptr, 1, (int)g.vol, HIPFFT_Z2Z, batch);
     if (fr != HIPFFT_SUCCESS) return gf_fft_fail(fr, "plan");
-    b.have_plan = true;
-    fr = hipfftSetStream(b.plan, stream());
-    if (fr != HIPFFT_SUCCESS) return gf_fft_fail(fr, "set stream");
+    fr = hipfftSetStream(plan, stream());
+    if (fr != HIPFFT_SUCCESS) {
+        hipfftDestroy(plan);
+        return gf_fft_fail(fr, "set stream");
+    }
+    g_gf_plans.push_back({g.n0, g.n1, g.n2, batch, plan});
+    *out = plan;
     return NRQ_OK;
 }
 
@@ -437,7 +453,8 @@ static int gaugefix_t(nrq_gauge *h, double tol, double alpha, int64_t maxit, int
         (void)hipGetLastError();
         return fail(NRQ_ERR_ALLOC, "gauge fixing: device allocation failed (V=%lld)", (long long)v);
     }
-    if (int e = gf_plan(b, g, 9)) return e;
+    hipfftHandle plan = 0;
+    if (int e = gf_plan(g, 9, &plan)) return e;
     cplx<R> *u = (cplx<R> *)gauge_links_mut(h);
     const double fwd = g.a[1] * g.a[2] * g.a[3];
     const double bwd = 1.0 / ((double)g.n0 * g.a[1] * (double)g.n1 * g.a[2] * (double)g.n2 * g.a[3]);
@@ -453,10 +470,10 @@ static int gaugefix_t(nrq_gauge *h, double tol, double alpha, int64_t maxit, int
         dev = *b.hmax;
         if (!std::isfinite(dev)) return fail(NRQ_ERR_NOCONV, "gauge fixing: non-finite divergence");
         if (dev <= tol || done >= maxit) break;
-        hipfftResult fr = hipfftExecZ2Z(b.plan, (hipfftDoubleComplex *)b.fld, (hipfftDoubleComplex *)b.fld, HIPFFT_FORWARD);
+        hipfftResult fr = hipfftExecZ2Z(plan, (hipfftDoubleComplex *)b.fld, (hipfftDoubleComplex *)b.fld, HIPFFT_FORWARD);
         if (fr != HIPFFT_SUCCESS) return gf_fft_fail(fr, "exec (forward)");
         hipLaunchKernelGGL(k_gf_accelerate, dim3(nblk), dim3(GF_THREADS), 0, stream(), b.fld, g, pmax, fwd * bwd * alpha);
-        fr = hipfftExecZ2Z(b.plan, (hipfftDoubleComplex *)b.fld, (hipfftDoubleComplex *)b.fld, HIPFFT_BACKWARD);
+        fr = hipfftExecZ2Z(plan, (hipfftDoubleComplex *)b.fld, (hipfftDoubleComplex *)b.fld, HIPFFT_BACKWARD);
         if (fr != HIPFFT_SUCCESS) return gf_fft_fail(fr, "exec (backward)");
         hipLaunchKernelGGL(k_gf_exp, dim3(nblk), dim3(GF_THREADS), 0, stream(), b.fld, g);
         hipLaunchKernelGGL(k_gf_apply<R>, dim3(nblk), dim3(GF_THREADS), 0, stream(), u, b.fld, g);
@@ -481,7 +498,8 @@ template <typename R> static int correlators_t(const nrq_gauge *h, double *aa, d
         return fail(NRQ_ERR_ALLOC, "correlators: device allocation failed (V=%lld)", (long long)v);
     }
     dout = b.partial;
-    if (int e = gf_plan(b, g, 24)) return e;
+    hipfftHandle plan = 0;
+    if (int e = gf_plan(g, 24, &plan)) return e;
     const double fwd = g.a[1] * g.a[2] * g.a[3];
     const double vphys = (double)g.n0 * g.a[1] * (double)g.n1 * g.a[2] * (double)g.n2 * g.a[3];
     const double scale = fwd * fwd / vphys / 8.0;
@@ -492,7 +510,7 @@ template <typename R> static int correlators_t(const nrq_gauge *h, double *aa, d
         if (!host) continue;
         hipLaunchKernelGGL(k_corr_fields<R>, dim3(nblk), dim3(GF_THREADS), 0, stream(), u, ef, g, which, b.fld);
         NRQ_HIP(hipGetLastError());
-        hipfftResult fr = hipfftExecZ2Z(b.plan, (hipfftDoubleComplex *)b.fld, (hipfftDoubleComplex *)b.fld, HIPFFT_FORWARD);
+        hipfftResult fr = hipfftExecZ2Z(plan, (hipfftDoubleComplex *)b.fld, (hipfftDoubleComplex *)b.fld, HIPFFT_FORWARD);
         if (fr != HIPFFT_SUCCESS) return gf_fft_fail(fr, "exec (forward)");
         hipLaunchKernelGGL(k_corr_project, dim3(nblk), dim3(GF_THREADS), 0, stream(), b.fld, g, scale, dout);
         NRQ_HIP(hipGetLastError());

@@ -3,6 +3,7 @@
     python -m nrqcd_amd.workloads mixed      # 128^3 FP32 storage vs FP64, CN solvers at 128^3
     python -m nrqcd_amd.workloads ensemble   # 32 x 48^3 batched ensemble with 4 masses each
     python -m nrqcd_amd.workloads learned    # FGMRES + trained CNN vs GMRES at several step widths
+    python -m nrqcd_amd.workloads gluon      # Coulomb gauge fixing + transverse correlators, f(p) at 64^3
 
 Each prints one JSON object.  Inputs are synthetic (the seeded box initial
 condition of gaugeconfiguration_su3.f90:906-951); all timing uses HIP events
@@ -17,7 +18,7 @@ import time
 
 import numpy as np
 
-from . import (GaugeConfiguration, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
+from . import (GaugeConfiguration, GetNorm2Momentum_G, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
                NRQ_SOLVER_CHEBYSHEV_MIXED, NRQ_SOLVER_FGMRES_CNN, NRQ_SOLVER_GMRES_FP64, NRQ_SOLVER_GMRES_MIXED,
                PreconditionerNet, event_elapsed_ms, event_record, lattice_finalize, lattice_init, leapfrog_batch,
                nrqcd_finalize, nrqcd_init, nrqcd_set_preconditioner, nrqcd_set_solver, random_init, synchronize,
@@ -207,9 +208,48 @@ def learned(size: int = 32, stepwidths=(1.0, 10.0, 40.0), seed: int = 7, evolve:
             "rows": rows}
 
 
+def gluon(size: int = 64, steps: int = 20, seed: int = 2, iterations: int = 200, tol: float = 1e-8,
+          alpha: float = 0.08, precision: int = 64) -> dict:
+    """Gluon distribution of the evolved box IC as the reference's mode 1
+    computes it (main.f90:3020-3190): Coulomb gauge fixing of a copy (at most
+    `iterations` transformations), transverse AA and EE correlators,
+    f(p) = sqrt(AA EE); f is also averaged in |p| bins of width 0.25."""
+    lattice_init((size,) * 3, SPACINGS)
+    random_init(seed)
+    g = GaugeConfiguration(NRQ_PRECISION_FP32 if precision == 32 else NRQ_PRECISION_FP64)
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    g.Update(1.0, steps)
+    warm = g.copy()
+    warm.CoulombGaugefixing(tol, alpha, 2)  # hipFFT plan and kernels outside the timed region
+    warm.transverse_correlators()
+    del warm
+    fixed = g.copy()
+    done = [0, 0.0]
+    ms_fix = _timed(lambda: done.__setitem__(slice(0, 2), fixed.CoulombGaugefixing(tol, alpha, iterations)))
+    corr = {}
+    ms_corr = _timed(lambda: corr.setdefault("aaee", fixed.transverse_correlators()))
+    aa, ee = corr["aaee"]
+    f = np.sqrt(np.maximum(aa * ee, 0.0))
+    p = np.array([GetNorm2Momentum_G(k) for k in range(1, size ** 3 + 1)])
+    edges = np.arange(0.0, p.max() + 0.25, 0.25)
+    idx = np.digitize(p, edges)
+    bins = [{"p": float(p[idx == k].mean()), "f": float(f[idx == k].mean()), "modes": int((idx == k).sum())}
+            for k in np.unique(idx) if k > 0 and (idx == k).sum() > 0 and p[idx == k].mean() > 0]
+    out = {
+        "workload": f"{size}^3 box IC Qs=0.5 seed {seed}, {steps} leapfrog steps, FP{precision} links, "
+                    f"gauge fixing tol {tol} alpha {alpha} at most {iterations} transformations",
+        "transformations": done[0], "final_max_deviation": done[1],
+        "gaugefix_ms": ms_fix, "gaugefix_us_per_iteration": 1e3 * ms_fix / max(done[0], 1),
+        "correlators_ms": ms_corr, "f_binned": bins,
+    }
+    del fixed, g
+    lattice_finalize()
+    return out
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=("mixed", "ensemble", "learned"))
+    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon"))
     ap.add_argument("--size", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--members", type=int, default=32)
@@ -220,6 +260,8 @@ def main(argv=None) -> None:
         out = mixed(**kw)
     elif a.workload == "learned":
         out = learned(**{k: v for k, v in kw.items() if k in ("size", "cn_steps")})
+    elif a.workload == "gluon":
+        out = gluon(**{k: v for k, v in kw.items() if k in ("size", "steps")})
     else:
         out = ensemble(members=a.members, **kw)
     print(json.dumps(out), flush=True)
