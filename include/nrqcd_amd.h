@@ -263,6 +263,27 @@ int nrq_heavy_norms(const nrq_heavy *h, double *norm_quark, double *norm_antiq);
 /* GetMesoncorrelator_{O1_1S0, O1_3S1, O8_1S0, O8_3S1}_ZeroMomentum,
  * nrqcd.f90:1484-1661; out = 4 complex numbers in that order */
 int nrq_heavy_correlators(const nrq_heavy *h, double out[8]);
+/* GetMesoncorrelator_{O1_1S0, O1_3S1, O8_1S0, O8_3S1}_ZeroMomentum_gconjg,
+ * nrqcd.f90:1728-1905 (conj(A) in place of A^dag); same order as above */
+int nrq_heavy_correlators_gconjg(const nrq_heavy *h, double out[8]);
+
+/* ---- spectral functions (main.f90 ComputeSpectrum, twfft.f90) --------- */
+
+/* ComputeSpectrum, main.f90:1865-2100 (Symmetrisation :1970-1992,
+ * ApplyHannWindow windowing.f90:11-25, Symm2FFTformat :2000-2016, t2w
+ * twfft.f90:73-105, spectrum = 2 Im), for `batch` correlators at once.
+ * signals: batch x n complex (t = 0, dt, ..., (n-1) dt; n >= 2);
+ * spectra: batch x (2n - 2) reals in transform order (index 0: omega = 0,
+ * index l: omega = 2 pi l / (dt (2n - 2)), upper half negative frequencies);
+ * stages: NULL, or batch x 3 x (2n - 2) complex receiving the symmetrised,
+ * the Hann-windowed and the reordered signal of every correlator (the
+ * reference's symm.txt, hanned.txt and fftformat.txt). */
+int nrq_spectral_functions(const double *signals, int64_t n, int64_t batch, double dt, double *spectra,
+                           double *stages);
+/* t2w, twfft.f90:73-105: in place, F_l = dt sum_j f_j exp(+2 pi i j l / n) */
+int nrq_t2w(double *data, int64_t n, double dt);
+/* w2t, twfft.f90:33-66: in place, f_j = sum_l F_l exp(-2 pi i j l / n) / (dt n) */
+int nrq_w2t(double *data, int64_t n, double dt);
 
 /* ---- learned preconditioner ------------------------------------------
  * No reference counterpart (the reference uses PETSc's ILU(0)/block-Jacobi,

@@ -154,6 +154,10 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_cn_set_preconditioner": (I, [P]),
         "nrq_heavy_norms": (I, [P, P, P]),
         "nrq_heavy_correlators": (I, [P, P]),
+        "nrq_heavy_correlators_gconjg": (I, [P, P]),
+        "nrq_spectral_functions": (I, [P, I64, I64, D, P, P]),
+        "nrq_t2w": (I, [P, I64, D]),
+        "nrq_w2t": (I, [P, I64, D]),
     }
     for name, (res, args) in sig.items():
         f = getattr(lib, name)
@@ -529,9 +533,12 @@ class NRQCDField:
     def GetNorm_AntiQ(self) -> float:
         return self.norms()[1]
 
-    def correlators(self) -> dict:
+    def correlators(self, gconjg: bool = False) -> dict:
+        """GetMesoncorrelator_{O1,O8}_{1S0,3S1}_ZeroMomentum (nrqcd.f90:1484-1661),
+        or their _gconjg variants (:1728-1905) with conj(A) in place of A^dag."""
         out = np.zeros(4, np.complex128)
-        _check(library().nrq_heavy_correlators(self._h, _ptr(out)))
+        fn = library().nrq_heavy_correlators_gconjg if gconjg else library().nrq_heavy_correlators
+        _check(fn(self._h, _ptr(out)))
         return {"O1_1S0": out[0], "O1_3S1": out[1], "O8_1S0": out[2], "O8_3S1": out[3]}
 
     def GetMesoncorrelator_O1_3S1_ZeroMomentum(self) -> complex:
@@ -606,3 +613,35 @@ class PreconditionerNet:
 def nrqcd_set_preconditioner(net) -> None:
     """Network used by NRQ_SOLVER_FGMRES_CNN (None: none)."""
     _check(library().nrq_cn_set_preconditioner(net.handle if net is not None else None))
+
+
+def spectral_functions(signals, dt: float, stages: bool = False):
+    """ComputeSpectrum (main.f90:1865-2100) on the GPU for a batch of
+    correlators: signals (batch, n) or (n,) complex at t = 0, dt, ...;
+    returns spectra (batch, 2n - 2) in transform order (and, with stages,
+    the symmetrised, windowed and reordered signals, (batch, 3, 2n - 2))."""
+    sig = np.ascontiguousarray(np.atleast_2d(np.asarray(signals, np.complex128)))
+    batch, n = sig.shape
+    N = max(2 * n - 2, 0)
+    spec = np.zeros((batch, N))
+    st = np.zeros((batch, 3, N), np.complex128) if stages else None
+    _check(library().nrq_spectral_functions(_ptr(sig), n, batch, float(dt), _ptr(spec), _ptr(st)))
+    squeeze = np.asarray(signals).ndim == 1
+    if squeeze:
+        spec = spec[0]
+        st = st[0] if stages else None
+    return (spec, st) if stages else spec
+
+
+def t2w(data, dt: float) -> np.ndarray:
+    """twfft.f90:73-105: dt sum_j f_j exp(+2 pi i j l / n)"""
+    x = np.ascontiguousarray(np.asarray(data, np.complex128)).copy()
+    _check(library().nrq_t2w(_ptr(x), x.size, float(dt)))
+    return x
+
+
+def w2t(data, dt: float) -> np.ndarray:
+    """twfft.f90:33-66: sum_l F_l exp(-2 pi i j l / n) / (dt n)"""
+    x = np.ascontiguousarray(np.asarray(data, np.complex128)).copy()
+    _check(library().nrq_w2t(_ptr(x), x.size, float(dt)))
+    return x

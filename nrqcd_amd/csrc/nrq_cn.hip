@@ -62,10 +62,11 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
     return t;
 }
 
-__global__ void k_fold8(const double *__restrict__ partial, int nblk, double *__restrict__ out)
+// out[q] = sum over nblk block partials of value q (q < nq <= 64)
+__global__ void k_fold(const double *__restrict__ partial, int nblk, int nq, double *__restrict__ out)
 {
     const int q = threadIdx.x;
-    if (q >= 8) return;
+    if (q >= nq) return;
     double s = 0;
     for (int b = 0; b < nblk; ++b) s += partial[(int64_t)q * nblk + b];
     out[q] = s;
@@ -83,88 +84,84 @@ __global__ void k_set_point(cplx<double> *psi, int64_t vol, int first_field, int
     psi[((int64_t)(first_field + t) * 3 + c) * vol + s] = {v, 0.0};
 }
 
-// Meson correlators per site (nrqcd.f90:1484-1661) from the 24 colour fields.
-// Vertex M = X (x) Y with X in {1, T^a} (reference generator table) and
-// Y in {1, sigma_k} (su2.f90:68-88, no 1/2):
-//   Tr(A^+ M Q M) = sum conj(A_pq) M_pr Q_rt M_tq.
-__device__ __forceinline__ cplx<double> gen_elem(int a, int r, int c)
-{
-    // T^a_{rc}, a = 0..7, reference convention (see nrq_su3.h)
-    const double h = 0.5, s = 0.28867513459481288225457439025098;
-    switch (a) {
-    case 0: return {(r == 0 && c == 1) || (r == 1 && c == 0) ? h : 0.0, 0.0};
-    case 1: return {0.0, (r == 0 && c == 1) ? h : ((r == 1 && c == 0) ? -h : 0.0)};
-    case 2: return {(r == c) ? (r == 0 ? h : (r == 1 ? -h : 0.0)) : 0.0, 0.0};
-    case 3: return {(r == 0 && c == 2) || (r == 2 && c == 0) ? h : 0.0, 0.0};
-    case 4: return {0.0, (r == 0 && c == 2) ? h : ((r == 2 && c == 0) ? -h : 0.0)};
-    case 5: return {(r == 1 && c == 2) || (r == 2 && c == 1) ? h : 0.0, 0.0};
-    case 6: return {0.0, (r == 1 && c == 2) ? h : ((r == 2 && c == 1) ? -h : 0.0)};
-    default: return {(r == c) ? (r == 2 ? -2.0 * s : s) : 0.0, 0.0};
-    }
-}
-__device__ __forceinline__ cplx<double> pauli_elem(int k, int r, int c)
-{
-    // sigma_k (k = 1..3) in the reference's convention, k = 0: identity
-    switch (k) {
-    case 0: return {r == c ? 1.0 : 0.0, 0.0};
-    case 1: return {r != c ? 1.0 : 0.0, 0.0};
-    case 2: return {0.0, (r == 0 && c == 1) ? 1.0 : ((r == 1 && c == 0) ? -1.0 : 0.0)};
-    default: return {r == c ? (r == 0 ? 1.0 : -1.0) : 0.0, 0.0};
-    }
-}
+// Zero-momentum meson correlators (nrqcd.f90:1484-1661, and the _gconjg
+// variants :1728-1905) as traces.  Each is sum_x i Tr(P M Q M) w over
+// vertices M = X (x) Y, colour-spin index d = 2c + s, with P = A^+ (plain) or
+// P = conj(A) (_gconjg).  The vertex sums collapse through
+//   sum_a T^a_ij T^a_kl = (delta_il delta_jk - delta_ij delta_kl / 3) / 2,
+//   sum_k sigma^k_ij sigma^k_kl = 2 delta_il delta_jk - delta_ij delta_kl,
+// which hold for the reference's tables (su3.f90:97-131, su2.f90:68-88):
+// they differ from the textbook matrices by signs only, and every vertex
+// enters twice.  What remains are four traces per site,
+//   t0 = Tr(P Q),  tX = Tr P Tr Q,
+//   tC = Tr(Ps Qs), Ps_ss' = sum_c P_(c,s),(c,s')   (colour traced),
+//   tS = Tr(Pc Qc), Pc_cc' = sum_s P_(c,s),(c',s)   (spin traced),
+// and O1(1S0) = i t0, O1(3S1) = i (2 tS - t0)/3, O8(1S0) = i 3/8 (tC - t0/3),
+// O8(3S1) = i/8 (2 tX - tC - (2 tS - t0)/3); the host forms these from the
+// lattice sums.  Per site the kernel reads the 72 complex numbers of Q and A
+// once and produces both sets: with As, Ac the partial traces of A,
+//   plain:   t0 = sum_e conj(A_e) Q_e,      tC = sum_e conj(As_e) Qs_e, ...
+//   _gconjg: t0 = sum_xy conj(A_xy) Q_yx,   tC = sum_ss' conj(As_ss') Qs_s's, ...
+// Sums out[v * gridDim.x + block]: v = 0..13 = re/im of t0, tX, tC, tS (plain),
+// t0, tC, tS (_gconjg); tX is the same for both.
+constexpr int CORR_SUMS = 14;
 
-__global__ __launch_bounds__(REDUCE_THREADS) void k_corr(const cplx<double> *__restrict__ psi, int64_t vol,
-                                                         double *__restrict__ partial)
+__global__ __launch_bounds__(REDUCE_THREADS) void k_corr_traces(const cplx<double> *__restrict__ psi, int64_t vol,
+                                                                double *__restrict__ partial)
 {
-    // acc: O1_1S0, O1_3S1, O8_1S0, O8_3S1 (re, im)
-    double acc[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+    cplx<double> t0 = {0, 0}, tx = {0, 0}, tc = {0, 0}, ts = {0, 0}, g0 = {0, 0}, gc = {0, 0}, gs = {0, 0};
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x) {
-        cplx<double> Q[36], A[36];
+        // quark: field 2j + (d & 1), colour plane d >> 1 holds Q_dj
+        cplx<double> Q[36];
+#pragma unroll
         for (int j = 0; j < 6; ++j)
-            for (int sp = 0; sp < 2; ++sp)
-                for (int c = 0; c < 3; ++c) {
-                    const int d = 2 * c + sp;
-                    Q[d + 6 * j] = psi[((int64_t)(2 * j + sp) * 3 + c) * vol + s];
-                    A[d + 6 * j] = psi[((int64_t)(12 + 2 * j + sp) * 3 + c) * vol + s];
-                }
-        for (int kind = 0; kind < 4; ++kind) {
-            const int na = (kind >= 2) ? 8 : 1;
-            const int nk = (kind == 1 || kind == 3) ? 3 : 1;
-            const double w = kind == 0 ? 1.0 : (kind == 1 ? 1.0 / 3.0 : (kind == 2 ? 6.0 / 8.0 : 6.0 / 24.0));
-            cplx<double> tot = {0.0, 0.0};
-            for (int a = 0; a < na; ++a)
-                for (int k = 0; k < nk; ++k) {
-                    // M_{(c,s),(c',s')} = X_{cc'} Y_{ss'}
-                    cplx<double> M[36];
-                    for (int p = 0; p < 6; ++p)
-                        for (int q = 0; q < 6; ++q) {
-                            const cplx<double> xv = kind >= 2 ? gen_elem(a, p / 2, q / 2) : cplx<double>{p / 2 == q / 2 ? 1.0 : 0.0, 0.0};
-                            const cplx<double> yv = pauli_elem((kind == 1 || kind == 3) ? k + 1 : 0, p % 2, q % 2);
-                            M[p + 6 * q] = xv * yv;
-                        }
-                    // N = M Q M ; tot += sum conj(A_pq) N_pq
-                    for (int p = 0; p < 6; ++p)
-                        for (int q = 0; q < 6; ++q) {
-                            cplx<double> nv = {0.0, 0.0};
-                            for (int rr = 0; rr < 6; ++rr) {
-                                const cplx<double> mpr = M[p + 6 * rr];
-                                if (mpr.re == 0.0 && mpr.im == 0.0) continue;
-                                cplx<double> qm = {0.0, 0.0};
-                                for (int t = 0; t < 6; ++t) qm = qm + Q[rr + 6 * t] * M[t + 6 * q];
-                                nv = nv + mpr * qm;
-                            }
-                            tot = tot + cmul(A[p + 6 * q], nv);
-                        }
-                }
-            // i * tot * w
-            acc[2 * kind] += -tot.im * w;
-            acc[2 * kind + 1] += tot.re * w;
-        }
+#pragma unroll
+            for (int d = 0; d < 6; ++d) Q[d + 6 * j] = psi[((int64_t)(2 * j + (d & 1)) * 3 + (d >> 1)) * vol + s];
+        cplx<double> As[4], Ac[9], tra = {0, 0};
+#pragma unroll
+        for (int e = 0; e < 4; ++e) As[e] = {0, 0};
+#pragma unroll
+        for (int e = 0; e < 9; ++e) Ac[e] = {0, 0};
+#pragma unroll
+        for (int j = 0; j < 6; ++j)
+#pragma unroll
+            for (int d = 0; d < 6; ++d) {
+                const cplx<double> a = psi[((int64_t)(12 + 2 * j + (d & 1)) * 3 + (d >> 1)) * vol + s];
+                t0 = t0 + cmul(a, Q[d + 6 * j]);
+                g0 = g0 + cmul(a, Q[j + 6 * d]);
+                if ((d >> 1) == (j >> 1)) As[(d & 1) + 2 * (j & 1)] = As[(d & 1) + 2 * (j & 1)] + a;
+                if ((d & 1) == (j & 1)) Ac[(d >> 1) + 3 * (j >> 1)] = Ac[(d >> 1) + 3 * (j >> 1)] + a;
+                if (d == j) tra = tra + a;
+            }
+        cplx<double> trq = {0, 0};
+#pragma unroll
+        for (int d = 0; d < 6; ++d) trq = trq + Q[d + 6 * d];
+        tx = tx + cmul(tra, trq);
+#pragma unroll
+        for (int r = 0; r < 2; ++r)
+#pragma unroll
+            for (int c = 0; c < 2; ++c) {
+                const cplx<double> qs = Q[(2 * 0 + r) + 6 * (2 * 0 + c)] + Q[(2 * 1 + r) + 6 * (2 * 1 + c)] +
+                                        Q[(2 * 2 + r) + 6 * (2 * 2 + c)];
+                tc = tc + cmul(As[r + 2 * c], qs);
+                gc = gc + cmul(As[c + 2 * r], qs);
+            }
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const cplx<double> qc = Q[(2 * r) + 6 * (2 * c)] + Q[(2 * r + 1) + 6 * (2 * c + 1)];
+                ts = ts + cmul(Ac[r + 3 * c], qc);
+                gs = gs + cmul(Ac[c + 3 * r], qc);
+            }
     }
-    __shared__ double sh[8];
-    for (int q = 0; q < 8; ++q) {
-        const double v = block_sum(acc[q], sh);
-        if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = v;
+    const double v[CORR_SUMS] = {t0.re, t0.im, tx.re, tx.im, tc.re, tc.im, ts.re, ts.im,
+                                 g0.re, g0.im, gc.re, gc.im, gs.re, gs.im};
+    __shared__ double sh[REDUCE_THREADS / 64];
+#pragma unroll
+    for (int q = 0; q < CORR_SUMS; ++q) {
+        const double b = block_sum(v[q], sh);
+        if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = b;
     }
 }
 
@@ -438,20 +435,43 @@ extern "C" int nrq_heavy_norms(const nrq_heavy *h, double *nq, double *na)
     return NRQ_OK;
 }
 
-extern "C" int nrq_heavy_correlators(const nrq_heavy *h, double out[8])
+// Lattice sums of the correlator traces (k_corr_traces) -> out[8] for the
+// plain (gconjg = false) or the _gconjg correlators.
+static int heavy_correlators(const nrq_heavy *h, bool gconjg, double out[8])
 {
     NEED_HEAVY(h);
     if (!out) return fail(NRQ_ERR_ARG, "null argument");
     if (int r = krylov_reserve(h->vol, g_cn.initialised ? g_cn.solver.restart : 30, 0)) return r;
-    const int nb = 256;
+    // one block per 1024 sites, at most 1024 blocks (within the 8 x 256 x 48
+    // doubles of the reduction scratch: 14 x 1024 partials + 14 sums)
+    const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (h->vol + 1023) / 1024));
     double *part = krylov_reduce_scratch();
-    double *res = part + 8 * nb;
-    hipLaunchKernelGGL(k_corr, dim3(nb), dim3(REDUCE_THREADS), 0, stream(), (const cplx<double> *)h->psi, h->vol, part);
-    hipLaunchKernelGGL(k_fold8, dim3(1), dim3(64), 0, stream(), part, nb, res);
+    double *res = part + (size_t)CORR_SUMS * nb;
+    hipLaunchKernelGGL(k_corr_traces, dim3(nb), dim3(REDUCE_THREADS), 0, stream(), (const cplx<double> *)h->psi,
+                       h->vol, part);
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, stream(), part, nb, CORR_SUMS, res);
     NRQ_HIP(hipGetLastError());
     double *hostbuf = krylov_host_scratch();
-    NRQ_HIP(hipMemcpyAsync(hostbuf, res, sizeof(double) * 8, hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipMemcpyAsync(hostbuf, res, sizeof(double) * CORR_SUMS, hipMemcpyDeviceToHost, stream()));
     NRQ_HIP(hipStreamSynchronize(stream()));
-    memcpy(out, hostbuf, sizeof(double) * 8);
+    using zc = std::complex<double>;
+    const int o = gconjg ? 8 : 0;
+    const zc t0(hostbuf[o], hostbuf[o + 1]), tx(hostbuf[2], hostbuf[3]);
+    const zc tc(hostbuf[gconjg ? 10 : 4], hostbuf[gconjg ? 11 : 5]);
+    const zc ts(hostbuf[gconjg ? 12 : 6], hostbuf[gconjg ? 13 : 7]);
+    const zc i(0, 1);
+    const zc c[4] = {i * t0, i * (2.0 * ts - t0) / 3.0, i * 0.375 * (tc - t0 / 3.0),
+                     i * 0.125 * (2.0 * tx - tc - (2.0 * ts - t0) / 3.0)};
+    for (int k = 0; k < 4; ++k) {
+        out[2 * k] = c[k].real();
+        out[2 * k + 1] = c[k].imag();
+    }
     return NRQ_OK;
+}
+
+extern "C" int nrq_heavy_correlators(const nrq_heavy *h, double out[8]) { return heavy_correlators(h, false, out); }
+
+extern "C" int nrq_heavy_correlators_gconjg(const nrq_heavy *h, double out[8])
+{
+    return heavy_correlators(h, true, out);
 }

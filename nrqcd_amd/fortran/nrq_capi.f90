@@ -237,6 +237,31 @@ module nrq_capi
        type(c_ptr), value :: h
        complex(c_double_complex), intent(out) :: out(4)
      end function nrq_heavy_correlators
+     integer(c_int) function nrq_heavy_correlators_gconjg(h, out) bind(C, name='nrq_heavy_correlators_gconjg')
+       import :: c_int, c_ptr, c_double_complex
+       type(c_ptr), value :: h
+       complex(c_double_complex), intent(out) :: out(4)
+     end function nrq_heavy_correlators_gconjg
+
+     integer(c_int) function nrq_spectral_functions(signals, n, batch, dt, spectra, stages) &
+          bind(C, name='nrq_spectral_functions')
+       import :: c_int, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: signals, spectra, stages
+       integer(c_int64_t), value :: n, batch
+       real(c_double), value :: dt
+     end function nrq_spectral_functions
+     integer(c_int) function nrq_t2w(data, n, dt) bind(C, name='nrq_t2w')
+       import :: c_int, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: data
+       integer(c_int64_t), value :: n
+       real(c_double), value :: dt
+     end function nrq_t2w
+     integer(c_int) function nrq_w2t(data, n, dt) bind(C, name='nrq_w2t')
+       import :: c_int, c_int64_t, c_double, c_ptr
+       type(c_ptr), value :: data
+       integer(c_int64_t), value :: n
+       real(c_double), value :: dt
+     end function nrq_w2t
   end interface
 
 contains

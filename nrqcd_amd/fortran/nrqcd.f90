@@ -30,6 +30,10 @@ module nrqcd
      procedure, public :: GetMesoncorrelator_O1_3S1_ZeroMomentum
      procedure, public :: GetMesoncorrelator_O8_1S0_ZeroMomentum
      procedure, public :: GetMesoncorrelator_O8_3S1_ZeroMomentum
+     procedure, public :: GetMesoncorrelator_O1_1S0_ZeroMomentum_gconjg
+     procedure, public :: GetMesoncorrelator_O1_3S1_ZeroMomentum_gconjg
+     procedure, public :: GetMesoncorrelator_O8_1S0_ZeroMomentum_gconjg
+     procedure, public :: GetMesoncorrelator_O8_3S1_ZeroMomentum_gconjg
      procedure, public :: GetQuarkProp_G
      procedure, public :: GetAntiQProp_G
      procedure, public :: Download
@@ -198,6 +202,44 @@ contains
     c = correlators(HeavyField)
     res = c(4)
   end function GetMesoncorrelator_O8_3S1_ZeroMomentum
+
+  function correlators_gconjg(HeavyField)
+    class(NRQCDField), intent(in) :: HeavyField
+    complex(fp) :: correlators_gconjg(4)
+    call nrq_check(nrq_heavy_correlators_gconjg(HeavyField%handle, correlators_gconjg), 'meson correlators (gconjg)')
+  end function correlators_gconjg
+
+  !> nrqcd.f90:1728-1761
+  complex(fp) function GetMesoncorrelator_O1_1S0_ZeroMomentum_gconjg(HeavyField) result(res)
+    class(NRQCDField), intent(in) :: HeavyField
+    complex(fp) :: c(4)
+    c = correlators_gconjg(HeavyField)
+    res = c(1)
+  end function GetMesoncorrelator_O1_1S0_ZeroMomentum_gconjg
+
+  !> nrqcd.f90:1769-1809
+  complex(fp) function GetMesoncorrelator_O1_3S1_ZeroMomentum_gconjg(HeavyField) result(res)
+    class(NRQCDField), intent(in) :: HeavyField
+    complex(fp) :: c(4)
+    c = correlators_gconjg(HeavyField)
+    res = c(2)
+  end function GetMesoncorrelator_O1_3S1_ZeroMomentum_gconjg
+
+  !> nrqcd.f90:1817-1857
+  complex(fp) function GetMesoncorrelator_O8_1S0_ZeroMomentum_gconjg(HeavyField) result(res)
+    class(NRQCDField), intent(in) :: HeavyField
+    complex(fp) :: c(4)
+    c = correlators_gconjg(HeavyField)
+    res = c(3)
+  end function GetMesoncorrelator_O8_1S0_ZeroMomentum_gconjg
+
+  !> nrqcd.f90:1865-1905
+  complex(fp) function GetMesoncorrelator_O8_3S1_ZeroMomentum_gconjg(HeavyField) result(res)
+    class(NRQCDField), intent(in) :: HeavyField
+    complex(fp) :: c(4)
+    c = correlators_gconjg(HeavyField)
+    res = c(4)
+  end function GetMesoncorrelator_O8_3S1_ZeroMomentum_gconjg
 
   !> Copies both propagators (6,6,V) to the host.
   subroutine Download(HeavyField, QuarkProp, AntiQProp)

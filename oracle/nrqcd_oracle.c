@@ -1008,13 +1008,17 @@ static void pauli(int k, zc s[4])
 
 void orc_pauli(int k, zc s[4]) { pauli(k, s); }
 
-/* sum_x Tr( i A^dag M Q M ) * w over a list of 6x6 vertex matrices M */
-static zc corr_generic(const zc *quark, const zc *antiq, const zc *Ms, int nm, double w)
+/* sum_x Tr( i P M Q M ) * w over a list of 6x6 vertex matrices M, with
+ * P = A^dag, or P = conj(A) for the _gconjg variants (nrqcd.f90:1728-1905) */
+static zc corr_generic(const zc *quark, const zc *antiq, const zc *Ms, int nm, double w, int gconjg)
 {
     zc tot = 0;
     for (int64_t x = 1; x <= VOL; ++x) {
         zc ad[36], t1[36], t2[36];
-        dag6(PRP(antiq, x), ad);
+        if (gconjg)
+            for (int e = 0; e < 36; ++e) ad[e] = conj(PRP(antiq, x)[e]);
+        else
+            dag6(PRP(antiq, x), ad);
         zc loc = 0;
         for (int m = 0; m < nm; ++m) {
             const zc *M = Ms + 36 * m;
@@ -1030,36 +1034,36 @@ static zc corr_generic(const zc *quark, const zc *antiq, const zc *Ms, int nm, d
     return tot;
 }
 
-/* GetMesoncorrelator_O1_1S0_ZeroMomentum, nrqcd.f90:1484-1517 */
-zc orc_corr_o1_1s0(const zc *quark, const zc *antiq)
+/* GetMesoncorrelator_O1_1S0_ZeroMomentum, nrqcd.f90:1484-1517 (gconjg: :1728-1761) */
+static zc corr_o1_1s0(const zc *quark, const zc *antiq, int gconjg)
 {
     zc M[36] = {0};
     for (int d = 0; d < 6; ++d) M[d + 6 * d] = 1;
-    return corr_generic(quark, antiq, M, 1, 1.0);
+    return corr_generic(quark, antiq, M, 1, 1.0, gconjg);
 }
 
-/* GetMesoncorrelator_O1_3S1_ZeroMomentum, nrqcd.f90:1525-1566: S2CS(sigma_k) */
-zc orc_corr_o1_3s1(const zc *quark, const zc *antiq)
+/* GetMesoncorrelator_O1_3S1_ZeroMomentum, nrqcd.f90:1525-1566: S2CS(sigma_k) (gconjg: :1769-1809) */
+static zc corr_o1_3s1(const zc *quark, const zc *antiq, int gconjg)
 {
     zc one3[9] = {1, 0, 0, 0, 1, 0, 0, 0, 1}, s[4], Ms[3 * 36];
     for (int k = 1; k <= 3; ++k) {
         pauli(k, s);
         orc_kron(one3, 3, 3, s, 2, 2, Ms + 36 * (k - 1));
     }
-    return corr_generic(quark, antiq, Ms, 3, 1.0 / 3.0);
+    return corr_generic(quark, antiq, Ms, 3, 1.0 / 3.0, gconjg);
 }
 
-/* GetMesoncorrelator_O8_1S0_ZeroMomentum, nrqcd.f90:1574-1611: C2CS(T^a) */
-zc orc_corr_o8_1s0(const zc *quark, const zc *antiq)
+/* GetMesoncorrelator_O8_1S0_ZeroMomentum, nrqcd.f90:1574-1611: C2CS(T^a) (gconjg: :1817-1857) */
+static zc corr_o8_1s0(const zc *quark, const zc *antiq, int gconjg)
 {
     init_generators();
     zc one2[4] = {1, 0, 0, 1}, Ms[8 * 36];
     for (int a = 0; a < 8; ++a) orc_kron(GEN[a], 3, 3, one2, 2, 2, Ms + 36 * a);
-    return corr_generic(quark, antiq, Ms, 8, 6.0 / 8.0);
+    return corr_generic(quark, antiq, Ms, 8, 6.0 / 8.0, gconjg);
 }
 
-/* GetMesoncorrelator_O8_3S1_ZeroMomentum, nrqcd.f90:1619-1661: T^a (x) sigma_k */
-zc orc_corr_o8_3s1(const zc *quark, const zc *antiq)
+/* GetMesoncorrelator_O8_3S1_ZeroMomentum, nrqcd.f90:1619-1661: T^a (x) sigma_k (gconjg: :1865-1905) */
+static zc corr_o8_3s1(const zc *quark, const zc *antiq, int gconjg)
 {
     init_generators();
     zc s[4], Ms[24 * 36];
@@ -1068,8 +1072,17 @@ zc orc_corr_o8_3s1(const zc *quark, const zc *antiq)
         pauli(k, s);
         for (int a = 0; a < 8; ++a) orc_kron(GEN[a], 3, 3, s, 2, 2, Ms + 36 * (m++));
     }
-    return corr_generic(quark, antiq, Ms, 24, 6.0 / 24.0);
+    return corr_generic(quark, antiq, Ms, 24, 6.0 / 24.0, gconjg);
 }
+
+zc orc_corr_o1_1s0(const zc *quark, const zc *antiq) { return corr_o1_1s0(quark, antiq, 0); }
+zc orc_corr_o1_3s1(const zc *quark, const zc *antiq) { return corr_o1_3s1(quark, antiq, 0); }
+zc orc_corr_o8_1s0(const zc *quark, const zc *antiq) { return corr_o8_1s0(quark, antiq, 0); }
+zc orc_corr_o8_3s1(const zc *quark, const zc *antiq) { return corr_o8_3s1(quark, antiq, 0); }
+zc orc_corr_o1_1s0_gconjg(const zc *quark, const zc *antiq) { return corr_o1_1s0(quark, antiq, 1); }
+zc orc_corr_o1_3s1_gconjg(const zc *quark, const zc *antiq) { return corr_o1_3s1(quark, antiq, 1); }
+zc orc_corr_o8_1s0_gconjg(const zc *quark, const zc *antiq) { return corr_o8_1s0(quark, antiq, 1); }
+zc orc_corr_o8_3s1_gconjg(const zc *quark, const zc *antiq) { return corr_o8_3s1(quark, antiq, 1); }
 
 int orc_num_threads(void)
 {

@@ -89,6 +89,10 @@ zc      orc_corr_o1_1s0(const zc *quark, const zc *antiq);
 zc      orc_corr_o1_3s1(const zc *quark, const zc *antiq);
 zc      orc_corr_o8_1s0(const zc *quark, const zc *antiq);
 zc      orc_corr_o8_3s1(const zc *quark, const zc *antiq);
+zc      orc_corr_o1_1s0_gconjg(const zc *quark, const zc *antiq);
+zc      orc_corr_o1_3s1_gconjg(const zc *quark, const zc *antiq);
+zc      orc_corr_o8_1s0_gconjg(const zc *quark, const zc *antiq);
+zc      orc_corr_o8_3s1_gconjg(const zc *quark, const zc *antiq);
 
 int     orc_num_threads(void);
 

@@ -69,7 +69,9 @@ class Oracle:
             f = getattr(L, name)
             f.restype = res
             f.argtypes = args
-        for name in ["orc_corr_o1_1s0", "orc_corr_o1_3s1", "orc_corr_o8_1s0", "orc_corr_o8_3s1"]:
+        for name in ["orc_corr_o1_1s0", "orc_corr_o1_3s1", "orc_corr_o8_1s0", "orc_corr_o8_3s1",
+                     "orc_corr_o1_1s0_gconjg", "orc_corr_o1_3s1_gconjg", "orc_corr_o8_1s0_gconjg",
+                     "orc_corr_o8_3s1_gconjg"]:
             f = getattr(L, name)
             f.restype = _Z
             f.argtypes = [P, P]
@@ -243,10 +245,13 @@ class Oracle:
     def prop_norm(self, Q) -> float:
         return self.lib.orc_prop_norm(_ptr(Q))
 
-    def correlators(self, Q, A) -> dict:
+    def correlators(self, Q, A, gconjg=False) -> dict:
+        """O1/O8 x 1S0/3S1 at zero momentum; gconjg selects the variants with
+        conj(A) in place of A^dag."""
+        sfx = "_gconjg" if gconjg else ""
         out = {}
-        for key, fn in [("O1_1S0", self.lib.orc_corr_o1_1s0), ("O1_3S1", self.lib.orc_corr_o1_3s1),
-                        ("O8_1S0", self.lib.orc_corr_o8_1s0), ("O8_3S1", self.lib.orc_corr_o8_3s1)]:
+        for key in ("O1_1S0", "O1_3S1", "O8_1S0", "O8_3S1"):
+            fn = getattr(self.lib, "orc_corr_" + key.lower() + sfx)
             r = fn(_ptr(Q), _ptr(A))
             out[key] = complex(r.re, r.im)
         return out

@@ -58,10 +58,11 @@ def test_cn_step_matches_oracle(nrq, oracle, cn):
     nq, na = h.norms()
     assert nq == pytest.approx(oracle.prop_norm(Q), rel=1e-11)
     assert na == pytest.approx(oracle.prop_norm(A), rel=1e-11)
-    corr = h.correlators()
-    ocorr = oracle.correlators(Q, A)
-    for k in ocorr:
-        assert abs(corr[k] - ocorr[k]) < 1e-10 * max(1.0, abs(ocorr[k])), k
+    for gconjg in (False, True):
+        corr = h.correlators(gconjg)
+        ocorr = oracle.correlators(Q, A, gconjg)
+        for k in ocorr:
+            assert abs(corr[k] - ocorr[k]) < 1e-10 * max(1.0, abs(ocorr[k])), (k, gconjg)
 
 
 def test_cn_general_propagator_and_anisotropy(nrq, oracle, cn):
@@ -82,6 +83,13 @@ def test_cn_general_propagator_and_anisotropy(nrq, oracle, cn):
     h.upload(Q, A)
     Qd, Ad = h.download()
     assert np.array_equal(Qd, Q) and np.array_equal(Ad, A)
+    # all eight correlators of general (non-colour-diagonal) propagators:
+    # the GPU's trace form against the oracle's explicit vertex sums
+    for gconjg in (False, True):
+        corr = h.correlators(gconjg)
+        ocorr = oracle.correlators(Q, A, gconjg)
+        for k in ocorr:
+            assert abs(corr[k] - ocorr[k]) < 1e-12 * abs(ocorr[k]), (k, gconjg)
     h.Update(g, 1.3, c, 0.5)
     rc, _ = oracle.cn_update(U, Q, A, 1.3, c, 0.5, 1e-12)
     assert rc == 0

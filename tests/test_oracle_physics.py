@@ -200,6 +200,9 @@ def test_cn_free_field_closed_form(oracle):
     want = 1j * np.mean(((1 - 0.5j * dt * K / mass) / (1 + 0.5j * dt * K / mass)) ** (2 * nsteps))
     for key, v in corr.items():
         assert abs(v - want) < 1e-11, key
+    # the _gconjg variants see A^T = A here
+    for key, v in oracle.correlators(Q, A, gconjg=True).items():
+        assert abs(v - want) < 1e-11, key
 
 
 def test_cn_norm_conservation_and_reversibility(oracle):
