@@ -62,15 +62,6 @@ __device__ __forceinline__ double block_sum(double v, double *sh)
     return t;
 }
 
-// out[q] = sum over nblk block partials of value q (q < nq <= 64)
-__global__ void k_fold(const double *__restrict__ partial, int nblk, int nq, double *__restrict__ out)
-{
-    const int q = threadIdx.x;
-    if (q >= nq) return;
-    double s = 0;
-    for (int b = 0; b < nblk; ++b) s += partial[(int64_t)q * nblk + b];
-    out[q] = s;
-}
 
 __global__ void k_set_point(cplx<double> *psi, int64_t vol, int first_field, int64_t s, double v)
 {
@@ -98,71 +89,112 @@ __global__ void k_set_point(cplx<double> *psi, int64_t vol, int first_field, int
 //   tS = Tr(Pc Qc), Pc_cc' = sum_s P_(c,s),(c',s)   (spin traced),
 // and O1(1S0) = i t0, O1(3S1) = i (2 tS - t0)/3, O8(1S0) = i 3/8 (tC - t0/3),
 // O8(3S1) = i/8 (2 tX - tC - (2 tS - t0)/3); the host forms these from the
-// lattice sums.  Per site the kernel reads the 72 complex numbers of Q and A
+// lattice sums.  The same pass sums |Q|^2 and |A|^2 for the norms
+// (nrqcd.f90:547-611).  Per site the kernel reads the 72 complex numbers of Q and A
 // once and produces both sets: with As, Ac the partial traces of A,
 //   plain:   t0 = sum_e conj(A_e) Q_e,      tC = sum_e conj(As_e) Qs_e, ...
 //   _gconjg: t0 = sum_xy conj(A_xy) Q_yx,   tC = sum_ss' conj(As_ss') Qs_s's, ...
 // Sums out[v * gridDim.x + block]: v = 0..13 = re/im of t0, tX, tC, tS (plain),
-// t0, tC, tS (_gconjg); tX is the same for both.
-constexpr int CORR_SUMS = 14;
+// t0, tC, tS (_gconjg), tX being the same for both; v = 14, 15: |Q|^2, |A|^2.
+constexpr int CORR_SUMS = 16;
+
+// Accumulates one propagator entry (row r, column c) of Q and A into the
+// partial traces, traces and squared norms.
+struct corr_acc {
+    cplx<double> qs[4], as[4], qc[9], ac[9], trq, tra;
+    double nq, na;
+    __device__ __forceinline__ void entry(int r, int c, cplx<double> q, cplx<double> a)
+    {
+        nq += q.re * q.re + q.im * q.im;
+        na += a.re * a.re + a.im * a.im;
+        if ((r >> 1) == (c >> 1)) { // same colour: spin block of the colour trace
+            qs[(r & 1) + 2 * (c & 1)] = qs[(r & 1) + 2 * (c & 1)] + q;
+            as[(r & 1) + 2 * (c & 1)] = as[(r & 1) + 2 * (c & 1)] + a;
+        }
+        if ((r & 1) == (c & 1)) { // same spin: colour block of the spin trace
+            qc[(r >> 1) + 3 * (c >> 1)] = qc[(r >> 1) + 3 * (c >> 1)] + q;
+            ac[(r >> 1) + 3 * (c >> 1)] = ac[(r >> 1) + 3 * (c >> 1)] + a;
+        }
+        if (r == c) {
+            trq = trq + q;
+            tra = tra + a;
+        }
+    }
+};
 
 __global__ __launch_bounds__(REDUCE_THREADS) void k_corr_traces(const cplx<double> *__restrict__ psi, int64_t vol,
                                                                 double *__restrict__ partial)
 {
     cplx<double> t0 = {0, 0}, tx = {0, 0}, tc = {0, 0}, ts = {0, 0}, g0 = {0, 0}, gc = {0, 0}, gs = {0, 0};
+    double nq = 0, na = 0;
+    // entry (d, j) of the quark (field 2j + (d & 1), colour plane d >> 1) and
+    // of the antiquark (fields 12 + ...)
+    auto at = [&](int f0, int d, int j, int64_t s) { return psi[((int64_t)(f0 + 2 * j + (d & 1)) * 3 + (d >> 1)) * vol + s]; };
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x) {
-        // quark: field 2j + (d & 1), colour plane d >> 1 holds Q_dj
-        cplx<double> Q[36];
+        corr_acc w;
 #pragma unroll
-        for (int j = 0; j < 6; ++j)
+        for (int e = 0; e < 4; ++e) w.qs[e] = w.as[e] = {0, 0};
 #pragma unroll
-            for (int d = 0; d < 6; ++d) Q[d + 6 * j] = psi[((int64_t)(2 * j + (d & 1)) * 3 + (d >> 1)) * vol + s];
-        cplx<double> As[4], Ac[9], tra = {0, 0};
+        for (int e = 0; e < 9; ++e) w.qc[e] = w.ac[e] = {0, 0};
+        w.trq = w.tra = {0, 0};
+        w.nq = w.na = 0;
+        // entries in transposed pairs (d, j), (j, d): t0 pairs equal
+        // positions, the _gconjg t0 pairs transposed ones
 #pragma unroll
-        for (int e = 0; e < 4; ++e) As[e] = {0, 0};
+        for (int d = 0; d < 6; ++d)
 #pragma unroll
-        for (int e = 0; e < 9; ++e) Ac[e] = {0, 0};
-#pragma unroll
-        for (int j = 0; j < 6; ++j)
-#pragma unroll
-            for (int d = 0; d < 6; ++d) {
-                const cplx<double> a = psi[((int64_t)(12 + 2 * j + (d & 1)) * 3 + (d >> 1)) * vol + s];
-                t0 = t0 + cmul(a, Q[d + 6 * j]);
-                g0 = g0 + cmul(a, Q[j + 6 * d]);
-                if ((d >> 1) == (j >> 1)) As[(d & 1) + 2 * (j & 1)] = As[(d & 1) + 2 * (j & 1)] + a;
-                if ((d & 1) == (j & 1)) Ac[(d >> 1) + 3 * (j >> 1)] = Ac[(d >> 1) + 3 * (j >> 1)] + a;
-                if (d == j) tra = tra + a;
+            for (int j = d; j < 6; ++j) {
+                const cplx<double> q1 = at(0, d, j, s), a1 = at(12, d, j, s);
+                w.entry(d, j, q1, a1);
+                t0 = t0 + cmul(a1, q1);
+                if (j == d) {
+                    g0 = g0 + cmul(a1, q1);
+                } else {
+                    const cplx<double> q2 = at(0, j, d, s), a2 = at(12, j, d, s);
+                    w.entry(j, d, q2, a2);
+                    t0 = t0 + cmul(a2, q2);
+                    g0 = g0 + cmul(a1, q2) + cmul(a2, q1);
+                }
             }
-        cplx<double> trq = {0, 0};
-#pragma unroll
-        for (int d = 0; d < 6; ++d) trq = trq + Q[d + 6 * d];
-        tx = tx + cmul(tra, trq);
+        nq += w.nq;
+        na += w.na;
+        tx = tx + cmul(w.tra, w.trq);
 #pragma unroll
         for (int r = 0; r < 2; ++r)
 #pragma unroll
             for (int c = 0; c < 2; ++c) {
-                const cplx<double> qs = Q[(2 * 0 + r) + 6 * (2 * 0 + c)] + Q[(2 * 1 + r) + 6 * (2 * 1 + c)] +
-                                        Q[(2 * 2 + r) + 6 * (2 * 2 + c)];
-                tc = tc + cmul(As[r + 2 * c], qs);
-                gc = gc + cmul(As[c + 2 * r], qs);
+                tc = tc + cmul(w.as[r + 2 * c], w.qs[r + 2 * c]);
+                gc = gc + cmul(w.as[c + 2 * r], w.qs[r + 2 * c]);
             }
 #pragma unroll
         for (int r = 0; r < 3; ++r)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const cplx<double> qc = Q[(2 * r) + 6 * (2 * c)] + Q[(2 * r + 1) + 6 * (2 * c + 1)];
-                ts = ts + cmul(Ac[r + 3 * c], qc);
-                gs = gs + cmul(Ac[c + 3 * r], qc);
+                ts = ts + cmul(w.ac[r + 3 * c], w.qc[r + 3 * c]);
+                gs = gs + cmul(w.ac[c + 3 * r], w.qc[r + 3 * c]);
             }
     }
     const double v[CORR_SUMS] = {t0.re, t0.im, tx.re, tx.im, tc.re, tc.im, ts.re, ts.im,
-                                 g0.re, g0.im, gc.re, gc.im, gs.re, gs.im};
+                                 g0.re, g0.im, gc.re, gc.im, gs.re, gs.im, nq, na};
     __shared__ double sh[REDUCE_THREADS / 64];
 #pragma unroll
     for (int q = 0; q < CORR_SUMS; ++q) {
         const double b = block_sum(v[q], sh);
         if (threadIdx.x == 0) partial[(int64_t)q * gridDim.x + blockIdx.x] = b;
     }
+}
+
+// out[q] = sum of the nblk partials of value q; one block per value, fixed
+// order (thread t sums t, t + 256, ..., then the block sum).
+__global__ __launch_bounds__(REDUCE_THREADS) void k_fold(const double *__restrict__ partial, int nblk,
+                                                         double *__restrict__ out)
+{
+    const int q = blockIdx.x;
+    double s = 0;
+    for (int b = threadIdx.x; b < nblk; b += REDUCE_THREADS) s += partial[(int64_t)q * nblk + b];
+    __shared__ double sh[REDUCE_THREADS / 64];
+    s = block_sum(s, sh);
+    if (threadIdx.x == 0) out[q] = s;
 }
 
 } // namespace nrq
@@ -235,6 +267,7 @@ extern "C" int nrq_heavy_create(nrq_heavy **out)
     }
     (void)hipMemsetAsync(h->psi, 0, heavy_bytes(h->vol), stream());
     h->live = 0;
+    ++h->version;
     *out = h;
     return NRQ_OK;
 }
@@ -261,6 +294,12 @@ extern "C" int nrq_heavy_copy(nrq_heavy *dst, const nrq_heavy *src)
     }
     NRQ_HIP(hipMemcpyAsync(dst->psi, src->psi, heavy_bytes(src->vol), hipMemcpyDeviceToDevice, stream()));
     dst->live = src->live;
+    ++dst->version;
+    dst->obs_version = ~(uint64_t)0;
+    if (src->obs_version == src->version) { // the copy has the same observables
+        std::memcpy(dst->obs, src->obs, sizeof(dst->obs));
+        dst->obs_version = dst->version;
+    }
     return NRQ_OK;
 }
 
@@ -268,6 +307,7 @@ extern "C" int nrq_heavy_init_point(nrq_heavy *h, int64_t iq, int64_t ia)
 {
     NEED_HEAVY(h);
     if (iq < 1 || iq > h->vol || ia < 1 || ia > h->vol) return fail(NRQ_ERR_ARG, "heavy: lattice index out of range");
+    ++h->version;
     NRQ_HIP(hipMemsetAsync(h->psi, 0, heavy_bytes(h->vol), stream()));
     const double v = 1.0 / std::sqrt(6.0);
     hipLaunchKernelGGL(k_set_point, dim3(1), dim3(64), 0, stream(), (cplx<double> *)h->psi, h->vol, 0, iq - 1, v);
@@ -287,6 +327,7 @@ extern "C" int nrq_heavy_init_point_dof(nrq_heavy *h, int spin_q, int colour_q, 
     if (iq < 1 || iq > h->vol || ia < 1 || ia > h->vol) return fail(NRQ_ERR_ARG, "heavy: lattice index out of range");
     if (spin_q < 1 || spin_q > 2 || spin_a < 1 || spin_a > 2 || colour_q < 1 || colour_q > 3 || colour_a < 1 || colour_a > 3)
         return fail(NRQ_ERR_ARG, "heavy: spin 1..2, colour 1..3");
+    ++h->version;
     NRQ_HIP(hipMemsetAsync(h->psi, 0, heavy_bytes(h->vol), stream()));
     // nrqcd.f90:437-444: index = colour + 3 (spin - 1) (1-based), used as both
     // row and column of the 6x6 propagator.  In the C2CS row ordering d =
@@ -325,6 +366,7 @@ extern "C" int nrq_heavy_upload(nrq_heavy *h, const double *quark, const double 
                     if (z[0] != 0.0 || z[1] != 0.0) live |= 1u << (12 * p + 2 * j + sp);
                 }
         h->live = (h->live & ~(0xfffu << (12 * p))) | live;
+        ++h->version;
         NRQ_HIP(hipMemcpyAsync((cplx<double> *)h->psi + (size_t)12 * p * 3 * (size_t)v, buf.data(), buf.size() * sizeof(cplx<double>),
                                hipMemcpyHostToDevice, stream()));
         NRQ_HIP(hipStreamSynchronize(stream()));
@@ -401,6 +443,7 @@ static int heavy_update_n(nrq_heavy *const *hs, const double *mass, int n, const
     nrq_solve_stats st = {0, 0, 0, 0.0};
     st.solves = R;
     if (int r = krylov_reserve(hs[0]->vol, g_cn.solver.restart, R)) return r;
+    for (int k = 0; k < n; ++k) ++hs[k]->version; // the solve overwrites the live fields
     const int status = krylov_cn_step(gf, fields, R, alpha, beta, g_cn.solver, &st);
     if (status != NRQ_OK && status != NRQ_ERR_NOCONV) return status;
     NRQ_HIP(hipStreamSynchronize(stream()));
@@ -420,45 +463,52 @@ extern "C" int nrq_heavy_update_batch(nrq_heavy *const *h, const double *mass, i
     return heavy_update_n(h, mass, n, gf, c, stepwidth, stats);
 }
 
-extern "C" int nrq_heavy_norms(const nrq_heavy *h, double *nq, double *na)
+// Brings h->obs up to date with the field (one k_corr_traces pass when the
+// field changed since the last evaluation).
+static int heavy_observables(const nrq_heavy *hc)
 {
-    NEED_HEAVY(h);
-    if (!nq || !na) return fail(NRQ_ERR_ARG, "null argument");
+    NEED_HEAVY(hc);
+    nrq_heavy *h = const_cast<nrq_heavy *>(hc); // the cache is not part of the field's value
+    if (h->obs_version == h->version) return NRQ_OK;
     if (int r = krylov_reserve(h->vol, g_cn.initialised ? g_cn.solver.restart : 30, 0)) return r;
-    double n2[NFIELDS];
-    if (int e = krylov_norms2((const cplx<double> *)h->psi, NFIELDS, n2)) return e;
-    double q = 0, a = 0;
-    for (int r = 0; r < 12; ++r) q += n2[r];
-    for (int r = 12; r < 24; ++r) a += n2[r];
-    *nq = std::sqrt(q);
-    *na = std::sqrt(a);
-    return NRQ_OK;
-}
-
-// Lattice sums of the correlator traces (k_corr_traces) -> out[8] for the
-// plain (gconjg = false) or the _gconjg correlators.
-static int heavy_correlators(const nrq_heavy *h, bool gconjg, double out[8])
-{
-    NEED_HEAVY(h);
-    if (!out) return fail(NRQ_ERR_ARG, "null argument");
-    if (int r = krylov_reserve(h->vol, g_cn.initialised ? g_cn.solver.restart : 30, 0)) return r;
-    // one block per 1024 sites, at most 1024 blocks (within the 8 x 256 x 48
-    // doubles of the reduction scratch: 14 x 1024 partials + 14 sums)
-    const int nb = (int)std::min<int64_t>(1024, std::max<int64_t>(1, (h->vol + 1023) / 1024));
+    // one site per thread up to 4096 blocks (the partials, 16 x 4096 doubles,
+    // and the 16 sums fit the reduction scratch)
+    static_assert((size_t)CORR_SUMS * 4096 + CORR_SUMS <= KRYLOV_REDUCE_DOUBLES, "correlator partials");
+    const int nb = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (h->vol + REDUCE_THREADS - 1) / REDUCE_THREADS));
     double *part = krylov_reduce_scratch();
     double *res = part + (size_t)CORR_SUMS * nb;
     hipLaunchKernelGGL(k_corr_traces, dim3(nb), dim3(REDUCE_THREADS), 0, stream(), (const cplx<double> *)h->psi,
                        h->vol, part);
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, stream(), part, nb, CORR_SUMS, res);
+    hipLaunchKernelGGL(k_fold, dim3(CORR_SUMS), dim3(REDUCE_THREADS), 0, stream(), part, nb, res);
     NRQ_HIP(hipGetLastError());
     double *hostbuf = krylov_host_scratch();
     NRQ_HIP(hipMemcpyAsync(hostbuf, res, sizeof(double) * CORR_SUMS, hipMemcpyDeviceToHost, stream()));
     NRQ_HIP(hipStreamSynchronize(stream()));
+    std::memcpy(h->obs, hostbuf, sizeof(double) * CORR_SUMS);
+    h->obs_version = h->version;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_heavy_norms(const nrq_heavy *h, double *nq, double *na)
+{
+    if (!nq || !na) return fail(NRQ_ERR_ARG, "null argument");
+    if (int r = heavy_observables(h)) return r;
+    *nq = std::sqrt(h->obs[14]);
+    *na = std::sqrt(h->obs[15]);
+    return NRQ_OK;
+}
+
+// The plain (gconjg = false) or _gconjg correlators from the trace sums.
+static int heavy_correlators(const nrq_heavy *h, bool gconjg, double out[8])
+{
+    if (!out) return fail(NRQ_ERR_ARG, "null argument");
+    if (int r = heavy_observables(h)) return r;
     using zc = std::complex<double>;
+    const double *t = h->obs;
     const int o = gconjg ? 8 : 0;
-    const zc t0(hostbuf[o], hostbuf[o + 1]), tx(hostbuf[2], hostbuf[3]);
-    const zc tc(hostbuf[gconjg ? 10 : 4], hostbuf[gconjg ? 11 : 5]);
-    const zc ts(hostbuf[gconjg ? 12 : 6], hostbuf[gconjg ? 13 : 7]);
+    const zc t0(t[o], t[o + 1]), tx(t[2], t[3]);
+    const zc tc(t[gconjg ? 10 : 4], t[gconjg ? 11 : 5]);
+    const zc ts(t[gconjg ? 12 : 6], t[gconjg ? 13 : 7]);
     const zc i(0, 1);
     const zc c[4] = {i * t0, i * (2.0 * ts - t0) / 3.0, i * 0.375 * (tc - t0 / 3.0),
                      i * 0.125 * (2.0 * tx - tc - (2.0 * ts - t0) / 3.0)};

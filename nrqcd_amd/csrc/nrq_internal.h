@@ -66,4 +66,11 @@ struct nrq_heavy {
     int64_t vol = 0;
     void *psi = nullptr;      // 24 colour-vector fields [field][colour][site] complex double
     unsigned live = 0;        // bit f set: field f may be non-zero (clear bits are exactly zero)
+    // Every change of psi increments version.  obs holds the lattice sums of
+    // the correlator traces and the two squared norms of field version
+    // obs_version, so the ten observables a driver reads per time step
+    // (eight correlators, two norms) cost one pass over psi.
+    uint64_t version = 0;
+    uint64_t obs_version = ~(uint64_t)0;
+    double obs[16] = {};
 };

@@ -48,7 +48,9 @@ void krylov_release();
 const cplx<double> *krylov_last_rhs();
 // squared norms of nf consecutive FP64 fields into out[nf] (host)
 int krylov_norms2(const cplx<double> *x, int nf, double *out);
-// partial sums for the correlator kernel: device scratch of >= 8 x 256 doubles
+// partial sums for the correlator kernel: device scratch of
+// KRYLOV_REDUCE_DOUBLES doubles
+constexpr size_t KRYLOV_REDUCE_DOUBLES = (size_t)NSLOTS * 256 * 8;
 double *krylov_reduce_scratch();
 double *krylov_host_scratch(); // pinned, >= 64 doubles
 

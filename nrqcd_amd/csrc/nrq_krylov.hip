@@ -609,7 +609,7 @@ struct workspace {
                   dm((void **)&hdev, sizeof(cplx<double>) * NSLOTS * (mm + 1)) &&
                   dm((void **)&ydev, sizeof(cplx<double>) * NSLOTS * (mm + 1)) &&
                   dm((void **)&cpart, sizeof(cplx<double>) * NSLOTS * (mm + 1) * VBLOCKS) &&
-                  dm((void **)&rpart, sizeof(double) * NSLOTS * VBLOCKS * 8) &&
+                  dm((void **)&rpart, sizeof(double) * std::max(KRYLOV_REDUCE_DOUBLES, (size_t)NSLOTS * VBLOCKS * 8)) &&
                   dm((void **)&spart, sizeof(double) * NSLOTS * (size_t)sb) && dm((void **)&nrm, sizeof(double) * NRM) &&
                   dm((void **)&active, sizeof(int) * NSLOTS) && dm((void **)&nused, sizeof(int) * NSLOTS) &&
                   hipHostMalloc((void **)&hbuf, sizeof(cplx<double>) * NSLOTS * (mm + 1), hipHostMallocDefault) == hipSuccess &&

@@ -29,37 +29,40 @@ namespace nrq {
 
 constexpr int SPEC_THREADS = 256;
 
+// Writes symmetrised entry k of member b: window, rotation, stages.
+__device__ __forceinline__ void spec_put(cplx<double> s, int64_t k, int64_t b, int64_t N, cplx<double> *__restrict__ f,
+                                         cplx<double> *__restrict__ stages)
+{
+    const double pi = 3.14159265358979323846;
+    // sin(pi (i-1) / N)^2 with the reference's evaluation order
+    const double sn = sin((pi * (double)k) / (double)N);
+    const double w = sn * sn;
+    const cplx<double> h = {s.re * w, s.im * w};
+    int64_t j = k + N / 2;
+    if (j >= N) j -= N;
+    f[b * N + j] = h;
+    if (stages) {
+        stages[(b * 3 + 0) * N + k] = s;
+        stages[(b * 3 + 1) * N + k] = h;
+        stages[(b * 3 + 2) * N + j] = h;
+    }
+}
+
 // Builds f (rotated windowed signals, batch x N) from the correlators
-// c (batch x n); optionally the three intermediate signals of every member,
+// c (batch x n), one thread per correlator value, which lands at two places:
+// C(t_m) at k = m + n - 1 (m <= n - 2) and -conj(C(t_m)) at k = n - 1 - m
+// (m >= 1).  Optionally the three intermediate signals of every member,
 // stages[(b * 3 + q) * N + k] for q = symmetrised, windowed, rotated.
 __global__ __launch_bounds__(SPEC_THREADS) void k_spec_build(const cplx<double> *__restrict__ c, int64_t n,
                                                             int64_t total, cplx<double> *__restrict__ f,
                                                             cplx<double> *__restrict__ stages)
 {
     const int64_t N = 2 * n - 2;
-    const double pi = 3.14159265358979323846;
     for (int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; i < total; i += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t b = i / N, k = i - b * N;
-        const cplx<double> *cb = c + b * n;
-        cplx<double> s;
-        if (k <= n - 2) {
-            const cplx<double> v = cb[n - 1 - k];
-            s = {-v.re, v.im};
-        } else {
-            s = cb[k - n + 1];
-        }
-        // sin(pi (i-1) / N)^2 with the reference's evaluation order
-        const double sn = sin((pi * (double)k) / (double)N);
-        const double w = sn * sn;
-        const cplx<double> h = {s.re * w, s.im * w};
-        int64_t j = k + N / 2;
-        if (j >= N) j -= N;
-        f[b * N + j] = h;
-        if (stages) {
-            stages[(b * 3 + 0) * N + k] = s;
-            stages[(b * 3 + 1) * N + k] = h;
-            stages[(b * 3 + 2) * N + j] = h;
-        }
+        const int64_t b = i / n, m = i - b * n;
+        const cplx<double> v = c[i];
+        if (m <= n - 2) spec_put(v, m + n - 1, b, N, f, stages);
+        if (m >= 1) spec_put({-v.re, v.im}, n - 1 - m, b, N, f, stages);
     }
 }
 
@@ -149,7 +152,8 @@ static int spectral_functions(const double *signals, int64_t n, int64_t batch, d
     cplx<double> *c = (cplx<double> *)g_spec_signal.ptr, *f = (cplx<double> *)g_spec_work.ptr;
     cplx<double> *st = stages ? (cplx<double> *)g_spec_stages.ptr : nullptr;
     NRQ_HIP(hipMemcpyAsync(c, signals, sizeof(cplx<double>) * (size_t)(n * batch), hipMemcpyHostToDevice, stream()));
-    hipLaunchKernelGGL(k_spec_build, dim3(spec_blocks(total)), dim3(SPEC_THREADS), 0, stream(), c, n, total, f, st);
+    hipLaunchKernelGGL(k_spec_build, dim3(spec_blocks(n * batch)), dim3(SPEC_THREADS), 0, stream(), c, n, n * batch, f,
+                       st);
     NRQ_HIP(hipGetLastError());
     const hipfftResult r = hipfftExecZ2Z(plan, (hipfftDoubleComplex *)f, (hipfftDoubleComplex *)f, HIPFFT_BACKWARD);
     if (r != HIPFFT_SUCCESS) return spec_fft_fail(r, "backward transform");

@@ -4,6 +4,8 @@
     python -m nrqcd_amd.workloads ensemble   # 32 x 48^3 batched ensemble with 4 masses each
     python -m nrqcd_amd.workloads learned    # FGMRES + trained CNN vs GMRES at several step widths
     python -m nrqcd_amd.workloads gluon      # Coulomb gauge fixing + transverse correlators, f(p) at 64^3
+    python -m nrqcd_amd.workloads spectrum   # mode-31 rows at 64^3 (CN step, 8 correlators, norms) and
+                                             # spectral functions of 1024 correlators of 4097 points
 
 Each prints one JSON object.  Inputs are synthetic (the seeded box initial
 condition of gaugeconfiguration_su3.f90:906-951); all timing uses HIP events
@@ -21,8 +23,8 @@ import numpy as np
 from . import (GaugeConfiguration, GetNorm2Momentum_G, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
                NRQ_SOLVER_CHEBYSHEV_MIXED, NRQ_SOLVER_FGMRES_CNN, NRQ_SOLVER_GMRES_FP64, NRQ_SOLVER_GMRES_MIXED,
                PreconditionerNet, event_elapsed_ms, event_record, lattice_finalize, lattice_init, leapfrog_batch,
-               nrqcd_finalize, nrqcd_init, nrqcd_set_preconditioner, nrqcd_set_solver, random_init, synchronize,
-               update_batch)
+               nrqcd_finalize, nrqcd_init, nrqcd_set_preconditioner, nrqcd_set_solver, random_init, spectral_functions,
+               synchronize, update_batch)
 
 SPACINGS = (0.05, 1.0, 1.0, 1.0)
 C1 = [1, 0, 0, 0]
@@ -247,9 +249,66 @@ def gluon(size: int = 64, steps: int = 20, seed: int = 2, iterations: int = 200,
     return out
 
 
+def spectrum(size: int = 64, steps: int = 8, seed: int = 2, reps: int = 10, batch: int = 1024,
+             points: int = 4097) -> dict:
+    """The measurement loop of the reference's mode 31 (main.f90:586-619) at
+    `size`^3: per row a CN step and a leapfrog step, then the eight
+    zero-momentum correlators and the two norms (timed against the same rows
+    without them); and mode 4's spectral
+    function (main.f90:1865-2100) for `batch` correlators of `points` time
+    points at once (host arrays in and out, so the time includes the PCIe
+    copies)."""
+    lattice_init((size,) * 3, SPACINGS)
+    random_init(seed)
+    nrqcd_init(2.0, C1, 1.0, 1e-10)
+    nrqcd_set_solver(NRQ_SOLVER_CHEBYSHEV_MIXED, 30, 10000)
+    g = GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    h = NRQCDField()
+    h.InitSinglePoint(1, 1)
+    for _ in range(steps):
+        h.Update(g, 2.0, C1)
+        g.Update(1.0, 1)
+    h.correlators()
+
+    def rows(observe: bool):
+        for _ in range(reps):
+            h.Update(g, 2.0, C1)
+            g.Update(1.0, 1)
+            if observe:  # what the driver writes per row: 8 correlators, 2 norms
+                h.correlators()
+                h.correlators(True)
+                h.norms()
+
+    ms_row = _timed(lambda: rows(True)) / reps
+    ms_evolve = _timed(lambda: rows(False)) / reps
+    c = h.correlators()
+    cg = h.correlators(True)
+    del h, g
+    nrqcd_finalize()
+    lattice_finalize()
+    rng = np.random.default_rng(seed)
+    t = np.arange(points) * SPACINGS[0]
+    e = rng.uniform(0.5, 4.0, size=(batch, 1))
+    sig = 1j * np.exp(-1j * e * t[None, :]) * np.exp(-0.01 * t[None, :])
+    spectral_functions(sig, SPACINGS[0])
+    t0 = time.perf_counter()
+    spec = spectral_functions(sig, SPACINGS[0])
+    ms_spec = 1e3 * (time.perf_counter() - t0)
+    peak = np.argmax(spec, axis=1) * 2 * np.pi / (SPACINGS[0] * spec.shape[1])
+    return {
+        "workload": f"{size}^3 box IC Qs=0.5 seed {seed}, point source, {steps} CN + leapfrog steps, "
+                    f"mixed Chebyshev CN; {batch} correlators of {points} points",
+        "row_ms": ms_row, "cn_and_leapfrog_ms": ms_evolve, "observables_ms": ms_row - ms_evolve,
+        "O1_1S0": [c["O1_1S0"].real, c["O1_1S0"].imag], "O1_1S0_gconjg": [cg["O1_1S0"].real, cg["O1_1S0"].imag],
+        "spectra_ms_with_pcie": ms_spec,
+        "spectra_peak_max_error": float(np.max(np.abs(peak - e[:, 0]))),
+    }
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon"))
+    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon", "spectrum"))
     ap.add_argument("--size", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--members", type=int, default=32)
@@ -262,6 +321,8 @@ def main(argv=None) -> None:
         out = learned(**{k: v for k, v in kw.items() if k in ("size", "cn_steps")})
     elif a.workload == "gluon":
         out = gluon(**{k: v for k, v in kw.items() if k in ("size", "steps")})
+    elif a.workload == "spectrum":
+        out = spectrum(**{k: v for k, v in kw.items() if k in ("size", "steps")})
     else:
         out = ensemble(members=a.members, **kw)
     print(json.dumps(out), flush=True)

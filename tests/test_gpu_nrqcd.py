@@ -98,6 +98,60 @@ def test_cn_general_propagator_and_anisotropy(nrq, oracle, cn):
     assert np.max(np.abs(Ag - A)) < 1e-10 * np.max(np.abs(A))
 
 
+def _observables(h):
+    return (h.correlators(), h.correlators(True), h.norms())
+
+
+def _oracle_observables(oracle, Q, A):
+    return (oracle.correlators(Q, A), oracle.correlators(Q, A, True), (oracle.prop_norm(Q), oracle.prop_norm(A)))
+
+
+def _same(got, want, rel=1e-11):
+    for g, w in zip(got[:2], want[:2]):
+        for k in w:
+            assert abs(g[k] - w[k]) <= rel * max(1.0, abs(w[k])), k
+    assert got[2][0] == pytest.approx(want[2][0], rel=rel, abs=1e-300)
+    assert got[2][1] == pytest.approx(want[2][1], rel=rel, abs=1e-300)
+
+
+def test_observables_follow_every_change(nrq, oracle, cn):
+    """The library evaluates the eight correlators and both norms in one pass
+    and reuses them until the field changes: after creation, point sources,
+    uploads, copies and CN steps the values must be those of the new field."""
+    ext = (6, 5, 4)
+    _say("observables after every change 6x5x4")
+    cn(ext, SP)
+    oracle.lattice_init(ext, SP)
+    U, E = oracle.box_init(4, 0.5, 1.0, 1.0)
+    g = nrq.GaugeConfiguration()
+    g.upload(U, E)
+    h = nrq.NRQCDField()
+    z = _observables(h)
+    assert z[2] == (0.0, 0.0) and all(v == 0 for v in z[0].values())
+    h.InitSinglePoint(1, 1)
+    Q, A = oracle.point_prop(1), oracle.point_prop(1)
+    _same(_observables(h), _oracle_observables(oracle, Q, A))
+    h.Update(g, 2.0, C1)
+    rc, _ = oracle.cn_update(U, Q, A, 2.0, C1, 1.0, 1e-12)
+    assert rc == 0
+    _same(_observables(h), _oracle_observables(oracle, Q, A), rel=1e-10)
+    c = h.copy()
+    _same(_observables(c), _observables(h), rel=0)
+    c.Update(g, 2.0, C1)
+    rc, _ = oracle.cn_update(U, Q, A, 2.0, C1, 1.0, 1e-12)
+    _same(_observables(c), _oracle_observables(oracle, Q, A), rel=1e-10)
+    assert abs(c.correlators()["O1_1S0"] - h.correlators()["O1_1S0"]) > 1e-6
+    rng = np.random.default_rng(8)
+    v = oracle.vol
+    Qr = np.asfortranarray(rng.normal(size=(6, 6, v)) + 1j * rng.normal(size=(6, 6, v)))
+    h.upload(Qr, None)
+    _same(_observables(h), _oracle_observables(oracle, Qr, h.download()[1]), rel=1e-12)
+    h.InitSinglePointSingleDoF(1, 2, 3, 2, 1, 3)
+    Qd, Ad = h.download()
+    assert np.count_nonzero(Qd) == 1 and np.count_nonzero(Ad) == 1
+    _same(_observables(h), _oracle_observables(oracle, Qd, Ad), rel=1e-12)
+
+
 def test_cn_rejects_higher_wilson_terms(nrq, cn):
     cn((4, 4, 4), SP)
     g = nrq.GaugeConfiguration()
