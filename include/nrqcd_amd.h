@@ -115,6 +115,21 @@ int nrq_gauge_cold(nrq_gauge *h);
  * Amplitude, Coupling), gaugeconfiguration_su3.f90:906-951 and 1263-1446.
  * Draws from the library's ranlxd stream (nrq_random_init). */
 int nrq_gauge_box_ic(nrq_gauge *h, double saturation_scale, double amplitude, double coupling);
+/* The same with its optional outputs aa_correlator, ee_correlator
+ * (gaugeconfiguration_su3.f90:906-951 -> 1390-1421): per momentum index
+ * (lattice index - 1) sum_a Re[X_i P_ij conj(X_j)] / V / 8 of the
+ * momentum-space fields X = A, E with the real transverse projector P and the
+ * physical volume V; either pointer may be NULL. */
+int nrq_gauge_box_ic_correlators(nrq_gauge *h, double saturation_scale, double amplitude, double coupling, double *aa,
+                                 double *ee);
+/* TransversePolarisedOccupiedInit(Occupation, [aa], [ee]),
+ * gaugeconfiguration_su3.f90:1263-1446 (specific_sender = 0), or
+ * TransversePolarisedOccupiedInit_specificSender, :959-1255 (specific_sender
+ * = 1: complex amplitudes drawn in momentum space in lattice order, then made
+ * Hermitian), for any occupation[V] >= 0 by lattice index.  Both are private
+ * in the reference, which reaches the first through the box form.  Fails
+ * when the random module is not initialised, as the reference does. */
+int nrq_gauge_transverse_init(nrq_gauge *h, const double *occupation, int specific_sender, double *aa, double *ee);
 /* Upload / download in the reference layouts links(3,3,3,V) complex128 and
  * efield(8,3,V) real64 (gaugeconfiguration_su3.f90:35-37).  Either pointer may
  * be NULL to skip that field. */

@@ -156,6 +156,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_heavy_correlators": (I, [P, P]),
         "nrq_heavy_correlators_gconjg": (I, [P, P]),
         "nrq_spectral_functions": (I, [P, I64, I64, D, P, P]),
+        "nrq_gauge_box_ic_correlators": (I, [P, D, D, D, P, P]),
+        "nrq_gauge_transverse_init": (I, [P, P, I, P, P]),
         "nrq_t2w": (I, [P, I64, D]),
         "nrq_w2t": (I, [P, I64, D]),
     }
@@ -320,8 +322,35 @@ class GaugeConfiguration:
         _check(library().nrq_gauge_cold(self._h))
 
     def TransversePolarisedOccupiedInit_Box(self, SaturationScale: float, Amplitude: float,
-                                            Coupling: float) -> None:
-        _check(library().nrq_gauge_box_ic(self._h, SaturationScale, Amplitude, Coupling))
+                                            Coupling: float, correlators: bool = False):
+        """gaugeconfiguration_su3.f90:906-951; with correlators, also returns the
+        momentum-space (aa, ee) per momentum index (its optional outputs)."""
+        if not correlators:
+            _check(library().nrq_gauge_box_ic(self._h, SaturationScale, Amplitude, Coupling))
+            return None
+        v = lattice_size()
+        aa, ee = np.zeros(v), np.zeros(v)
+        _check(library().nrq_gauge_box_ic_correlators(self._h, SaturationScale, Amplitude, Coupling, _ptr(aa),
+                                                      _ptr(ee)))
+        return aa, ee
+
+    def _transverse_init(self, Occupation, sender: int, correlators: bool):
+        v = lattice_size()
+        occ = np.ascontiguousarray(Occupation, np.float64)
+        if occ.size != v:
+            raise ValueError(f"occupation has {occ.size} entries, the lattice {v}")
+        aa = np.zeros(v) if correlators else None
+        ee = np.zeros(v) if correlators else None
+        _check(library().nrq_gauge_transverse_init(self._h, _ptr(occ), sender, _ptr(aa), _ptr(ee)))
+        return (aa, ee) if correlators else None
+
+    def TransversePolarisedOccupiedInit(self, Occupation, correlators: bool = False):
+        """gaugeconfiguration_su3.f90:1263-1446 for any occupation by lattice index."""
+        return self._transverse_init(Occupation, 0, correlators)
+
+    def TransversePolarisedOccupiedInit_specificSender(self, Occupation, correlators: bool = False):
+        """gaugeconfiguration_su3.f90:959-1255: amplitudes drawn in momentum space."""
+        return self._transverse_init(Occupation, 1, correlators)
 
     # -- state exchange ----------------------------------------------------
     def upload(self, links: np.ndarray | None = None, efield: np.ndarray | None = None) -> None:

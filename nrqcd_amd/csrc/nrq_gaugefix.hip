@@ -485,6 +485,24 @@ static int gaugefix_t(nrq_gauge *h, double tol, double alpha, int64_t maxit, int
     return NRQ_OK;
 }
 
+int transverse_project(const cplx<double> *fld, double scale, double *out)
+{
+    const geom g = lat().g;
+    const int64_t v = g.vol;
+    double *dout = nullptr;
+    if (hipMalloc((void **)&dout, sizeof(double) * (size_t)v) != hipSuccess) {
+        (void)hipGetLastError();
+        return fail(NRQ_ERR_ALLOC, "correlators: device allocation failed (V=%lld)", (long long)v);
+    }
+    const unsigned nblk = (unsigned)((v + GF_THREADS - 1) / GF_THREADS);
+    hipLaunchKernelGGL(k_corr_project, dim3(nblk), dim3(GF_THREADS), 0, stream(), fld, g, scale, dout);
+    hipError_t e = hipGetLastError();
+    if (e == hipSuccess) e = hipMemcpyAsync(out, dout, sizeof(double) * (size_t)v, hipMemcpyDeviceToHost, stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(stream());
+    (void)hipFree(dout);
+    return e == hipSuccess ? NRQ_OK : check_hip(e, "transverse projection");
+}
+
 template <typename R> static int correlators_t(const nrq_gauge *h, double *aa, double *ee)
 {
     const geom g = lat().g;

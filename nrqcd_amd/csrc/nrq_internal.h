@@ -41,6 +41,12 @@ bool rng_ready();
 constexpr int REDUCE_BLOCKS = 1024;
 constexpr int REDUCE_THREADS = 256;
 
+// Per momentum index s: scale * sum_a Re[sum_ij F_i P_ij conj(F_j)] of the 24
+// momentum-space fields fld[(a + 8 i) V + s] (device) with the real transverse
+// projector of the backward-derivative momentum, into host out[V]
+// (gaugeconfiguration_su3.f90:1954-1972).
+int transverse_project(const cplx<double> *fld, double scale, double *out);
+
 struct scratch_buf {
     void *ptr = nullptr;
     size_t bytes = 0;

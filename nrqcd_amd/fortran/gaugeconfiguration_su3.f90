@@ -40,6 +40,8 @@ module gaugeconfiguration_su3
      procedure, public :: Reunitarise
      procedure, public :: SetReunitarisationInterval
      procedure, public :: CoulombGaugefixing
+     procedure, public :: TransversePolarisedOccupiedInit
+     procedure, public :: TransversePolarisedOccupiedInit_specificSender
      procedure, public :: GetTransverseAACorrelator
      procedure, public :: GetTransverseEECorrelator
      procedure, private :: Update_Leapfrog
@@ -109,15 +111,81 @@ contains
     call nrq_check(nrq_gauge_cold(GaugeConf%handle), 'ColdInit')
   end subroutine ColdInit
 
-  !> TransversePolarisedOccupiedInit_Box(SaturationScale, Amplitude, Coupling),
-  !! gaugeconfiguration_su3.f90:906-951 (draws from the random module's stream)
-  subroutine TransversePolarisedOccupiedInit_Box(GaugeConf, SaturationScale, Amplitude, Coupling)
+  !> TransversePolarisedOccupiedInit_Box(SaturationScale, Amplitude, Coupling,
+  !! [aa_correlator], [ee_correlator]), gaugeconfiguration_su3.f90:906-951
+  !! (draws from the random module's stream); the optional outputs are the
+  !! transverse correlators of the momentum-space fields, one per momentum index.
+  subroutine TransversePolarisedOccupiedInit_Box(GaugeConf, SaturationScale, Amplitude, Coupling, &
+       aa_correlator, ee_correlator)
+    use lattice, only: GetLocalLatticeSize
     class(GaugeConfiguration), intent(inout) :: GaugeConf
     real(fp), intent(in) :: SaturationScale, Amplitude, Coupling
+    real(fp), optional, allocatable, target, intent(out) :: aa_correlator(:), ee_correlator(:)
+    type(c_ptr) :: pa, pe
     call GaugeConf%Allocate
-    call nrq_check(nrq_gauge_box_ic(GaugeConf%handle, SaturationScale, Amplitude, Coupling), &
+    pa = c_null_ptr
+    pe = c_null_ptr
+    if (present(aa_correlator)) then
+       allocate(aa_correlator(GetLocalLatticeSize()))
+       pa = c_loc(aa_correlator)
+    end if
+    if (present(ee_correlator)) then
+       allocate(ee_correlator(GetLocalLatticeSize()))
+       pe = c_loc(ee_correlator)
+    end if
+    call nrq_check(nrq_gauge_box_ic_correlators(GaugeConf%handle, SaturationScale, Amplitude, Coupling, pa, pe), &
          'TransversePolarisedOccupiedInit_Box')
   end subroutine TransversePolarisedOccupiedInit_Box
+
+  !> TransversePolarisedOccupiedInit(Occupation, [aa_correlator],
+  !! [ee_correlator]), gaugeconfiguration_su3.f90:1263-1446: any occupation
+  !! per lattice index (private in the reference, behind the box form)
+  subroutine TransversePolarisedOccupiedInit(GaugeConf, Occupation, aa_correlator, ee_correlator)
+    class(GaugeConfiguration), intent(inout) :: GaugeConf
+    real(fp), intent(in), target, contiguous :: Occupation(:)
+    real(fp), optional, allocatable, target, intent(out) :: aa_correlator(:), ee_correlator(:)
+    call transverse_init(GaugeConf, Occupation, 0_c_int, 'TransversePolarisedOccupiedInit', aa_correlator, ee_correlator)
+  end subroutine TransversePolarisedOccupiedInit
+
+  !> TransversePolarisedOccupiedInit_specificSender(Occupation,
+  !! [aa_correlator], [ee_correlator]), gaugeconfiguration_su3.f90:959-1255:
+  !! amplitudes drawn in momentum space in lattice order, independent of the
+  !! number of processes, then made Hermitian (private in the reference)
+  subroutine TransversePolarisedOccupiedInit_specificSender(GaugeConf, Occupation, aa_correlator, ee_correlator)
+    class(GaugeConfiguration), intent(inout) :: GaugeConf
+    real(fp), intent(in), target, contiguous :: Occupation(:)
+    real(fp), optional, allocatable, target, intent(out) :: aa_correlator(:), ee_correlator(:)
+    call transverse_init(GaugeConf, Occupation, 1_c_int, 'TransversePolarisedOccupiedInit_specificSender', &
+         aa_correlator, ee_correlator)
+  end subroutine TransversePolarisedOccupiedInit_specificSender
+
+  subroutine transverse_init(GaugeConf, Occupation, sender, what, aa_correlator, ee_correlator)
+    use lattice, only: GetLocalLatticeSize
+    class(GaugeConfiguration), intent(inout) :: GaugeConf
+    real(fp), intent(in), target, contiguous :: Occupation(:)
+    integer(c_int), intent(in) :: sender
+    character(len=*), intent(in) :: what
+    real(fp), optional, allocatable, target, intent(out) :: aa_correlator(:), ee_correlator(:)
+    type(c_ptr) :: pa, pe
+    if (size(Occupation, kind=int64) /= GetLocalLatticeSize()) then
+       write(*, '(A,A)') 'Error in ', what
+       write(*, '(A,I0,A,I0)') 'occupation has ', size(Occupation, kind=int64), ' entries, the lattice ', &
+            GetLocalLatticeSize()
+       error stop 1
+    end if
+    call GaugeConf%Allocate
+    pa = c_null_ptr
+    pe = c_null_ptr
+    if (present(aa_correlator)) then
+       allocate(aa_correlator(GetLocalLatticeSize()))
+       pa = c_loc(aa_correlator)
+    end if
+    if (present(ee_correlator)) then
+       allocate(ee_correlator(GetLocalLatticeSize()))
+       pe = c_loc(ee_correlator)
+    end if
+    call nrq_check(nrq_gauge_transverse_init(GaugeConf%handle, c_loc(Occupation), sender, pa, pe), what)
+  end subroutine transverse_init
 
   !> GetLink_G(i, LatticeIndex), gaugeconfiguration_su3.f90:160-196
   function GetLink_G(GaugeConf, i, LatticeIndex)

@@ -93,6 +93,18 @@ module nrq_capi
        type(c_ptr), value :: h
        real(c_double), value :: qs, amp, g
      end function nrq_gauge_box_ic
+     integer(c_int) function nrq_gauge_box_ic_correlators(h, qs, amp, g, aa, ee) &
+          bind(C, name='nrq_gauge_box_ic_correlators')
+       import :: c_int, c_ptr, c_double
+       type(c_ptr), value :: h, aa, ee
+       real(c_double), value :: qs, amp, g
+     end function nrq_gauge_box_ic_correlators
+     integer(c_int) function nrq_gauge_transverse_init(h, occupation, specific_sender, aa, ee) &
+          bind(C, name='nrq_gauge_transverse_init')
+       import :: c_int, c_ptr
+       type(c_ptr), value :: h, occupation, aa, ee
+       integer(c_int), value :: specific_sender
+     end function nrq_gauge_transverse_init
      integer(c_int) function nrq_gauge_upload(h, links, efield) bind(C, name='nrq_gauge_upload')
        import :: c_int, c_ptr
        type(c_ptr), value :: h, links, efield
