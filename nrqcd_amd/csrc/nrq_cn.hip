@@ -408,9 +408,8 @@ static int heavy_update_n(nrq_heavy *const *hs, const double *mass, int n, const
     if (!gf || !gf->u[0]) return fail(NRQ_ERR_ARG, "heavy: null gauge configuration");
     if (!c) return fail(NRQ_ERR_ARG, "heavy: null Wilson coefficients");
     if (!std::isfinite(stepwidth)) return fail(NRQ_ERR_ARG, "heavy: bad step width");
-    for (int k = 1; k < 4; ++k)
-        if (c[2 * k] != 0.0 || c[2 * k + 1] != 0.0)
-            return fail(NRQ_ERR_UNSUPPORTED, "heavy: only the kinetic term c1 is implemented (c%d != 0)", k + 1);
+    bool full = false; // any of c2..c4: the full Hamiltonian (nrq_hamiltonian.hip)
+    for (int k = 1; k < 4; ++k) full = full || c[2 * k] != 0.0 || c[2 * k + 1] != 0.0;
     const double a0 = lat().g.a[0];
     const std::complex<double> c1(c[0], c[1]);
     const std::complex<double> mi(0.0, -(stepwidth / 2.0 * a0));
@@ -418,6 +417,24 @@ static int heavy_update_n(nrq_heavy *const *hs, const double *mass, int n, const
     // stays zero, as KSPSolve returns x = 0 for b = 0).  rhs = S psi with
     // S = 1 + alpha K (SystMat_herm before the in-place Hermitian
     // transpose), then S^+ psi' = rhs with S^+ = 1 + conj(alpha) K.
+    if (full) {
+        for (int k = 0; k < n; ++k) {
+            const nrq_heavy *h = hs[k];
+            NEED_HEAVY(h);
+            if (gf->vol != h->vol) return fail(NRQ_ERR_ARG, "heavy: gauge and heavy field lattices differ");
+            if (!std::isfinite(mass[k]) || mass[k] == 0.0) return fail(NRQ_ERR_ARG, "heavy: bad mass %g", mass[k]);
+            for (int j = 0; j < k; ++j)
+                if (hs[j] == h) return fail(NRQ_ERR_ARG, "heavy: field %d listed twice", k);
+        }
+        for (int k = 0; k < n; ++k) ++hs[k]->version; // the solve overwrites the live columns
+        nrq_solve_stats st = {0, 0, 0, 0.0};
+        const int status = ham_cn_update(hs, mass, n, gf, c, stepwidth, g_cn.solver.restart, g_cn.solver.maxit,
+                                         g_cn.solver.tol, &st);
+        if (status != NRQ_OK && status != NRQ_ERR_NOCONV) return status;
+        NRQ_HIP(hipStreamSynchronize(stream()));
+        if (stats) *stats = st;
+        return status;
+    }
     cplx<double> *fields[NSLOTS];
     cplx<double> alpha[NSLOTS], beta[NSLOTS];
     int R = 0;

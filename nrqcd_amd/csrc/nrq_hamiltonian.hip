@@ -1,0 +1,615 @@
+// nrq_hamiltonian.hip -- the Crank-Nicolson step with the full NRQCD
+// Hamiltonian of BuildSystemMatrix (nrqcd.f90:1175-1387): the kinetic c1 term
+// plus the c2 (magnetic), c3 (E derivative) and c4 (spin-orbit) terms, taken
+// when any of c2..c4 is non-zero.
+//
+// S = 1 + sum_t a_t T_t with (dt the step width, pre = -i dt a0 / 2)
+//   T_1 psi = sum_i [psi(x) - (U_i(x) psi(x+i) + U_i(x-i)^+ psi(x-i))/2] / a_i^2,
+//            a_1 = pre c1 / M;
+//   T_2 = C2CS(Bsum(x)),  Bsum = sum_i (F_jk - F_kj)/2 (nrqcd.f90:1669-1693; no
+//            Pauli matrix, a reference quirk), a_2 = pre (-c2) / (2M);
+//   T_3 = C2CS(Dsum(x)),  Dsum = sum_i (U_i(x) e_i(x+i) U_i(x)^+
+//            - U_i(x-i)^+ e_i(x-i) U_i(x-i)) / (2 a_i), e = -F_0i,
+//            a_3 = pre (-c3) / (8 M^2);
+//   T_4 hops to x +- i with Kron(G, sigma_k) for the cyclic (eps = +1) and
+//            anticyclic (eps = -1) (i, j, k):
+//            to x + i: G = eps (e_j(x) U_i(x) + U_i(x) e_j(x+i)) / (2 a_i),
+//            to x - i: G = -eps (e_j(x) W + W e_j(x-i)) / (2 a_i), W = U_i(x-i)^+,
+//            a_4 = pre (-c4) i / (8 M^2).
+// The field strength F_ij = Re[ArgLogU(P_ij)] / (a_i a_j) is the element-wise
+// real part (gaugeconfiguration_su3.f90:1563-1581) with the spatial plaquette
+// or, for F_0j, the temporal plaquette exp(i E_j^a T^a) (:1656-1679).  The
+// antiquark uses (-M, conj c) (nrqcd.f90:1130-1161).
+//
+// Per step: rhs = S psi, then S^+ psi' = rhs (S^+ = 1 + sum_t conj(a_t) T_t^+)
+// for every column of both propagators.  c4 mixes the spin rows, so a column
+// is the pair of colour-vector fields (2j, 2j+1), 6V complex doubles,
+// contiguous in the field storage.  The solver is FP64 GMRES(m) with
+// classical Gram-Schmidt and zero initial guess (PETSc's defaults,
+// nrqcd.f90:803-804) that stops on the FP64 true relative residual.
+//
+// The field strengths (kernel k_ham_fields: 3 temporal and 6 spatial
+// logarithms per site) and the colour matrices Dsum and G (k_ham_blocks) are
+// built once per step; k_ham_apply applies S or S^+ to several columns at
+// once.
+#include "nrq_internal.h"
+#include "nrq_grouplog.h"
+#include <algorithm>
+#include <cmath>
+#include <complex>
+#include <vector>
+
+namespace nrq {
+
+const void *gauge_links(const nrq_gauge *h);
+
+constexpr int HAM_THREADS = 256;
+constexpr int HAM_FLD = 36;   // doubles per site: F0_j (3 x 9), Bsum (9)
+constexpr int HAM_BLK = 13;   // colour matrices per site: Dsum, G[i][sign][p]
+constexpr int HAM_MAXCOL = 12; // columns per apply launch (6 per propagator)
+constexpr int HAM_DOT_BLOCKS = 512;
+
+template <typename R> __device__ __forceinline__ mat3<double> ham_link(const cplx<R> *u, int64_t vol, int dir, int64_t s)
+{
+    mat3<double> m;
+    const cplx<R> *p = u + (int64_t)dir * 9 * vol;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) {
+        const cplx<R> a = p[(int64_t)e * vol + s];
+        m.m[e] = {(double)a.re, (double)a.im};
+    }
+    return m;
+}
+
+__device__ __forceinline__ mat3<double> madd(const mat3<double> &a, const mat3<double> &b, double sb)
+{
+    mat3<double> r;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) r.m[e] = {a.m[e].re + sb * b.m[e].re, a.m[e].im + sb * b.m[e].im};
+    return r;
+}
+
+// F0_j(x) and Bsum(x) into fld[(q * 9 + e) V + s], q = 0..2 for F0_j, 3 for Bsum.
+template <typename R>
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_fields(const cplx<R> *__restrict__ u, const R *__restrict__ ef, geom g,
+                                                           double *__restrict__ fld)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.vol) return;
+    const int64_t V = g.vol;
+    for (int j = 0; j < 3; ++j) {
+        double al[8];
+        for (int a = 0; a < 8; ++a) al[a] = (double)ef[((int64_t)j * 8 + a) * V + s];
+        const mat3<double> H = group_log_matrix(exp_i_hermitian(algebra_matrix(al)));
+        const double sc = 1.0 / (g.a[0] * g.a[j + 1]);
+        for (int e = 0; e < 9; ++e) fld[((int64_t)j * 9 + e) * V + s] = H.m[e].re * sc;
+    }
+    const lsite p = make_site(g, s);
+    double b[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
+    for (int i = 0; i < 3; ++i) {
+        const int j = (i + 1) % 3, k = (i + 2) % 3;
+        for (int o = 0; o < 2; ++o) {
+            // P_mn(x) = U_m(x) U_n(x+m) U_m(x+n)^+ U_n(x)^+ for (m, n) = (j, k), then (k, j)
+            const int m = o == 0 ? j : k, n = o == 0 ? k : j;
+            const mat3<double> P = mul_nd(mul_nd(mul(ham_link(u, V, m, s), ham_link(u, V, n, site_up(g, p, m).s)),
+                                                 ham_link(u, V, m, site_up(g, p, n).s)),
+                                          ham_link(u, V, n, s));
+            const mat3<double> H = group_log_matrix(P);
+            const double sc = (o == 0 ? 0.5 : -0.5) / (g.a[m + 1] * g.a[n + 1]);
+            for (int e = 0; e < 9; ++e) b[e] += H.m[e].re * sc;
+        }
+    }
+    for (int e = 0; e < 9; ++e) fld[((int64_t)27 + e) * V + s] = b[e];
+}
+
+__device__ __forceinline__ mat3<double> ham_e(const double *__restrict__ fld, int64_t V, int j, int64_t s)
+{
+    mat3<double> m;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) m.m[e] = {-fld[((int64_t)j * 9 + e) * V + s], 0.0};
+    return m;
+}
+
+// Dsum (matrix 0) and G[i][sign][p] (matrix 1 + (2 i + sign) 2 + p) into
+// blk[(mat * 9 + e) V + s].
+template <typename R>
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_blocks(const cplx<R> *__restrict__ u, const double *__restrict__ fld,
+                                                           geom g, cplx<double> *__restrict__ blk)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.vol) return;
+    const int64_t V = g.vol;
+    const lsite p = make_site(g, s);
+    mat3<double> dsum;
+    set_zero(dsum);
+    for (int i = 0; i < 3; ++i) {
+        const int64_t sp = site_up(g, p, i).s, sm = site_dn(g, p, i).s;
+        const double inv2a = 1.0 / (2.0 * g.a[i + 1]);
+        const mat3<double> Ui = ham_link(u, V, i, s), Um = ham_link(u, V, i, sm);
+        // c3: U e_i(x+i) U^+ - Um^+ e_i(x-i) Um
+        const mat3<double> t1 = mul_nd(mul(Ui, ham_e(fld, V, i, sp)), Ui);
+        const mat3<double> t2 = mul(mul_dn(Um, ham_e(fld, V, i, sm)), Um);
+        dsum = madd(dsum, madd(t1, t2, -1.0), inv2a);
+        // c4
+        mat3<double> W;
+        for (int e = 0; e < 9; ++e) W.m[e] = {Um.m[3 * (e % 3) + e / 3].re, -Um.m[3 * (e % 3) + e / 3].im};
+        for (int q = 0; q < 2; ++q) {
+            const int j = q == 0 ? (i + 1) % 3 : (i + 2) % 3;
+            const double eps = q == 0 ? 1.0 : -1.0;
+            const mat3<double> ex = ham_e(fld, V, j, s);
+            const mat3<double> gp = madd(mul(ex, Ui), mul(Ui, ham_e(fld, V, j, sp)), 1.0);
+            const mat3<double> gm = madd(mul(ex, W), mul(W, ham_e(fld, V, j, sm)), 1.0);
+            const int mp = 1 + (2 * i + 0) * 2 + q, mm = 1 + (2 * i + 1) * 2 + q;
+            for (int e = 0; e < 9; ++e) {
+                blk[((int64_t)mp * 9 + e) * V + s] = {eps * inv2a * gp.m[e].re, eps * inv2a * gp.m[e].im};
+                blk[((int64_t)mm * 9 + e) * V + s] = {-eps * inv2a * gm.m[e].re, -eps * inv2a * gm.m[e].im};
+            }
+        }
+    }
+    for (int e = 0; e < 9; ++e) blk[(int64_t)e * V + s] = dsum.m[e];
+}
+
+struct ham_cols {
+    const cplx<double> *in[HAM_MAXCOL];
+    cplx<double> *out[HAM_MAXCOL];
+    cplx<double> a[HAM_MAXCOL][4];
+};
+
+// Pauli matrix entry sigma_k[r][c] of the reference's table (su2.f90:68-88):
+// k = 0: [[0,1],[1,0]], k = 1: [[0,i],[-i,0]], k = 2: diag(1,-1).
+__device__ __forceinline__ cplx<double> ham_sigma(int k, int r, int c)
+{
+    if (k == 0) return {r != c ? 1.0 : 0.0, 0.0};
+    if (k == 1) return {0.0, r == c ? 0.0 : (r == 0 ? 1.0 : -1.0)};
+    return {r == c ? (r == 0 ? 1.0 : -1.0) : 0.0, 0.0};
+}
+
+__device__ __forceinline__ void ham_load(const cplx<double> *__restrict__ x, int64_t V, int64_t s, cplx<double> (&v)[2][3])
+{
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[sp][c] = x[((int64_t)sp * 3 + c) * V + s];
+}
+
+__device__ __forceinline__ mat3<double> ham_blk(const cplx<double> *__restrict__ blk, int64_t V, int m, int64_t s)
+{
+    mat3<double> r;
+#pragma unroll
+    for (int e = 0; e < 9; ++e) r.m[e] = blk[((int64_t)m * 9 + e) * V + s];
+    return r;
+}
+
+// acc[sp] += M v[sp] (ADJ: M^+ v[sp])
+template <bool ADJ>
+__device__ __forceinline__ void ham_colour(const mat3<double> &M, const cplx<double> (&v)[2][3], cplx<double> (&acc)[2][3])
+{
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+        for (int r = 0; r < 3; ++r)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                if (ADJ) mac_cn(acc[sp][r], M.m[3 * c + r], v[sp][c]);
+                else mac(acc[sp][r], M.m[3 * r + c], v[sp][c]);
+            }
+}
+
+// acc += Kron(G, sigma_k) v (ADJ: Kron(G^+, sigma_k) v; sigma is Hermitian)
+template <bool ADJ>
+__device__ __forceinline__ void ham_kron(const mat3<double> &G, int k, const cplx<double> (&v)[2][3], cplx<double> (&acc)[2][3])
+{
+    cplx<double> w[2][3];
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cplx<double> t = {0.0, 0.0};
+#pragma unroll
+            for (int q = 0; q < 2; ++q) mac(t, ham_sigma(k, sp, q), v[q][c]);
+            w[sp][c] = t;
+        }
+    ham_colour<ADJ>(G, w, acc);
+}
+
+// y = S x (ADJ = false) or S^+ x for the columns of `cols` (blockIdx.y).
+template <typename R, bool ADJ>
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_apply(const cplx<R> *__restrict__ u, const double *__restrict__ fld,
+                                                          const cplx<double> *__restrict__ blk, geom g, ham_cols cols)
+{
+    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (s >= g.vol) return;
+    const int col = blockIdx.y;
+    const int64_t V = g.vol;
+    const cplx<double> *x = cols.in[col];
+    cplx<double> a[4];
+#pragma unroll
+    for (int t = 0; t < 4; ++t) a[t] = ADJ ? cplx<double>{cols.a[col][t].re, -cols.a[col][t].im} : cols.a[col][t];
+    const lsite p = make_site(g, s);
+    cplx<double> v0[2][3], kin[2][3], c2[2][3], c3[2][3], c4[2][3];
+    ham_load(x, V, s, v0);
+#pragma unroll
+    for (int sp = 0; sp < 2; ++sp)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) kin[sp][c] = c2[sp][c] = c3[sp][c] = c4[sp][c] = {0.0, 0.0};
+    // c2, c3 (site-diagonal colour matrices)
+    {
+        mat3<double> B;
+        for (int e = 0; e < 9; ++e) B.m[e] = {fld[((int64_t)27 + e) * V + s], 0.0};
+        ham_colour<ADJ>(B, v0, c2);
+        ham_colour<ADJ>(ham_blk(blk, V, 0, s), v0, c3);
+    }
+    for (int i = 0; i < 3; ++i) {
+        const int64_t sp_ = site_up(g, p, i).s, sm = site_dn(g, p, i).s;
+        const double w2 = 1.0 / (g.a[i + 1] * g.a[i + 1]);
+        cplx<double> vp[2][3], vm[2][3], h[2][3];
+        ham_load(x, V, sp_, vp);
+        ham_load(x, V, sm, vm);
+        // kinetic: v0 - (U_i(x) v(x+i) + U_i(x-i)^+ v(x-i)) / 2
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) h[q][c] = {0.0, 0.0};
+        ham_colour<false>(ham_link(u, V, i, s), vp, h);
+        ham_colour<true>(ham_link(u, V, i, sm), vm, h);
+#pragma unroll
+        for (int q = 0; q < 2; ++q)
+#pragma unroll
+            for (int c = 0; c < 3; ++c)
+                kin[q][c] = kin[q][c] + w2 * (v0[q][c] - 0.5 * h[q][c]);
+        // spin-orbit hops
+        for (int q = 0; q < 2; ++q) {
+            const int k = q == 0 ? (i + 2) % 3 : (i + 1) % 3;
+            const int mp = 1 + (2 * i + 0) * 2 + q, mm = 1 + (2 * i + 1) * 2 + q;
+            if (!ADJ) {
+                ham_kron<false>(ham_blk(blk, V, mp, s), k, vp, c4);
+                ham_kron<false>(ham_blk(blk, V, mm, s), k, vm, c4);
+            } else {
+                ham_kron<true>(ham_blk(blk, V, mp, sm), k, vm, c4);
+                ham_kron<true>(ham_blk(blk, V, mm, sp_), k, vp, c4);
+            }
+        }
+    }
+    cplx<double> *y = cols.out[col];
+#pragma unroll
+    for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) {
+            cplx<double> r = v0[q][c];
+            mac(r, a[0], kin[q][c]);
+            mac(r, a[1], c2[q][c]);
+            mac(r, a[2], c3[q][c]);
+            mac(r, a[3], c4[q][c]);
+            y[((int64_t)q * 3 + c) * V + s] = r;
+        }
+}
+
+// ---- vector kernels (n complex doubles) -----------------------------------
+
+__device__ __forceinline__ double ham_wsum(double v)
+{
+#pragma unroll
+    for (int off = 32; off > 0; off >>= 1) v += __shfl_down(v, off, 64);
+    return v;
+}
+
+__device__ __forceinline__ double ham_block_sum(double v, double *sh)
+{
+    v = ham_wsum(v);
+    const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    __syncthreads();
+    if (lane == 0) sh[w] = v;
+    __syncthreads();
+    double t = 0;
+    if (threadIdx.x == 0)
+        for (int k = 0; k < (int)(blockDim.x >> 6); ++k) t += sh[k];
+    return t;
+}
+
+// partial[(2 i + {0: re, 1: im}) * gridDim.x + block] of <V_i, w>, i = blockIdx.y
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_dots(const cplx<double> *__restrict__ basis, int64_t n,
+                                                         const cplx<double> *__restrict__ w, double *__restrict__ partial)
+{
+    const int i = blockIdx.y;
+    const cplx<double> *vi = basis + (int64_t)i * n;
+    cplx<double> acc = {0.0, 0.0};
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        mac_cn(acc, vi[e], w[e]);
+    __shared__ double sh[HAM_THREADS / 64];
+    const double re = ham_block_sum(acc.re, sh);
+    const double im = ham_block_sum(acc.im, sh);
+    if (threadIdx.x == 0) {
+        partial[(int64_t)(2 * i) * gridDim.x + blockIdx.x] = re;
+        partial[(int64_t)(2 * i + 1) * gridDim.x + blockIdx.x] = im;
+    }
+}
+
+// partial[block] = sum |w|^2
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_norm2(const cplx<double> *__restrict__ w, int64_t n,
+                                                          double *__restrict__ partial)
+{
+    double acc = 0;
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        acc += w[e].re * w[e].re + w[e].im * w[e].im;
+    __shared__ double sh[HAM_THREADS / 64];
+    const double t = ham_block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+}
+
+// out[q] = sum_b partial[q * nb + b] for q = blockIdx.x (fixed order)
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_fold(const double *__restrict__ partial, int nb,
+                                                         double *__restrict__ out)
+{
+    const int q = blockIdx.x;
+    double s = 0;
+    for (int b = threadIdx.x; b < nb; b += HAM_THREADS) s += partial[(int64_t)q * nb + b];
+    __shared__ double sh[HAM_THREADS / 64];
+    s = ham_block_sum(s, sh);
+    if (threadIdx.x == 0) out[q] = s;
+}
+
+// w += sign sum_{i < k} c_i V_i (c: complex coefficients on the device)
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_combine(cplx<double> *__restrict__ w, const cplx<double> *__restrict__ basis,
+                                                            int64_t n, const double *__restrict__ c, int k, double sign)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
+        cplx<double> acc = {0.0, 0.0};
+        for (int i = 0; i < k; ++i) mac(acc, cplx<double>{c[2 * i], c[2 * i + 1]}, basis[(int64_t)i * n + e]);
+        w[e] = {w[e].re + sign * acc.re, w[e].im + sign * acc.im};
+    }
+}
+
+// y = a x + b z (real a, b)
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_axpby(cplx<double> *__restrict__ y, const cplx<double> *__restrict__ x,
+                                                          double a, const cplx<double> *__restrict__ z, double b, int64_t n)
+{
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        y[e] = {a * x[e].re + b * z[e].re, a * x[e].im + b * z[e].im};
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+
+struct ham_workspace {
+    scratch_buf fld, blk, basis, vec, red, small;
+    double *host = nullptr; // pinned: 2 (m + 2) doubles, kept until the process ends
+    int host_m = -1;
+};
+static ham_workspace g_ham;
+
+static unsigned ham_blocks(int64_t n)
+{
+    return (unsigned)std::min<int64_t>(std::max<int64_t>((n + HAM_THREADS - 1) / HAM_THREADS, 1), 8192);
+}
+
+struct ham_operator {
+    const void *u;
+    int precision;
+    geom g;
+    const double *fld;
+    const cplx<double> *blk;
+};
+
+static int ham_apply(const ham_operator &op, bool adj, int ncol, const cplx<double> *const *in, cplx<double> *const *out,
+                     const cplx<double> (*a)[4])
+{
+    ham_cols cols;
+    for (int c = 0; c < ncol; ++c) {
+        cols.in[c] = in[c];
+        cols.out[c] = out[c];
+        for (int t = 0; t < 4; ++t) cols.a[c][t] = a[c][t];
+    }
+    const dim3 grd((unsigned)((op.g.vol + HAM_THREADS - 1) / HAM_THREADS), (unsigned)ncol);
+    if (op.precision == 64) {
+        if (adj)
+            hipLaunchKernelGGL((k_ham_apply<double, true>), grd, dim3(HAM_THREADS), 0, stream(), (const cplx<double> *)op.u, op.fld, op.blk, op.g, cols);
+        else
+            hipLaunchKernelGGL((k_ham_apply<double, false>), grd, dim3(HAM_THREADS), 0, stream(), (const cplx<double> *)op.u, op.fld, op.blk, op.g, cols);
+    } else {
+        if (adj)
+            hipLaunchKernelGGL((k_ham_apply<float, true>), grd, dim3(HAM_THREADS), 0, stream(), (const cplx<float> *)op.u, op.fld, op.blk, op.g, cols);
+        else
+            hipLaunchKernelGGL((k_ham_apply<float, false>), grd, dim3(HAM_THREADS), 0, stream(), (const cplx<float> *)op.u, op.fld, op.blk, op.g, cols);
+    }
+    NRQ_HIP(hipGetLastError());
+    return NRQ_OK;
+}
+
+// ||w||^2 to the host
+static int ham_norm2(const cplx<double> *w, int64_t n, double *red, double *out_dev, double *host, double *res)
+{
+    hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), w, n, red);
+    hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red, HAM_DOT_BLOCKS, out_dev);
+    NRQ_HIP(hipGetLastError());
+    NRQ_HIP(hipMemcpyAsync(host, out_dev, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    *res = host[0];
+    return NRQ_OK;
+}
+
+// GMRES(m) for S^+ x = b on one column (n = 6V complex), zero initial guess,
+// stopping on ||b - S^+ x|| <= tol ||b||.
+static int ham_gmres(const ham_operator &op, const cplx<double> (&a)[4], const cplx<double> *b, cplx<double> *x, int64_t n,
+                     int m, int64_t maxit, double tol, nrq_solve_stats *st)
+{
+    cplx<double> *basis = (cplx<double> *)g_ham.basis.ptr;
+    cplx<double> *w = (cplx<double> *)g_ham.vec.ptr, *r = w + n;
+    double *red = (double *)g_ham.red.ptr;
+    double *dsmall = (double *)g_ham.small.ptr; // 2 (m + 2) doubles
+    double *host = g_ham.host;
+    const cplx<double> coef[1][4] = {{a[0], a[1], a[2], a[3]}};
+    const unsigned nb = ham_blocks(n);
+    double bn2;
+    if (int e = ham_norm2(b, n, red, dsmall, host, &bn2)) return e;
+    NRQ_HIP(hipMemsetAsync(x, 0, sizeof(cplx<double>) * (size_t)n, stream()));
+    if (bn2 == 0.0) return NRQ_OK; // x = 0, as KSPSolve returns for b = 0
+    ++st->solves;
+    const double bnorm = std::sqrt(bn2);
+    // r = b (x = 0)
+    hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), r, b, 1.0, b, 0.0, n);
+    double rnorm = bnorm;
+    int64_t its = 0;
+    std::vector<std::complex<double>> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    while (true) {
+        ++st->restarts;
+        // V_0 = r / ||r||
+        hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), basis, r, 1.0 / rnorm, r, 0.0, n);
+        std::fill(g.begin(), g.end(), 0.0);
+        g[0] = rnorm;
+        int k = 0;
+        for (; k < m && its < maxit; ++k) {
+            ++its;
+            cplx<double> *vk = basis + (int64_t)k * n, *vk1 = basis + (int64_t)(k + 1) * n;
+            if (int e = ham_apply(op, true, 1, (const cplx<double> *const *)&vk, &vk1, coef)) return e;
+            // classical Gram-Schmidt: h = V^+ w, w -= V h
+            hipLaunchKernelGGL(k_ham_dots, dim3(HAM_DOT_BLOCKS, k + 1), dim3(HAM_THREADS), 0, stream(), basis, n, vk1, red);
+            hipLaunchKernelGGL(k_ham_fold, dim3(2 * (k + 1)), dim3(HAM_THREADS), 0, stream(), red, HAM_DOT_BLOCKS, dsmall);
+            hipLaunchKernelGGL(k_ham_combine, dim3(nb), dim3(HAM_THREADS), 0, stream(), vk1, basis, n, dsmall, k + 1, -1.0);
+            hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), vk1, n, red);
+            hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red, HAM_DOT_BLOCKS, dsmall + 2 * (k + 1));
+            NRQ_HIP(hipGetLastError());
+            NRQ_HIP(hipMemcpyAsync(host, dsmall, sizeof(double) * (2 * (k + 1) + 1), hipMemcpyDeviceToHost, stream()));
+            NRQ_HIP(hipStreamSynchronize(stream()));
+            for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] = {host[2 * i], host[2 * i + 1]};
+            const double hk1 = std::sqrt(host[2 * (k + 1)]);
+            H[(size_t)(k + 1) * m + k] = hk1;
+            if (hk1 > 0)
+                hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), vk1, vk1, 1.0 / hk1, vk1, 0.0, n);
+            // Givens rotations
+            for (int i = 0; i < k; ++i) {
+                const std::complex<double> t = cs[i] * H[(size_t)i * m + k] + sn[i] * H[(size_t)(i + 1) * m + k];
+                H[(size_t)(i + 1) * m + k] = -std::conj(sn[i]) * H[(size_t)i * m + k] + cs[i] * H[(size_t)(i + 1) * m + k];
+                H[(size_t)i * m + k] = t;
+            }
+            const std::complex<double> hkk = H[(size_t)k * m + k];
+            const double den = std::sqrt(std::norm(hkk) + hk1 * hk1);
+            if (den == 0.0) {
+                cs[k] = 1.0;
+                sn[k] = 0.0;
+            } else {
+                cs[k] = std::abs(hkk) / den;
+                const std::complex<double> ph = std::abs(hkk) > 0 ? hkk / std::abs(hkk) : std::complex<double>(1.0, 0.0);
+                sn[k] = ph * hk1 / den;
+            }
+            H[(size_t)k * m + k] = cs[k] * hkk + sn[k] * hk1;
+            H[(size_t)(k + 1) * m + k] = 0.0;
+            g[k + 1] = -std::conj(sn[k]) * g[k];
+            g[k] = cs[k] * g[k];
+            if (std::abs(g[k + 1]) <= 0.5 * tol * bnorm || hk1 == 0.0) {
+                ++k;
+                break;
+            }
+        }
+        // y = H^-1 g (upper triangular, k x k), x += V y
+        for (int i = k - 1; i >= 0; --i) {
+            std::complex<double> t = g[i];
+            for (int j = i + 1; j < k; ++j) t -= H[(size_t)i * m + j] * y[j];
+            y[i] = t / H[(size_t)i * m + i];
+        }
+        for (int i = 0; i < k; ++i) {
+            host[2 * i] = y[i].real();
+            host[2 * i + 1] = y[i].imag();
+        }
+        NRQ_HIP(hipMemcpyAsync(dsmall, host, sizeof(double) * 2 * k, hipMemcpyHostToDevice, stream()));
+        hipLaunchKernelGGL(k_ham_combine, dim3(nb), dim3(HAM_THREADS), 0, stream(), x, basis, n, dsmall, k, 1.0);
+        // true residual r = b - S^+ x
+        if (int e = ham_apply(op, true, 1, (const cplx<double> *const *)&x, &r, coef)) return e;
+        hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), r, b, 1.0, r, -1.0, n);
+        double rn2;
+        if (int e = ham_norm2(r, n, red, dsmall, host, &rn2)) return e;
+        rnorm = std::sqrt(rn2);
+        st->iterations += k;
+        if (rnorm <= tol * bnorm) break;
+        if (its >= maxit) {
+            st->max_true_relres = std::max(st->max_true_relres, rnorm / bnorm);
+            return fail(NRQ_ERR_NOCONV, "CN (full Hamiltonian): no convergence after %lld iterations (relres %.3e)",
+                        (long long)its, rnorm / bnorm);
+        }
+    }
+    st->max_true_relres = std::max(st->max_true_relres, rnorm / bnorm);
+    return NRQ_OK;
+}
+
+int ham_cn_update(nrq_heavy *const *hs, const double *mass, int n, const nrq_gauge *gf, const double c[8],
+                  double stepwidth, int restart, int64_t maxit, double tol, nrq_solve_stats *stats)
+{
+    const geom g = lat().g;
+    const int64_t V = g.vol;
+    const int64_t nvec = 6 * V;
+    const int m = std::max(1, restart);
+    if (int r = g_ham.fld.reserve(sizeof(double) * HAM_FLD * (size_t)V)) return r;
+    if (int r = g_ham.blk.reserve(sizeof(cplx<double>) * 9 * HAM_BLK * (size_t)V)) return r;
+    if (int r = g_ham.basis.reserve(sizeof(cplx<double>) * (size_t)(m + 1) * (size_t)nvec)) return r;
+    if (int r = g_ham.vec.reserve(sizeof(cplx<double>) * 4 * (size_t)nvec)) return r;
+    if (int r = g_ham.red.reserve(sizeof(double) * 2 * (size_t)(m + 2) * HAM_DOT_BLOCKS)) return r;
+    if (int r = g_ham.small.reserve(sizeof(double) * 2 * (size_t)(m + 2))) return r;
+    if (g_ham.host_m < m) {
+        if (g_ham.host) (void)hipHostFree(g_ham.host);
+        g_ham.host = nullptr;
+        if (hipHostMalloc((void **)&g_ham.host, sizeof(double) * 2 * (size_t)(m + 2), hipHostMallocDefault) != hipSuccess) {
+            (void)hipGetLastError();
+            g_ham.host_m = -1;
+            return fail(NRQ_ERR_ALLOC, "CN (full Hamiltonian): pinned buffer allocation failed");
+        }
+        g_ham.host_m = m;
+    }
+    ham_operator op;
+    op.u = gauge_links(gf);
+    op.precision = gf->precision;
+    op.g = g;
+    op.fld = (const double *)g_ham.fld.ptr;
+    op.blk = (const cplx<double> *)g_ham.blk.ptr;
+    const unsigned nbs = (unsigned)((V + HAM_THREADS - 1) / HAM_THREADS);
+    if (gf->precision == 64) {
+        hipLaunchKernelGGL(k_ham_fields<double>, dim3(nbs), dim3(HAM_THREADS), 0, stream(), (const cplx<double> *)op.u,
+                           (const double *)gf->e, g, (double *)g_ham.fld.ptr);
+        hipLaunchKernelGGL(k_ham_blocks<double>, dim3(nbs), dim3(HAM_THREADS), 0, stream(), (const cplx<double> *)op.u,
+                           (const double *)g_ham.fld.ptr, g, (cplx<double> *)g_ham.blk.ptr);
+    } else {
+        hipLaunchKernelGGL(k_ham_fields<float>, dim3(nbs), dim3(HAM_THREADS), 0, stream(), (const cplx<float> *)op.u,
+                           (const float *)gf->e, g, (double *)g_ham.fld.ptr);
+        hipLaunchKernelGGL(k_ham_blocks<float>, dim3(nbs), dim3(HAM_THREADS), 0, stream(), (const cplx<float> *)op.u,
+                           (const double *)g_ham.fld.ptr, g, (cplx<double> *)g_ham.blk.ptr);
+    }
+    NRQ_HIP(hipGetLastError());
+    const double a0 = g.a[0];
+    const std::complex<double> pre(0.0, -stepwidth / 2.0 * a0), I(0.0, 1.0);
+    cplx<double> *bvec = (cplx<double> *)g_ham.vec.ptr + 2 * nvec; // rhs
+    nrq_solve_stats st = {0, 0, 0, 0.0};
+    int status = NRQ_OK;
+    for (int hk = 0; hk < n; ++hk) {
+        nrq_heavy *h = hs[hk];
+        for (int prop = 0; prop < 2; ++prop) {
+            const double M = prop == 0 ? mass[hk] : -mass[hk];
+            std::complex<double> cc[4];
+            for (int t = 0; t < 4; ++t) {
+                cc[t] = std::complex<double>(c[2 * t], c[2 * t + 1]);
+                if (prop == 1) cc[t] = std::conj(cc[t]);
+            }
+            const std::complex<double> at[4] = {pre * cc[0] / M, pre * (-cc[1]) / (2.0 * M),
+                                                pre * (-cc[2]) / (8.0 * M * M), pre * (-cc[3]) * I / (8.0 * M * M)};
+            cplx<double> a[4];
+            for (int t = 0; t < 4; ++t) a[t] = {at[t].real(), at[t].imag()};
+            for (int j = 0; j < 6; ++j) {
+                const int f0 = 12 * prop + 2 * j;
+                if (!((h->live >> f0) & 3u)) continue;
+                cplx<double> *xcol = (cplx<double> *)h->psi + (int64_t)f0 * 3 * V;
+                const cplx<double> coef[1][4] = {{a[0], a[1], a[2], a[3]}};
+                // rhs = S psi
+                if (int e = ham_apply(op, false, 1, (const cplx<double> *const *)&xcol, &bvec, coef)) return e;
+                const int r = ham_gmres(op, a, bvec, xcol, nvec, m, maxit, tol, &st);
+                h->live |= 3u << f0; // the spin-orbit term fills both spin rows
+                if (r == NRQ_ERR_NOCONV) {
+                    status = r;
+                    continue;
+                }
+                if (r != NRQ_OK) return r;
+            }
+        }
+    }
+    if (stats) *stats = st;
+    return status;
+}
+
+} // namespace nrq

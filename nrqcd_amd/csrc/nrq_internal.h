@@ -47,6 +47,12 @@ constexpr int REDUCE_THREADS = 256;
 // (gaugeconfiguration_su3.f90:1954-1972).
 int transverse_project(const cplx<double> *fld, double scale, double *out);
 
+// One CN step of the live columns of n heavy fields with the full
+// Hamiltonian (c2..c4 terms, nrqcd.f90:1175-1387): FP64 GMRES(restart) to
+// true relative residual tol, at most maxit iterations per column.
+int ham_cn_update(nrq_heavy *const *hs, const double *mass, int n, const nrq_gauge *gf, const double c[8],
+                  double stepwidth, int restart, int64_t maxit, double tol, nrq_solve_stats *stats);
+
 struct scratch_buf {
     void *ptr = nullptr;
     size_t bytes = 0;

@@ -271,6 +271,7 @@ class RefModules:
             ("ref_su3_generators", None, [P]), ("ref_su2_generators", None, [P]), ("ref_group_exp", None, [P, P]),
             ("ref_algebra_matrix", None, [P, P]), ("ref_algebra_coordinate", D, [I, P]),
             ("ref_trace_with_generator", None, [I, P, P]), ("ref_expah", None, [P, P]), ("ref_group_log", None, [P, P]),
+            ("ref_arglogu", None, [P, P]),
             ("ref_kron", None, [P, I, I, P, I, I, P]),
         ]:
             f = getattr(L, name)
@@ -336,6 +337,12 @@ class RefModules:
         uu = np.asfortranarray(U, np.complex128)
         out = np.zeros(8)
         self.lib.ref_group_log(_ptr(uu), _ptr(out))
+        return out
+
+    def arglogu(self, U):
+        uu = np.asfortranarray(U, np.complex128)
+        out = np.zeros((3, 3), np.complex128, order="F")
+        self.lib.ref_arglogu(_ptr(uu), _ptr(out))
         return out
 
     def kron(self, A, B):

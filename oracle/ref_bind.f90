@@ -125,6 +125,16 @@ contains
     res = GetGroupLog(uu)
   end subroutine ref_group_log
 
+  ! matrixoperations.f90:474-495 (the full matrix, trace included)
+  subroutine ref_arglogu(u, res) bind(C, name='ref_arglogu')
+    use matrixoperations, only: ArgLogU
+    complex(c_double_complex), intent(in) :: u(3, 3)
+    complex(c_double_complex), intent(out) :: res(3, 3)
+    complex(fp) :: uu(3, 3)
+    uu = u
+    res = ArgLogU(uu)
+  end subroutine ref_arglogu
+
   ! matrixoperations.f90:81-104
   subroutine ref_kron(a, ra, ca, b, rb, cb, ab) bind(C, name='ref_kron')
     use matrixoperations, only: GetKronProd

@@ -205,3 +205,38 @@ def test_mode4_spectrum(tmp_path):
         got = np.loadtxt(tmp_path / name)
         assert np.array_equal(got[:, 0], np.arange(1, x.size + 1))
         assert np.allclose(got[:, 1] + 1j * got[:, 2], x, rtol=1e-12, atol=1e-14), name
+
+
+def test_mode31_with_all_wilson_coefficients(oracle, tmp_path):
+    """The reference's templates run with c = (1, 1, 1, 1): mode 31 with the
+    full Hamiltonian against a replay with oracle/hamiltonian_oracle.py."""
+    from oracle import gaugefix_oracle as gf
+    from oracle import hamiltonian_oracle as ho
+
+    ext, sp, seed = (4, 3, 5), (0.05, 1.0, 1.0, 1.0), 8
+    t, smax, mass = 0.05, 0.2, 2.0
+    quark = (1e-12, mass, 1.0, 0.0, 1.0, 0.0, 1.0, 0.0, 1.0, 0.0)
+    _run([31, *ext, *sp, t, smax, seed, 0.5, 1.0, 1.0, *quark, "corr.txt", "norm.txt", "corr_g.txt"], tmp_path)
+    corr = np.loadtxt(tmp_path / "corr.txt", skiprows=1)
+    corr_g = np.loadtxt(tmp_path / "corr_g.txt", skiprows=1)
+    norm = np.loadtxt(tmp_path / "norm.txt", skiprows=1)
+    oracle.lattice_init(ext, sp)
+    lat = gf.Lattice(ext, sp, oracle.generators())
+    pauli = [oracle.pauli(k) for k in (1, 2, 3)]
+    U, E = oracle.box_init(seed, 0.5, 1.0, 1.0)
+    oracle.leapfrog(U, E, 1.0, 1)
+    rows, rows_g, norms = [], [], []
+    for it1 in (0, 1, 2):
+        U2, E2 = U.copy(order="F"), E.copy(order="F")
+        Q, A = oracle.point_prop(1), oracle.point_prop(1)
+        for _ in range(2 * it1):
+            ho.cn_update(lat, U2, E2, Q, A, mass, [1, 1, 1, 1], 1.0, pauli)
+            oracle.leapfrog(U2, E2, 1.0, 1)
+        rows.append(_correlator_row(oracle.correlators(Q, A)))
+        rows_g.append(_correlator_row(oracle.correlators(Q, A, gconjg=True)))
+        norms.append((oracle.prop_norm(Q), oracle.prop_norm(A)))
+        if it1 > 0:
+            oracle.leapfrog(U, E, -1.0, 1)
+    assert np.allclose(corr[:, 1:], np.array(rows), rtol=1e-8, atol=1e-9)
+    assert np.allclose(corr_g[:, 1:], np.array(rows_g), rtol=1e-8, atol=1e-9)
+    assert np.allclose(norm[:, 1:], np.array(norms), rtol=1e-8)

@@ -152,16 +152,6 @@ def test_observables_follow_every_change(nrq, oracle, cn):
     _same(_observables(h), _oracle_observables(oracle, Qd, Ad), rel=1e-12)
 
 
-def test_cn_rejects_higher_wilson_terms(nrq, cn):
-    cn((4, 4, 4), SP)
-    g = nrq.GaugeConfiguration()
-    h = nrq.NRQCDField()
-    h.InitSinglePoint(1, 1)
-    with pytest.raises(nrq.NRQError) as e:
-        h.Update(g, 2.0, [1, 0.5, 0, 0])
-    assert e.value.code == -4
-
-
 def test_cn_norm_conservation_and_reversibility(nrq, cn):
     ext = (8, 8, 8)
     _say("CN reversibility 8^3")
