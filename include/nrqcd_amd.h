@@ -130,6 +130,19 @@ int nrq_gauge_box_ic_correlators(nrq_gauge *h, double saturation_scale, double a
  * in the reference, which reaches the first through the box form.  Fails
  * when the random module is not initialised, as the reference does. */
 int nrq_gauge_transverse_init(nrq_gauge *h, const double *occupation, int specific_sender, double *aa, double *ee);
+/* GaugeConfiguration%EquilibriumInit(Beta, nefieldinit, nequilibrium,
+ * [tolerance_Eprojection], [ChargeDensity], ...), gaugeconfiguration_su3.f90:
+ * 380-814: from cold links, nefieldinit times thermal E (sigma = sqrt(4/beta)),
+ * Gauss-law projection to max 2|Tr(T^a G)| <= tolerance (<= 0: the
+ * reference's default 1E-3 as a single-precision literal), nequilibrium
+ * leapfrog steps.  charge_density: NULL or 3 x V (colour fastest, the
+ * reference's ChargeDensity(colour, index)).  Optional per-iteration outputs
+ * (nefieldinit entries, NULL to skip): energy and Gauss deviation after the
+ * leapfrog steps, projection sweeps.  The projection is capped at 10^6 sweeps
+ * (NRQ_ERR_NOCONV); the reference has no cap. */
+int nrq_gauge_equilibrium_init(nrq_gauge *h, double beta, int64_t nefieldinit, int64_t nequilibrium, double tolerance,
+                               const double *charge_density, double *energies, double *deviations,
+                               int64_t *projections);
 /* Upload / download in the reference layouts links(3,3,3,V) complex128 and
  * efield(8,3,V) real64 (gaugeconfiguration_su3.f90:35-37).  Either pointer may
  * be NULL to skip that field. */

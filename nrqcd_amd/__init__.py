@@ -158,6 +158,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_spectral_functions": (I, [P, I64, I64, D, P, P]),
         "nrq_gauge_box_ic_correlators": (I, [P, D, D, D, P, P]),
         "nrq_gauge_transverse_init": (I, [P, P, I, P, P]),
+        "nrq_gauge_equilibrium_init": (I, [P, D, I64, I64, D, P, P, P, P]),
         "nrq_t2w": (I, [P, I64, D]),
         "nrq_w2t": (I, [P, I64, D]),
     }
@@ -343,6 +344,22 @@ class GaugeConfiguration:
         ee = np.zeros(v) if correlators else None
         _check(library().nrq_gauge_transverse_init(self._h, _ptr(occ), sender, _ptr(aa), _ptr(ee)))
         return (aa, ee) if correlators else None
+
+    def EquilibriumInit(self, Beta: float, nefieldinit: int, nequilibrium: int, tolerance_Eprojection=None,
+                        ChargeDensity=None):
+        """gaugeconfiguration_su3.f90:380-814; returns per outer iteration
+        (energy, Gauss deviation after the leapfrog steps, projection sweeps)."""
+        v = lattice_size()
+        rho = None
+        if ChargeDensity is not None:
+            rho = np.asfortranarray(ChargeDensity, np.float64).reshape((3, v), order="F")
+        en = np.zeros(max(nefieldinit, 1))
+        dev = np.zeros(max(nefieldinit, 1))
+        nproj = np.zeros(max(nefieldinit, 1), np.int64)
+        tol = -1.0 if tolerance_Eprojection is None else float(tolerance_Eprojection)
+        _check(library().nrq_gauge_equilibrium_init(self._h, float(Beta), int(nefieldinit), int(nequilibrium), tol,
+                                                    _ptr(rho), _ptr(en), _ptr(dev), _ptr(nproj)))
+        return en[:nefieldinit], dev[:nefieldinit], nproj[:nefieldinit]
 
     def TransversePolarisedOccupiedInit(self, Occupation, correlators: bool = False):
         """gaugeconfiguration_su3.f90:1263-1446 for any occupation by lattice index."""

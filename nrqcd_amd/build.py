@@ -23,7 +23,7 @@ FSRC = PKG / "fortran"
 LIB = PKG / "libnrqcd_amd.so"
 ROCM = Path(os.environ.get("ROCM_PATH", "/opt/rocm"))
 
-HIP_SOURCES = ["nrq_core.hip", "nrq_gauge.hip", "nrq_ic.hip", "nrq_gaugefix.hip", "nrq_krylov.hip", "nrq_cn.hip", "nrq_cnn.hip", "nrq_spectrum.hip", "nrq_hamiltonian.hip"]
+HIP_SOURCES = ["nrq_core.hip", "nrq_gauge.hip", "nrq_ic.hip", "nrq_gaugefix.hip", "nrq_krylov.hip", "nrq_cn.hip", "nrq_cnn.hip", "nrq_spectrum.hip", "nrq_hamiltonian.hip", "nrq_equilibrium.hip"]
 HIPCC_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

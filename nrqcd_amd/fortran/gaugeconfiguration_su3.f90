@@ -40,6 +40,7 @@ module gaugeconfiguration_su3
      procedure, public :: Reunitarise
      procedure, public :: SetReunitarisationInterval
      procedure, public :: CoulombGaugefixing
+     procedure, public :: EquilibriumInit
      procedure, public :: TransversePolarisedOccupiedInit
      procedure, public :: TransversePolarisedOccupiedInit_specificSender
      procedure, public :: GetTransverseAACorrelator
@@ -136,6 +137,54 @@ contains
     call nrq_check(nrq_gauge_box_ic_correlators(GaugeConf%handle, SaturationScale, Amplitude, Coupling, pa, pe), &
          'TransversePolarisedOccupiedInit_Box')
   end subroutine TransversePolarisedOccupiedInit_Box
+
+  !> EquilibriumInit(Beta, nefieldinit, nequilibrium, [tolerance_Eprojection],
+  !! [ChargeDensity], [MeasureEnergy], [filename]),
+  !! gaugeconfiguration_su3.f90:380-814: thermal E projected onto the Gauss
+  !! law and leapfrog equilibration, nefieldinit times from cold links.  As in
+  !! the reference, the Gauss deviation after each iteration is printed and,
+  !! with MeasureEnergy, the iteration number and energy are written to
+  !! filename and printed.
+  subroutine EquilibriumInit(GaugeConf, Beta, nefieldinit, nequilibrium, tolerance_Eprojection, ChargeDensity, &
+       MeasureEnergy, filename)
+    use, intrinsic :: iso_fortran_env, only: output_unit
+    use mpiinterface, only: MPIstop
+    class(GaugeConfiguration), intent(inout) :: GaugeConf
+    real(fp), intent(in) :: Beta
+    integer(int64), intent(in) :: nefieldinit, nequilibrium
+    real(fp), intent(in), optional :: tolerance_Eprojection
+    real(fp), intent(in), optional, target, contiguous :: ChargeDensity(:, :)
+    logical, intent(in), optional :: MeasureEnergy
+    character(len=*), intent(in), optional :: filename
+    real(fp), allocatable, target :: energies(:), deviations(:)
+    integer(c_int64_t), allocatable, target :: projections(:)
+    real(fp) :: tol
+    type(c_ptr) :: prho
+    logical :: measure
+    integer :: fid
+    integer(int64) :: it
+    tol = -1
+    if (present(tolerance_Eprojection)) tol = tolerance_Eprojection
+    prho = c_null_ptr
+    if (present(ChargeDensity)) prho = c_loc(ChargeDensity)
+    measure = .false.
+    if (present(MeasureEnergy)) measure = MeasureEnergy
+    if (measure .and. .not. present(filename)) call MPIstop('Filename for energy output missing')
+    allocate(energies(max(nefieldinit, 1_int64)), deviations(max(nefieldinit, 1_int64)), &
+         projections(max(nefieldinit, 1_int64)))
+    call GaugeConf%Allocate
+    call nrq_check(nrq_gauge_equilibrium_init(GaugeConf%handle, Beta, nefieldinit, nequilibrium, tol, prho, &
+         c_loc(energies), c_loc(deviations), c_loc(projections)), 'EquilibriumInit')
+    if (measure) open(newunit=fid, file=trim(filename), status='replace', form='formatted', action='write')
+    do it = 1, nefieldinit
+       if (measure) then
+          write(fid, *) it, energies(it)
+          write(output_unit, *) it, energies(it)
+       end if
+       write(output_unit, *) 'G[U,E]=', deviations(it)
+    end do
+    if (measure) close(fid)
+  end subroutine EquilibriumInit
 
   !> TransversePolarisedOccupiedInit(Occupation, [aa_correlator],
   !! [ee_correlator]), gaugeconfiguration_su3.f90:1263-1446: any occupation

@@ -10,6 +10,10 @@
 !!   3  MeasureHeavyQuarkoniumCorrelators  (main.f90:2169-2447)
 !!        nrqcd_run 3 Nx Ny Nz ds ax ay az tmin tmax smax Seed Qs Amp g kspTol M
 !!                  c1re c1im c2re c2im c3re c3im c4re c4im FileCorrelator FileNorm
+!!   33 MeasureHeavyQuarkoniumCorrelators_Equilibrium (main.f90:18-280)
+!!        nrqcd_run 33 Nx Ny Nz a0 ax ay az smax Seed Beta nefieldinit nequilibrium kspTol M
+!!                  c1re c1im c2re c2im c3re c3im c4re c4im FileCorrelator FileNorm [MeasureEnergy [FileEnergy]]
+!!        -> as mode 32, from EquilibriumInit instead of cold links
 !!   4  ComputeSpectrum (main.f90:1865-2100)
 !!        nrqcd_run 4 nHeaderLines FileInput FileOutput MaxTime
 !!        -> FileOutput: omega and 2 Im of the transformed windowed correlator
@@ -62,6 +66,8 @@ program nrqcd_run
      call MeasureHeavyQuarkoniumCorrelators_oneT()
   case (32)
      call MeasureHeavyQuarkoniumCorrelators_free()
+  case (33)
+     call MeasureHeavyQuarkoniumCorrelators_Equilibrium()
   case default
      call usage()
   end select
@@ -69,7 +75,7 @@ program nrqcd_run
 contains
 
   subroutine usage()
-    write(output_unit, '(A)') 'usage: nrqcd_run 1|2|3|31|32|4 <positional parameters of the reference mode>'
+    write(output_unit, '(A)') 'usage: nrqcd_run 1|2|3|31|32|33|4 <positional parameters of the reference mode>'
     stop 2
   end subroutine usage
 
@@ -373,6 +379,74 @@ contains
     call FinalizeModule_NRQCD()
     write(output_unit, *) 'Simulation completed'
   end subroutine MeasureHeavyQuarkoniumCorrelators_free
+
+  !> Mode 33: as mode 32 from a thermal configuration (EquilibriumInit with
+  !! Beta, nefieldinit, nequilibrium) instead of cold links (main.f90:18-280)
+  subroutine MeasureHeavyQuarkoniumCorrelators_Equilibrium()
+    integer(int64) :: ext(3), seed, tsteps, it, nefieldinit, nequilibrium
+    real(fp) :: spacings(0:3), smax, beta, kspTol, mass
+    complex(fp) :: wc(4)
+    type(GaugeConfiguration) :: GaugeConf_t
+    type(NRQCDField) :: HeavyField_t
+    character(len=256) :: file_corr, file_norm, file_energy
+    logical :: measure
+    integer :: k, fc, fn
+
+    if (command_argument_count() < 25) call usage()
+    do k = 1, 3
+       ext(k) = iarg(1 + k)
+    end do
+    do k = 0, 3
+       spacings(k) = rarg(5 + k)
+    end do
+    smax = rarg(9)
+    seed = iarg(10)
+    beta = rarg(11)
+    nefieldinit = iarg(12)
+    nequilibrium = iarg(13)
+    call read_quark_args(14, kspTol, mass, wc)
+    call get_command_argument(24, file_corr)
+    call get_command_argument(25, file_norm)
+    ! a missing or zero flag: no energy measurement (the reference reads '(I1)')
+    measure = .false.
+    file_energy = ''
+    if (command_argument_count() >= 26) then
+       measure = iarg(26) /= 0
+       if (measure) call get_command_argument(27, file_energy)
+    end if
+
+    call InitModule_Lattice(ext, spacings)
+    call InitModule_Random(seed + ThisProc())
+    call InitModule_NRQCD(mass, wc, 1._fp, kspTol)
+
+    tsteps = nint(abs(smax) / spacings(0), int64)
+    if (measure) then
+       call GaugeConf_t%EquilibriumInit(beta, nefieldinit, nequilibrium, MeasureEnergy=measure, &
+            filename=trim(file_energy))
+    else
+       call GaugeConf_t%EquilibriumInit(beta, nefieldinit, nequilibrium)
+    end if
+    call HeavyField_t%InitSinglePoint(1_int64, 1_int64)
+
+    open(newunit=fc, file=trim(file_corr), status='replace', form='formatted', action='write')
+    open(newunit=fn, file=trim(file_norm), status='replace', form='formatted', action='write')
+    call write_correlator_header(fc)
+    write(fn, '(A1,1X,A5,1X,A5)') 's', 'Quark', 'Antiq'
+    call print_observables(0._fp, HeavyField_t, fc, fn)
+    do it = 1, tsteps
+       call HeavyField_t%Update(GaugeConf_t, mass, wc)
+       call GaugeConf_t%Update()
+       call print_observables(spacings(0) * it, HeavyField_t, fc, fn)
+       write(output_unit, '(F7.3,A1)') real(it) / tsteps * 100, '%'
+    end do
+    close(fc)
+    close(fn)
+    write(output_unit, '(A,I0)') 'Krylov iterations: ', TotalKrylovIterations
+    call HeavyField_t%Destructor
+    call GaugeConf_t%Deallocate
+    call FinalizeModule_NRQCD()
+    write(output_unit, *) 'Simulation completed'
+  end subroutine MeasureHeavyQuarkoniumCorrelators_Equilibrium
 
   !> Mode 31: correlators against the relative time s at one centre-of-mass
   !! time t (main.f90:524-830).  The gauge field is evolved to t; for

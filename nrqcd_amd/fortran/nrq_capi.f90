@@ -105,6 +105,13 @@ module nrq_capi
        type(c_ptr), value :: h, occupation, aa, ee
        integer(c_int), value :: specific_sender
      end function nrq_gauge_transverse_init
+     integer(c_int) function nrq_gauge_equilibrium_init(h, beta, nefieldinit, nequilibrium, tol, rho, energies, &
+          deviations, projections) bind(C, name='nrq_gauge_equilibrium_init')
+       import :: c_int, c_ptr, c_double, c_int64_t
+       type(c_ptr), value :: h, rho, energies, deviations, projections
+       real(c_double), value :: beta, tol
+       integer(c_int64_t), value :: nefieldinit, nequilibrium
+     end function nrq_gauge_equilibrium_init
      integer(c_int) function nrq_gauge_upload(h, links, efield) bind(C, name='nrq_gauge_upload')
        import :: c_int, c_ptr
        type(c_ptr), value :: h, links, efield

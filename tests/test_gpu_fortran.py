@@ -240,3 +240,37 @@ def test_mode31_with_all_wilson_coefficients(oracle, tmp_path):
     assert np.allclose(corr[:, 1:], np.array(rows), rtol=1e-8, atol=1e-9)
     assert np.allclose(corr_g[:, 1:], np.array(rows_g), rtol=1e-8, atol=1e-9)
     assert np.allclose(norm[:, 1:], np.array(norms), rtol=1e-8)
+
+
+def test_mode33_equilibrium_correlators(oracle, tmp_path):
+    """main.f90:18-280: EquilibriumInit, then the point source with CN and
+    leapfrog steps and all four correlators per step; the energy file of the
+    optional MeasureEnergy flag.  Against an oracle replay."""
+    from oracle import equilibrium_oracle as eo
+    from oracle import gaugefix_oracle as gf
+
+    ext, sp, seed = (4, 3, 4), (0.05, 1.0, 1.0, 1.0), 6
+    beta, nef, neq, smax = 16.0, 1, 3, 0.1
+    # mode Nx Ny Nz a0 ax ay az smax Seed Beta nefieldinit nequilibrium kspTol M c1..c4 files flag file
+    _run([33, *ext, *sp, smax, seed, beta, nef, neq, *QUARK_ARGS, "corr.txt", "norm.txt", 1, "energy.txt"], tmp_path)
+    corr = np.loadtxt(tmp_path / "corr.txt", skiprows=1)
+    norm = np.loadtxt(tmp_path / "norm.txt", skiprows=1)
+    energy = np.loadtxt(tmp_path / "energy.txt", ndmin=2)
+    oracle.lattice_init(ext, sp)
+    lat = gf.Lattice(ext, sp, oracle.generators())
+    oracle.ranlxd_init(2, seed)
+    U, E, rec = eo.equilibrium_init(oracle, lat, beta, nef, neq)
+    assert energy.shape == (1, 2) and energy[0, 0] == 1
+    assert energy[0, 1] == pytest.approx(rec[0][1], rel=1e-10)
+    Q, A = oracle.point_prop(1), oracle.point_prop(1)
+    rows = [_correlator_row(oracle.correlators(Q, A))]
+    norms = [(oracle.prop_norm(Q), oracle.prop_norm(A))]
+    for _ in range(2):
+        rc, _ = oracle.cn_update(U, Q, A, 2.0, [1, 0, 0, 0], 1.0, 1e-12)
+        assert rc == 0
+        oracle.leapfrog(U, E, 1.0, 1)
+        rows.append(_correlator_row(oracle.correlators(Q, A)))
+        norms.append((oracle.prop_norm(Q), oracle.prop_norm(A)))
+    assert corr.shape == (3, 9)
+    assert np.allclose(corr[:, 1:], np.array(rows), rtol=1e-8, atol=1e-9)
+    assert np.allclose(norm[:, 1:], np.array(norms), rtol=1e-8)
