@@ -15,8 +15,12 @@
 //     reference's, taken literally);
 //   nequilibrium leapfrog steps of step width 1 (:696-698).
 // The default tolerance is the reference's single-precision literal 1E-3.
-// k_eq_gauss writes G and per-block maxima of the deviation; k_eq_project
-// writes the new E into a second buffer, which then replaces the old one.
+// k_eq_gauss writes G and per-block maxima of the deviation, k_eq_fold_max
+// folds them; k_eq_project writes the new E into the other of two buffers.
+// The sweeps run in batches without host synchronisation: every kernel of a
+// sweep first reads a device flag that the previous sweep's fold set once
+// the deviation reached the tolerance, and returns at once if it is set, so
+// exactly the reference's number of sweeps changes E.
 #include "nrq_internal.h"
 #include <algorithm>
 #include <cmath>
@@ -50,12 +54,24 @@ template <typename R> __device__ __forceinline__ mat3<double> eq_efield(const R 
     return algebra_matrix(al);
 }
 
-// G(x) into gm[e V + s] (9 complex) and the block maxima of 2 |Tr(T^a G)|
+// Loop state on the device: done once the deviation is <= tol, the number of
+// sweeps that changed E, and the latest deviation.
+struct eq_state {
+    int done;
+    int pad;
+    long long sweeps;
+    double dev;
+};
+
+// G(x) into gm[e V + s] (9 complex) and the block maxima of 2 |Tr(T^a G)|;
+// nothing when st->done (st may be null: always)
 template <typename R>
 __global__ __launch_bounds__(EQ_THREADS) void k_eq_gauss(const cplx<R> *__restrict__ u, const R *__restrict__ ef,
                                                         const double *__restrict__ rho, geom g,
-                                                        cplx<double> *__restrict__ gm, double *__restrict__ bmax)
+                                                        cplx<double> *__restrict__ gm, double *__restrict__ bmax,
+                                                        const eq_state *__restrict__ st)
 {
+    if (st && st->done) return;
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     double dev = 0.0;
     if (s < g.vol) {
@@ -108,8 +124,13 @@ __global__ __launch_bounds__(EQ_THREADS) void k_eq_gauss(const cplx<R> *__restri
     }
 }
 
-__global__ __launch_bounds__(EQ_THREADS) void k_eq_fold_max(const double *__restrict__ bmax, int nb, double *__restrict__ out)
+// Deviation = max of the block maxima.  count = 0: the deviation of the
+// starting configuration (sets done, sweeps = 0); count = 1: the deviation
+// after a sweep, unless done (sweeps + 1, then done = dev <= tol).
+__global__ __launch_bounds__(EQ_THREADS) void k_eq_fold_max(const double *__restrict__ bmax, int nb, eq_state *__restrict__ st,
+                                                           double tol, int count)
 {
+    if (count && st->done) return;
     double m = 0.0;
     for (int b = threadIdx.x; b < nb; b += EQ_THREADS) m = fmax(m, bmax[b]);
     __shared__ double sh[EQ_THREADS / 64];
@@ -120,7 +141,9 @@ __global__ __launch_bounds__(EQ_THREADS) void k_eq_fold_max(const double *__rest
     if (threadIdx.x == 0) {
         double t = 0.0;
         for (int k = 0; k < EQ_THREADS / 64; ++k) t = fmax(t, sh[k]);
-        out[0] = t;
+        st->dev = t;
+        st->sweeps = count ? st->sweeps + 1 : 0;
+        st->done = t <= tol ? 1 : 0;
     }
 }
 
@@ -128,8 +151,9 @@ __global__ __launch_bounds__(EQ_THREADS) void k_eq_fold_max(const double *__rest
 template <typename R>
 __global__ __launch_bounds__(EQ_THREADS) void k_eq_project(const cplx<R> *__restrict__ u, const R *__restrict__ ef,
                                                           const cplx<double> *__restrict__ gm, geom g, double kdt,
-                                                          R *__restrict__ ef_new)
+                                                          R *__restrict__ ef_new, const eq_state *__restrict__ st)
 {
+    if (st->done) return;
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.vol) return;
     const int64_t V = g.vol;
@@ -156,6 +180,9 @@ __global__ __launch_bounds__(EQ_THREADS) void k_eq_project(const cplx<R> *__rest
     }
 }
 
+// Sweeps launched per batch between host checks of the loop state.
+constexpr int EQ_BATCH = 64;
+
 template <typename R>
 static int equilibrium_t(nrq_gauge *h, double beta, int64_t nefieldinit, int64_t nequilibrium, double tol,
                          const double *rho_host, double *energies, double *deviations, int64_t *projections)
@@ -165,37 +192,43 @@ static int equilibrium_t(nrq_gauge *h, double beta, int64_t nefieldinit, int64_t
     const unsigned nb = (unsigned)((V + EQ_THREADS - 1) / EQ_THREADS);
     struct bufs {
         cplx<double> *gm = nullptr;
-        double *bmax = nullptr, *dmax = nullptr, *rho = nullptr, *hmax = nullptr;
-        void *enew = nullptr;
+        double *bmax = nullptr, *rho = nullptr;
+        eq_state *st = nullptr, *hst = nullptr;
+        void *ealt = nullptr;
         ~bufs()
         {
             if (gm) (void)hipFree(gm);
             if (bmax) (void)hipFree(bmax);
-            if (dmax) (void)hipFree(dmax);
+            if (st) (void)hipFree(st);
             if (rho) (void)hipFree(rho);
-            if (enew) (void)hipFree(enew);
-            if (hmax) (void)hipHostFree(hmax);
+            if (ealt) (void)hipFree(ealt);
+            if (hst) (void)hipHostFree(hst);
         }
     } b;
     const size_t ebytes = sizeof(R) * 24 * (size_t)V;
     if (hipMalloc((void **)&b.gm, sizeof(cplx<double>) * 9 * (size_t)V) != hipSuccess ||
         hipMalloc((void **)&b.bmax, sizeof(double) * nb) != hipSuccess ||
-        hipMalloc((void **)&b.dmax, sizeof(double)) != hipSuccess || hipMalloc(&b.enew, ebytes) != hipSuccess ||
+        hipMalloc((void **)&b.st, sizeof(eq_state)) != hipSuccess || hipMalloc(&b.ealt, ebytes) != hipSuccess ||
         (rho_host && hipMalloc((void **)&b.rho, sizeof(double) * 3 * (size_t)V) != hipSuccess) ||
-        hipHostMalloc((void **)&b.hmax, sizeof(double), hipHostMallocDefault) != hipSuccess) {
+        hipHostMalloc((void **)&b.hst, sizeof(eq_state), hipHostMallocDefault) != hipSuccess) {
         (void)hipGetLastError();
         return fail(NRQ_ERR_ALLOC, "EquilibriumInit: device allocation failed (V=%lld)", (long long)V);
     }
     if (rho_host) NRQ_HIP(hipMemcpyAsync(b.rho, rho_host, sizeof(double) * 3 * (size_t)V, hipMemcpyHostToDevice, stream()));
     if (int r = nrq_gauge_cold(h)) return r;
-    auto deviation = [&](double *out) -> int {
+    R *ebuf[2] = {(R *)h->e, (R *)b.ealt};
+    // deviation of the configuration in h->e (starts the loop state); the
+    // links are looked up at every use, since the leapfrog swaps their buffers
+    auto start = [&]() -> int {
         hipLaunchKernelGGL(k_eq_gauss<R>, dim3(nb), dim3(EQ_THREADS), 0, stream(), (const cplx<R> *)gauge_links(h),
-                           (const R *)h->e, b.rho, g, b.gm, b.bmax);
-        hipLaunchKernelGGL(k_eq_fold_max, dim3(1), dim3(EQ_THREADS), 0, stream(), b.bmax, (int)nb, b.dmax);
+                           (const R *)h->e, b.rho, g, b.gm, b.bmax, (const eq_state *)nullptr);
+        hipLaunchKernelGGL(k_eq_fold_max, dim3(1), dim3(EQ_THREADS), 0, stream(), b.bmax, (int)nb, b.st, tol, 0);
         NRQ_HIP(hipGetLastError());
-        NRQ_HIP(hipMemcpyAsync(b.hmax, b.dmax, sizeof(double), hipMemcpyDeviceToHost, stream()));
+        return NRQ_OK;
+    };
+    auto read_state = [&]() -> int {
+        NRQ_HIP(hipMemcpyAsync(b.hst, b.st, sizeof(eq_state), hipMemcpyDeviceToHost, stream()));
         NRQ_HIP(hipStreamSynchronize(stream()));
-        *out = b.hmax[0];
         return NRQ_OK;
     };
     const double sigma = std::sqrt(4.0 / beta);
@@ -215,25 +248,40 @@ static int equilibrium_t(nrq_gauge *h, double beta, int64_t nefieldinit, int64_t
                 }
             }
         NRQ_HIP(hipMemcpyAsync(h->e, ehost.data(), ebytes, hipMemcpyHostToDevice, stream()));
-        double dev;
-        if (int r = deviation(&dev)) return r;
-        int64_t nproj = 0;
-        while (dev > tol) {
-            if (++nproj > EQ_MAX_PROJECTIONS)
+        if (int r = start()) return r;
+        if (int r = read_state()) return r;
+        // sweep j (from 0) reads buffer j % 2 and writes the other; sweeps
+        // after the one that reached the tolerance do nothing
+        const cplx<R> *u = (const cplx<R> *)gauge_links(h); // fixed during the projection
+        int64_t launched = 0;
+        while (!b.hst->done) {
+            if (launched >= EQ_MAX_PROJECTIONS)
                 return fail(NRQ_ERR_NOCONV, "EquilibriumInit: Gauss-law projection above %g after %lld sweeps (deviation %g)",
-                            tol, (long long)EQ_MAX_PROJECTIONS, dev);
-            hipLaunchKernelGGL(k_eq_project<R>, dim3(nb), dim3(EQ_THREADS), 0, stream(), (const cplx<R> *)gauge_links(h),
-                               (const R *)h->e, b.gm, g, EQ_KAPPA * a0, (R *)b.enew);
+                            tol, (long long)EQ_MAX_PROJECTIONS, b.hst->dev);
+            for (int j = 0; j < EQ_BATCH; ++j, ++launched) {
+                const R *src = ebuf[launched & 1];
+                R *dst = ebuf[(launched + 1) & 1];
+                hipLaunchKernelGGL(k_eq_project<R>, dim3(nb), dim3(EQ_THREADS), 0, stream(), u, src, b.gm, g, EQ_KAPPA * a0,
+                                   dst, (const eq_state *)b.st);
+                hipLaunchKernelGGL(k_eq_gauss<R>, dim3(nb), dim3(EQ_THREADS), 0, stream(), u, (const R *)dst, b.rho, g,
+                                   b.gm, b.bmax, (const eq_state *)b.st);
+                hipLaunchKernelGGL(k_eq_fold_max, dim3(1), dim3(EQ_THREADS), 0, stream(), b.bmax, (int)nb, b.st, tol, 1);
+            }
             NRQ_HIP(hipGetLastError());
-            NRQ_HIP(hipMemcpyAsync(h->e, b.enew, ebytes, hipMemcpyDeviceToDevice, stream()));
-            if (int r = deviation(&dev)) return r;
+            if (int r = read_state()) return r;
         }
+        const int64_t nproj = b.hst->sweeps;
+        if (nproj & 1) // the final E is in the second buffer
+            NRQ_HIP(hipMemcpyAsync(h->e, b.ealt, ebytes, hipMemcpyDeviceToDevice, stream()));
         if (nequilibrium > 0)
             if (int r = nrq_gauge_leapfrog(h, 1.0, nequilibrium)) return r;
         if (energies)
             if (int r = nrq_gauge_energy(h, &energies[it])) return r;
-        if (deviations)
-            if (int r = deviation(&deviations[it])) return r;
+        if (deviations) {
+            if (int r = start()) return r;
+            if (int r = read_state()) return r;
+            deviations[it] = b.hst->dev;
+        }
         if (projections) projections[it] = nproj;
     }
     return NRQ_OK;

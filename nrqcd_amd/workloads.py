@@ -6,6 +6,7 @@
     python -m nrqcd_amd.workloads gluon      # Coulomb gauge fixing + transverse correlators, f(p) at 64^3
     python -m nrqcd_amd.workloads spectrum   # mode-31 rows at 64^3 (CN step, 8 correlators, norms) and
                                              # spectral functions of 1024 correlators of 4097 points
+    python -m nrqcd_amd.workloads full       # CN steps with c = (1,1,1,1) and EquilibriumInit at 64^3
 
 Each prints one JSON object.  Inputs are synthetic (the seeded box initial
 condition of gaugeconfiguration_su3.f90:906-951); all timing uses HIP events
@@ -306,9 +307,46 @@ def spectrum(size: int = 64, steps: int = 8, seed: int = 2, reps: int = 10, batc
     }
 
 
+def full(size: int = 64, steps: int = 3, seed: int = 2, beta: float = 16.0, nequilibrium: int = 10) -> dict:
+    """The CN step with the full Hamiltonian (c = (1, 1, 1, 1), the reference
+    templates' coefficients; nrqcd.f90:1175-1387) against the kinetic-only
+    step with the same FP64 GMRES, and one pass of EquilibriumInit
+    (gaugeconfiguration_su3.f90:380-814) at `size`^3."""
+    lattice_init((size,) * 3, SPACINGS)
+    random_init(seed)
+    nrqcd_init(2.0, C1, 1.0, 1e-10)
+    nrqcd_set_solver(NRQ_SOLVER_GMRES_FP64, 30, 10000)
+    g = GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    g.Update(1.0, 20)
+    out = {"workload": f"{size}^3 box IC Qs=0.5 seed {seed}, 20 leapfrog steps, point source, Ma=2, FP64 GMRES(30) "
+                       f"to 1e-10; EquilibriumInit beta={beta}, one pass, {nequilibrium} leapfrog steps"}
+    for name, c in (("kinetic", [1, 0, 0, 0]), ("full", [1, 1, 1, 1])):
+        h = NRQCDField()
+        h.InitSinglePoint(1, 1)
+        h.Update(g, 2.0, c)  # warm-up (workspace, first step)
+        stats = []
+        ms = _timed(lambda: [stats.append(h.Update(g, 2.0, c)) for _ in range(steps)]) / steps
+        out[name] = {"ms_per_cn_step": ms, "iterations_per_column_solve": stats[-1].iterations / max(stats[-1].solves, 1),
+                     "worst_true_relres": max(s.max_true_relres for s in stats)}
+        del h
+    out["full_over_kinetic"] = out["full"]["ms_per_cn_step"] / out["kinetic"]["ms_per_cn_step"]
+    del g
+    e = GaugeConfiguration()
+    rec = {}
+    ms_eq = _timed(lambda: rec.setdefault("r", e.EquilibriumInit(beta, 1, nequilibrium)))
+    en, dev, nproj = rec["r"]
+    out["equilibrium"] = {"ms": ms_eq, "projection_sweeps": int(nproj[0]), "ms_per_sweep_upper_bound": ms_eq / max(int(nproj[0]), 1),
+                          "energy": float(en[0]), "gauss_deviation": float(dev[0])}
+    del e
+    nrqcd_finalize()
+    lattice_finalize()
+    return out
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon", "spectrum"))
+    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon", "spectrum", "full"))
     ap.add_argument("--size", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--members", type=int, default=32)
@@ -321,6 +359,8 @@ def main(argv=None) -> None:
         out = learned(**{k: v for k, v in kw.items() if k in ("size", "cn_steps")})
     elif a.workload == "gluon":
         out = gluon(**{k: v for k, v in kw.items() if k in ("size", "steps")})
+    elif a.workload == "full":
+        out = full(**{k: v for k, v in kw.items() if k in ("size", "steps")})
     elif a.workload == "spectrum":
         out = spectrum(**{k: v for k, v in kw.items() if k in ("size", "steps")})
     else:
