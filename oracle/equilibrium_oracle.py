@@ -22,10 +22,11 @@ measurement branches):
 
 Pinning (tests/test_oracle_equilibrium.py): G's deviation equals the C
 oracle's Gauss measure (GetGdeviation, itself pinned); a configuration that
-satisfies the Gauss law is a fixed point of the projection; the E draws are
+satisfies the Gauss law is a fixed point of the projection; the projection
+sweep agrees with its adjoint-representation form (R(U) g and the d-symbols
+of SU(3), neighbour from the site coordinates), which a dropped -G term, a
+flipped sign, G(x-k), U^+ G U or kappa without a0 each fail; the E draws are
 the Box-Muller normals of the pinned ranlxd stream at the predicted offsets.
-The product form of the projection has no independent check beyond these
-(parity of that step rests on the restatement).
 """
 from __future__ import annotations
 

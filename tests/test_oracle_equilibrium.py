@@ -69,3 +69,43 @@ def test_projection_reaches_the_tolerance(oracle):
     assert rec[0][0] > 0
     assert rec[0][2] <= 2 * eo.DEFAULT_TOL  # the leapfrog keeps the Gauss law
     assert eo.DEFAULT_TOL == 0.0010000000474974513
+
+
+def test_projection_sweep_in_the_adjoint_representation(oracle):
+    """One projection sweep by a second route.  G is Hermitian traceless, so
+    G = g^a T^a with g^a = 2 Re Tr(T^a G); U G U^+ has coordinates R(U) g with
+    R(U)_ab = 2 Re Tr(T^a U T^b U^+); and 2 Re Tr(T^c A B) = d^abc alpha^a beta^b / 2
+    for A = alpha^a T^a, B = beta^b T^b, with d^abc = 2 Tr({T^a, T^b} T^c).  Hence
+    E^c_k(x) <- kappa a0 [d^abc (R(U_k(x)) g(x))_a g(x+k)_b / 2 - g^c(x)] + E^c_k(x),
+    with x + k taken from the site coordinates.  A spacing a0 != 1 separates
+    kappa from kappa a0; random links separate U G U^+ from U^+ G U."""
+    ext, sp = (3, 4, 5), (0.07, 1.0, 0.9, 1.2)
+    lat = _lat(oracle, ext, sp)
+    U, E = oracle.box_init(3, 0.7, 1.0, 1.0)
+    oracle.leapfrog(U, E, 1.0, 2)
+    W = gf.links_sdrc(U)
+    G = eo.gauss_matrices(lat, W, E)
+    got = eo.project(lat, W, E, G)
+
+    Tm = np.transpose(lat.T, (2, 0, 1))  # (8, 3, 3)
+    g = 2 * np.real(np.einsum("arc,xcr->xa", Tm, G))
+    assert np.allclose(G, np.einsum("xa,arc->xrc", g, Tm), atol=1e-13)  # premise: G in the algebra
+    d = np.zeros((8, 8, 8))
+    for a in range(8):
+        for b in range(8):
+            ac = Tm[a] @ Tm[b] + Tm[b] @ Tm[a]
+            for c in range(8):
+                d[a, b, c] = 2 * np.real(np.trace(ac @ Tm[c]))
+    n0, n1, n2 = ext
+    kdt = eo.KAPPA * sp[0]
+    E = np.asarray(E)
+    for s in range(lat.vol):
+        x, y, z = s % n0, (s // n0) % n1, s // (n0 * n1)
+        for k in range(3):
+            q = [x, y, z]
+            q[k] = (q[k] + 1) % ext[k]
+            sk = q[0] + n0 * (q[1] + n1 * q[2])
+            u = W[s, k]
+            R = np.array([[2 * np.real(np.trace(Tm[a] @ u @ Tm[b] @ u.conj().T)) for b in range(8)] for a in range(8)])
+            want = kdt * (0.5 * np.einsum("abc,a,b->c", d, R @ g[s], g[sk]) - g[s]) + E[:, k, s]
+            assert np.allclose(got[:, k, s], want, rtol=0, atol=1e-13), (s, k)
