@@ -75,13 +75,16 @@ scratch_buf::~scratch_buf()
 // ranlxd (Luscher's RANLUX, double precision, 48-bit base; the algorithm of
 // ranlxd_generator.f90:77-349).  Each update() applies pr steps of the
 // subtract-with-borrow recurrence x[ir] <- x[jr] - x[ir] - carry on 12
-// doubles that are exact multiples of 2^-48; 12 numbers are handed out per
-// update.
+// numbers that are exact multiples of 2^-48 in [0, 1); 12 numbers are handed
+// out per update.  The numbers are held as their 48-bit integer numerators:
+// every step of the double-precision recurrence is exact, so the integer form
+// gives bit-identical output, and the borrow comes from the sign bit instead
+// of a data-dependent branch.
 // ---------------------------------------------------------------------------
 namespace {
 struct Ranlux48 {
-    double x[12];
-    double carry = 0, unit = 0;
+    int64_t x[12];   // numerators of the state, x / 2^48 in [0, 1)
+    int64_t carry = 0; // 0 or 1 (units of 2^-48)
     int ir = 0, jr = 0, ir_old = 0;
     int64_t pr = 0;
     bool ready = false;
@@ -91,19 +94,18 @@ struct Ranlux48 {
         if (level != 1 && level != 2) return -1;
         if (s <= 0 || s > 2147483647LL) return -2;
         pr = level == 1 ? 202 : 397;
-        unit = std::ldexp(1.0, -48);
         int bit[31];
         for (int k = 0; k < 31; ++k) bit[k] = (int)((s >> k) & 1);
         int a = 0, b = 18;
         for (int k = 0; k < 12; ++k) {
-            double acc = 0;
+            int64_t acc = 0;
             for (int l = 0; l < 48; ++l) {
-                acc = 2.0 * acc + (double)(1 - bit[a]);
+                acc = 2 * acc + (1 - bit[a]);
                 bit[a] ^= bit[b];
                 a = a == 30 ? 0 : a + 1;
                 b = b == 30 ? 0 : b + 1;
             }
-            x[k] = unit * acc;
+            x[k] = acc;
         }
         carry = 0;
         ir = 11;
@@ -113,23 +115,51 @@ struct Ranlux48 {
         return 0;
     }
 
+    // one step of the recurrence at (i, j); returns the new carry
+    static inline int64_t step(int64_t *x, int i, int j, int64_t c)
+    {
+        constexpr int64_t ONE = int64_t(1) << 48;
+        const int64_t y = x[j] - x[i] - c;
+        const int64_t borrow = y >> 63; // -1 if y < 0, else 0
+        x[i] = y + (borrow & ONE);
+        return borrow & 1;
+    }
+
     void refresh()
     {
-        for (int64_t k = 0; k < pr; ++k) {
-            double y = x[jr] - x[ir] - carry;
-            carry = y < 0 ? unit : 0.0;
-            x[ir] = y < 0 ? y + 1.0 : y;
-            ir = ir == 11 ? 0 : ir + 1;
-            jr = jr == 11 ? 0 : jr + 1;
+        int64_t c = carry;
+        int i = ir, j = jr;
+        int64_t k = 0;
+        // single steps up to i == 0, then whole turns of the 12 positions
+        // with constant indices (j - i = 8 mod 12 throughout), then the rest
+        for (; k < pr && i != 0; ++k) {
+            c = step(x, i, j, c);
+            i = i == 11 ? 0 : i + 1;
+            j = j == 11 ? 0 : j + 1;
         }
+        if (i == 0 && j == 8) {
+            for (; k + 12 <= pr; k += 12) {
+#pragma unroll
+                for (int q = 0; q < 12; ++q) c = step(x, q, (q + 8) % 12, c);
+            }
+        }
+        for (; k < pr; ++k) {
+            c = step(x, i, j, c);
+            i = i == 11 ? 0 : i + 1;
+            j = j == 11 ? 0 : j + 1;
+        }
+        carry = c;
+        ir = i;
+        jr = j;
         ir_old = ir;
     }
 
     double next()
     {
+        constexpr double UNIT = 1.0 / 281474976710656.0; // 2^-48, exact scaling
         ir = ir == 11 ? 0 : ir + 1;
         if (ir == ir_old) refresh();
-        return x[ir];
+        return (double)x[ir] * UNIT;
     }
 };
 Ranlux48 g_rng;

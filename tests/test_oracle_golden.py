@@ -33,6 +33,27 @@ def test_product_rng_stream_bit_exact(gold, seed):
     assert np.array_equal(nrq.random_uniform(3000), gold[f"ranlxd_l2_s{seed}"])
 
 
+@pytest.mark.parametrize("seed", [2, 987654321])
+def test_product_rng_long_stream_in_uneven_draws(oracle, seed):
+    """200,000 numbers (about 6,600 refreshes of the 12-number state) drawn in
+    chunks of uneven length equal the oracle's stream, itself pinned by the
+    golden vectors above; the library keeps the state as 48-bit integer
+    numerators, so this checks that form over many carries and wraps."""
+    import nrqcd_amd as nrq
+
+    nrq.random_init(seed)
+    oracle.ranlxd_init(2, seed)
+    sizes = [1, 11, 12, 13, 5, 24, 25, 7, 100, 997]
+    got, n = [], 0
+    k = 0
+    while n < 200000:
+        m = min(sizes[k % len(sizes)], 200000 - n)
+        got.append(nrq.random_uniform(m))
+        n += m
+        k += 1
+    assert np.array_equal(np.concatenate(got), oracle.ranlxd(200000))
+
+
 def test_product_rng_rejects_bad_seed():
     import nrqcd_amd as nrq
 
