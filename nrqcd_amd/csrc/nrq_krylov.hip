@@ -173,11 +173,12 @@ __device__ __forceinline__ void chunk_partials(double (&v)[NC], int r0, int nf, 
 // stencil kernels (grid: lf_map with one work item per chunk of NC slots)
 // ---------------------------------------------------------------------------
 
-// out_r = in_r + beta_r K in_r for the active slots.
+// out_r = in_r + beta_r K in_r for the active slots; with gco (not null)
+// instead out_r = -beta_r gco_r K in_r.
 template <int NC, typename RL, typename T>
 __global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u, slots<T> in, slots<T> out,
                                                  const cplx<double> *__restrict__ beta, const int *__restrict__ active,
-                                                 int nf, geom g, lf_map mp)
+                                                 int nf, geom g, lf_map mp, const cplx<double> *__restrict__ gco)
 {
     lsite p;
     int chunk;
@@ -197,8 +198,18 @@ __global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u,
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         if (!on[q]) continue;
-        const cplx<T> bt = to<T>(beta[r0 + q]);
         cplx<T> *o = out.p[r0 + q] + s;
+        if (gco) {
+            const cplx<double> b = beta[r0 + q], gc = gco[r0 + q];
+            const cplx<T> cf = to<T>(cplx<double>{-(b.re * gc.re - b.im * gc.im), -(b.re * gc.im + b.im * gc.re)});
+#pragma unroll
+            for (int c = 0; c < 3; ++c) {
+                const cplx<T> v0 = ld(q, (int64_t)c * g.vol + s);
+                store_nt(o + (int64_t)c * g.vol, cf * (dsum * v0 - T(0.5) * hop[q][c]));
+            }
+            continue;
+        }
+        const cplx<T> bt = to<T>(beta[r0 + q]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
             const cplx<T> v0 = ld(q, (int64_t)c * g.vol + s);
@@ -211,12 +222,14 @@ __global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u,
 //   y_r = psi_r + alpha_r K psi_r                     (S psi, nrqcd.f90:1112)
 //   x_r = psi_r + g_r K psi_r  (guess != 0)  or  x_r = 0
 // with g_r = alpha_r - beta_r the first-order expansion of
-// (1 + beta K)^-1 (1 + alpha K).  Partial ||y_r||^2 per site block.
+// (1 + beta K)^-1 (1 + alpha K).  Partial ||y_r||^2 per site block.  With wk,
+// K psi_r also goes to kout in FP32: the guess's residual is exactly
+// y - (1 + beta K) x = -beta g K^2 psi.
 template <int NC, typename RL>
 __global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, slots<double> psi, slots<double> y,
                                                 slots<double> x, const cplx<double> *__restrict__ alpha,
                                                 const cplx<double> *__restrict__ gco, int guess, int nf, geom g, lf_map mp,
-                                                double *__restrict__ partial)
+                                                double *__restrict__ partial, slots<float> kout, int wk)
 {
     lsite p = {0, 0, 0, 0};
     int64_t t = 0;
@@ -249,6 +262,7 @@ __global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, 
                 const cplx<double> yv = v0 + a * kf;
                 store_nt(y.p[r] + at, yv);
                 store_nt(x.p[r] + at, guess ? v0 + gc * kf : cplx<double>{0.0, 0.0});
+                if (wk) store_nt(kout.p[r] + at, cplx<float>{(float)kf.re, (float)kf.im});
                 nrm[q] += nrm2(yv);
             }
         }
@@ -745,12 +759,13 @@ static int nrm_to_host()
 
 // out_r = in_r + beta_r K in_r for the active slots
 template <typename RL, typename T>
-static void launch_stencil(const cplx<RL> *u, const slots<T> &in, const slots<T> &out, int nf)
+static void launch_stencil(const cplx<RL> *u, const slots<T> &in, const slots<T> &out, int nf,
+                           const cplx<double> *gco = nullptr)
 {
     const int nc = nc_of<T>();
     const lf_map mp = cn_map(nf, nc);
     NC_DISPATCH(nc, hipLaunchKernelGGL((k_stencil<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, in, out,
-                                       g_ws.beta, g_ws.active, nf, lat().g, mp));
+                                       g_ws.beta, g_ws.active, nf, lat().g, mp, gco));
 }
 
 // One Chebyshev step (k_cheb) for the active slots of R
@@ -781,12 +796,12 @@ static int64_t launch_resid(const cplx<RL> *u, int R, const slots<double> &x, co
 // to g_ws.spart.  Returns the partials per slot.
 template <typename RL>
 static int64_t launch_rhs(const cplx<RL> *u, int R, const slots<double> &psi, const slots<double> &y,
-                          const slots<double> &x, int guess)
+                          const slots<double> &x, int guess, const slots<float> &kout, int wk)
 {
     const int nc = shape().nc64;
     const lf_map mp = cn_map(R, nc);
     NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, psi, y, x,
-                                       g_ws.cpar, g_ws.cpar + NSLOTS, guess, R, lat().g, mp, g_ws.spart));
+                                       g_ws.cpar, g_ws.cpar + NSLOTS, guess, R, lat().g, mp, g_ws.spart, kout, wk));
     return mp.nsb;
 }
 
@@ -1158,7 +1173,17 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
         l32 = (const cplx<float> *)u;
     }
     const slots<double> tpsi = psi_slots(R), tx = ws_slots(g_ws.x, R);
-    if (int e = cn_resid<RL>(u, R, tx, g_ws.y32)) return e;
+    if (p.extrapolate) {
+        // the guess's residual -beta g K^2 psi from K psi (FP32, left in t32
+        // by the right-hand-side pass): one FP32 stencil pass instead of an
+        // FP64 residual pass; the FP64 true residual after the correction
+        // still decides
+        launch_stencil<float, float>(l32, ws_slots((const cplx<float> *)g_ws.t32, R), ws_slots(g_ws.y32, R), R,
+                                     g_ws.cpar + NSLOTS);
+        NRQ_HIP(hipGetLastError());
+    } else if (int e = cn_resid<RL>(u, R, tx, g_ws.y32)) {
+        return e;
+    }
     auto correct = [&](int k) -> int {
         cplx<float> *d = nullptr;
         const slots<double> &from = *in_psi ? tpsi : tx;
@@ -1273,7 +1298,9 @@ static int cn_step_t(const cplx<RL> *u, int R, const std::vector<zc> &alpha, con
     const bool guess = p.extrapolate && p.kind != KRYLOV_GMRES_FP64;
     // right-hand sides and guesses; ||y||^2 to nrm[32..)
     const slots<double> tpsi = psi_slots(R), ty = ws_slots(g_ws.y, R), tx = ws_slots(g_ws.x, R);
-    const int64_t nsb = launch_rhs<RL>(u, R, tpsi, ty, tx, guess ? 1 : 0);
+    // mixed Chebyshev from the guess takes K psi (FP32) for its first correction
+    const int wk = guess && p.kind == KRYLOV_CHEB_MIXED ? 1 : 0;
+    const int64_t nsb = launch_rhs<RL>(u, R, tpsi, ty, tx, guess ? 1 : 0, ws_slots(g_ws.t32, R), wk);
     NRQ_HIP(hipGetLastError());
     if (int e = fold_slots(g_ws.spart, nsb, R, nullptr, g_ws.nrm + workspace::NRM_Y)) return e;
     cplx<double> *xsol = g_ws.x;
