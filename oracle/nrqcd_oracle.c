@@ -27,8 +27,9 @@
  *   CN step, norms, correlators   free-field closed form in momentum space,
  *                                 Hermiticity by dense comparison, gauge
  *                                 covariance, norm conservation, reversibility
- *   polarisation vectors          parity unpinned beyond the orthogonality
- *                                 relations the construction guarantees
+ *   polarisation vectors          cross-product form (eps1 = z x p / |p12|
+ *                                 or -x x p / |p23|, eps2 = conj(eps1) x p^),
+ *                                 transversality, norms, orthogonality
  */
 #include "nrqcd_oracle.h"
 #include <math.h>

@@ -65,6 +65,38 @@ def test_polarisation_vectors_transverse_and_normalised(oracle):
     assert np.all(oracle.polarisation(oracle.momentum(1)) == 0)
 
 
+def test_polarisation_vectors_cross_product_form(oracle):
+    """The polarisation vectors of lattice.f90:1120-1176 (after Kaspar and
+    Hebenstreit, arXiv:1403.4849) by a second route, vector algebra: with
+    p^ = p / |p|, eps1 = z^ x p / |(p1, p2)| when |p1| > 0, else
+    eps1 = -x^ x p / |(p2, p3)|; eps2 = conj(eps1) x p^ and eps3 = p^.  This
+    fixes what transversality and norms leave open (signs, where the complex
+    conjugate sits, the order of the two transverse vectors, the branch)."""
+    rng = np.random.default_rng(5)
+    xh, zh = np.array([1, 0, 0], complex), np.array([0, 0, 1], complex)
+    oracle.lattice_init((6, 5, 4), (0.1, 1.0, 0.9, 1.1))
+    cases = [oracle.momentum(i) for i in range(2, oracle.vol + 1)]  # lattice momenta, both branches
+    cases += [rng.normal(size=3) + 1j * rng.normal(size=3) for _ in range(50)]
+    cases += [np.array([0, 0.3 - 0.2j, 1.1 + 0.4j]), np.array([0, 0, 0.7j]), np.array([0, -0.5, 0]), np.array([2.0, 0, 0])]
+    branches = set()
+    for p in cases:
+        if np.linalg.norm(np.abs(p)) < 1e-12:
+            continue
+        ph = p / np.linalg.norm(np.abs(p))
+        if abs(p[0]) > 1e-12:
+            e1 = np.cross(zh, p) / np.linalg.norm(np.abs(p[:2]))
+            branches.add(1)
+        else:
+            e1 = -np.cross(xh, p) / np.linalg.norm(np.abs(p[1:]))
+            branches.add(2)
+        e2 = np.cross(np.conj(e1), ph)
+        eps = oracle.polarisation(p)
+        assert np.allclose(eps[:, 0], e1, rtol=0, atol=1e-14), p
+        assert np.allclose(eps[:, 1], e2, rtol=0, atol=1e-14), p
+        assert np.allclose(eps[:, 2], ph, rtol=0, atol=1e-15), p
+    assert branches == {1, 2}
+
+
 def _algebra_from_link(U, T):
     """alpha with U = exp(i alpha^a T^a) (principal branch)."""
     w, V = np.linalg.eig(U)
