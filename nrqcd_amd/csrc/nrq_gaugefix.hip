@@ -155,16 +155,14 @@ __global__ void k_gf_accelerate(cplx<double> *__restrict__ div, geom g, double p
     for (int e = 0; e < 9; ++e) div[(int64_t)e * g.vol + s] = f * div[(int64_t)e * g.vol + s];
 }
 
-// g(x) = exp(Delta(x)) in place: exp(iQ) with Q = Hermitian part of -i Delta,
-// its trace split off as a phase (the traceless part goes through the
+// g = exp(Delta) at site s: exp(iQ) with Q = Hermitian part of -i Delta, its
+// trace split off as a phase (the traceless part goes through the
 // Cayley-Hamilton exponential).
-__global__ void k_gf_exp(cplx<double> *__restrict__ div, geom g)
+__device__ __forceinline__ mat3<double> gf_group(const cplx<double> *__restrict__ div, int64_t vol, int64_t s)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (s >= g.vol) return;
     mat3<double> d;
 #pragma unroll
-    for (int e = 0; e < 9; ++e) d.m[e] = div[(int64_t)e * g.vol + s];
+    for (int e = 0; e < 9; ++e) d.m[e] = div[(int64_t)e * vol + s];
     mat3<double> q;
 #pragma unroll
     for (int r = 0; r < 3; ++r)
@@ -178,29 +176,30 @@ __global__ void k_gf_exp(cplx<double> *__restrict__ div, geom g)
     q.m[0].re -= t;
     q.m[4].re -= t;
     q.m[8].re -= t;
-    const mat3<double> x = exp_i_hermitian(q);
+    mat3<double> x = exp_i_hermitian(q);
     double sn, cs;
     sincos(t, &sn, &cs);
     const cplx<double> ph = {cs, sn};
 #pragma unroll
-    for (int e = 0; e < 9; ++e) div[(int64_t)e * g.vol + s] = ph * x.m[e];
+    for (int e = 0; e < 9; ++e) x.m[e] = ph * x.m[e];
+    return x;
 }
 
-// U_i(x) <- g(x) U_i(x) g(x + i)^+ for the three directions at x
+// U_i(x) <- g(x) U_i(x) g(x + i)^+ for the three directions at x, with
+// g = exp(Delta) formed here from Delta (gf_group) for x and its three
+// forward neighbours, so no separate pass writes g: the exponentials are
+// FP64 arithmetic inside a memory-bound kernel.
 template <typename R>
-__global__ __launch_bounds__(GF_THREADS) void k_gf_apply(cplx<R> *__restrict__ u, const cplx<double> *__restrict__ gt, geom g)
+__global__ __launch_bounds__(GF_THREADS) void k_gf_apply(cplx<R> *__restrict__ u, const cplx<double> *__restrict__ delta,
+                                                         geom g)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (s >= g.vol) return;
     const lsite p = make_site(g, s);
-    mat3<double> gx;
-#pragma unroll
-    for (int e = 0; e < 9; ++e) gx.m[e] = gt[(int64_t)e * g.vol + s];
+    const mat3<double> gx = gf_group(delta, g.vol, s);
     for (int i = 0; i < 3; ++i) {
         const int64_t sp = site_up(g, p, i).s;
-        mat3<double> gy;
-#pragma unroll
-        for (int e = 0; e < 9; ++e) gy.m[e] = gt[(int64_t)e * g.vol + sp];
+        const mat3<double> gy = gf_group(delta, g.vol, sp);
         const mat3<double> v = mul_nd(mul(gx, load_link_d(u, g.vol, i, s)), gy);
         cplx<R> *b = u + (int64_t)i * 9 * g.vol;
 #pragma unroll
@@ -372,7 +371,6 @@ static int gaugefix_t(nrq_gauge *h, double tol, double alpha, int64_t maxit, int
         hipLaunchKernelGGL(k_gf_accelerate, dim3(nblk), dim3(GF_THREADS), 0, stream(), b.fld, g, pmax, fwd * bwd * alpha);
         fr = hipfftExecZ2Z(plan, (hipfftDoubleComplex *)b.fld, (hipfftDoubleComplex *)b.fld, HIPFFT_BACKWARD);
         if (fr != HIPFFT_SUCCESS) return gf_fft_fail(fr, "exec (backward)");
-        hipLaunchKernelGGL(k_gf_exp, dim3(nblk), dim3(GF_THREADS), 0, stream(), b.fld, g);
         hipLaunchKernelGGL(k_gf_apply<R>, dim3(nblk), dim3(GF_THREADS), 0, stream(), u, b.fld, g);
         NRQ_HIP(hipGetLastError());
         ++done;
