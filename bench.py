@@ -7,9 +7,10 @@ sources at lattice index 1, Ma = 2, c = (1, 0, 0, 0), every CN column solve
 to an FP64 true relative residual of 1e-10.
 
 A "step" is one leapfrog step of the whole lattice (3V link-updates).  The
-timed region runs `--steps` leapfrog steps back to back on the library's
-stream with the inputs resident in HBM; CN steps are timed separately for
-each solver: the plain FP64 GMRES(30) (PETSc's algorithm) and the fast ones.
+timed region runs `--steps` leapfrog steps on the library's stream with the
+inputs resident in HBM, and reads energy, Gauss law and plaquette every 10
+steps as the workload logs them; CN steps are timed separately for each
+solver: the plain FP64 GMRES(30) (PETSc's algorithm) and the fast ones.
 
 Prints one JSON line.  Options: --gpus N (only 1), --steps K, --warmup W,
 --size L, --cn-steps C, --no-cpu (skip the CPU baseline).
@@ -27,6 +28,14 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 METRIC = "link-updates/s & CN steps/s at FP64 rel.res 1e-10: learned+mixed vs plain"
+LOG_EVERY = 10  # steps between observables (energy, Gauss law, plaquette)
+# Algorithmic HBM bytes per site of the leapfrog launches, FP64: the fused
+# E + link pass reads and writes U and E of the three directions once; an
+# Update of n steps is one links-only pass, n - 1 fused passes and one E-only
+# pass (which leaves U and E at the same time for the observables).
+FUSED_BYTES = 1248.0        # U 432 + E 192, read and written
+LINKS_ONLY_BYTES = 1056.0   # U read and written, E read
+E_ONLY_BYTES = 816.0        # U read, E read and written
 HBM_PEAK_GBS = 8000.0
 # HBM bytes per fused leapfrog launch measured with rocprofv3 (FETCH_SIZE and
 # WRITE_SIZE, each calibrated on the copy kernel), kept with the profile
@@ -103,22 +112,44 @@ def main() -> None:
     obs0 = g.observables()
 
     # ---- leapfrog ------------------------------------------------------
+    # The workload logs energy, Gauss law and plaquette every 10 steps: the
+    # timed region runs the steps in chunks of (at most) 10, each followed by
+    # a logged observables computation that stays on the GPU (read once at
+    # the end).  HIP events around each chunk's Update give the leapfrog
+    # kernel time inside the same region, for the roofline; a pair of event
+    # slots is read back PAIRS chunks after it was recorded, so the host
+    # never drains the queue.
+    PAIRS = 7  # event slots 2..15
     g.Update(1.0, args.warmup)
     nrq.synchronize()
     nrq.event_record(0)
     t0 = time.perf_counter()
-    g.Update(1.0, args.steps)
+    lf_ms, lf_bytes, done, chunk = 0.0, 0.0, 0, 0
+    while done < args.steps:
+        n = min(LOG_EVERY, args.steps - done)
+        k = chunk % PAIRS
+        if chunk >= PAIRS:
+            lf_ms += nrq.event_elapsed_ms(2 + 2 * k, 3 + 2 * k)
+        nrq.event_record(2 + 2 * k)
+        g.Update(1.0, n)
+        nrq.event_record(3 + 2 * k)
+        g.log_observables()
+        lf_bytes += V * (FUSED_BYTES * (n - 1) + LINKS_ONLY_BYTES + E_ONLY_BYTES)
+        done += n
+        chunk += 1
     nrq.event_record(1)
     ms = nrq.event_elapsed_ms(0, 1)
     wall = time.perf_counter() - t0
-    obs1 = g.observables()
+    for c in range(max(0, chunk - PAIRS), chunk):
+        k = c % PAIRS
+        lf_ms += nrq.event_elapsed_ms(2 + 2 * k, 3 + 2 * k)
+    log = g.read_log()
+    obs1 = dict(zip(("energy", "gauss_ref", "gauss_max", "plaquette"), log[-1]))
     shape = g.launch_shape()
     lu_per_s = 3 * V * args.steps / (ms * 1e-3)
     ms_step = ms / args.steps
-    # dominant kernel: the fused E+link pass (steps-1 launches, plus one
-    # links-only and one E-only launch at the ends of each Update call)
-    bytes_per_site = 1248.0  # U, E read once and written once, FP64
-    achieved = bytes_per_site * V / (ms_step * 1e-3) / 1e9
+    # the leapfrog launches' algorithmic bytes over their event time
+    achieved = lf_bytes / (lf_ms * 1e-3) / 1e9
 
     # ---- CN ----------------------------------------------------------------
     nrq.nrqcd_init(2.0, [1, 0, 0, 0], 1.0, 1e-10)
@@ -180,6 +211,7 @@ def main() -> None:
         "cn_steps_per_s_plain": cn["plain_fp64_gmres"]["cn_steps_per_s"],
         "cn": cn,
         "cn_speedup_vs_plain": {k: v["cn_steps_per_s"] / cn["plain_fp64_gmres"]["cn_steps_per_s"] for k, v in cn.items()},
+        "logged_rows": int(log.shape[0]),
         "energy_drift_rel": (obs1["energy"] - obs0["energy"]) / obs0["energy"],
         "gauss_max_drift": obs1["gauss_max"] - obs0["gauss_max"],
         "wall_s_leapfrog": wall,
@@ -193,7 +225,9 @@ def main() -> None:
             "traffic": _measured_traffic(L, shape),
             "kernel": "k_fused<double> (E update + next link update per (site, dir))",
             "launch_shape": shape,
-            "bytes_per_launch": bytes_per_site * V,
+            "bytes_per_launch": FUSED_BYTES * V,
+            "achieved_over": "all leapfrog launches of the timed region (fused, links-only, E-only)",
+            "leapfrog_ms_per_step": lf_ms / args.steps,
         },
         "cpu_baseline": cpu,
     }

@@ -181,6 +181,20 @@ int nrq_gauge_mean_plaquette(const nrq_gauge *h, double *plaq);
 /* All four observables in one pass: out = {energy, gauss_ref, gauss_max,
  * mean_plaquette}. */
 int nrq_gauge_observables(const nrq_gauge *h, double out[4]);
+/* Observables log kept on the device.  nrq_gauge_log_observables enqueues
+ * the computation of the four observables above for the current
+ * configuration and appends them to the handle's log without waiting for the
+ * GPU, so a driver that logs every n steps (the reference's drivers compute
+ * and write energy, Gauss law and plaquette at every logged step,
+ * src/main.f90:2609-2620 with gaugeconfiguration_su3.f90:1480-1531,
+ * 1722-1737, 1769-1817) keeps the leapfrog queue full.  nrq_gauge_log_read
+ * waits for the GPU and moves the oldest min(held, max_rows) rows to out (4
+ * doubles per row, as nrq_gauge_observables), sets *rows and drops them from
+ * the log.  The log grows as needed (device memory, 40 B per row); it is not
+ * copied by nrq_gauge_copy and is freed with the handle.  No reference
+ * counterpart beyond the quantities themselves. */
+int nrq_gauge_log_observables(nrq_gauge *h);
+int nrq_gauge_log_read(nrq_gauge *h, double *out, int64_t max_rows, int64_t *rows);
 /* GaugeConfiguration%GetPlaquette_G(i, j, LatticeIndex) for spatial i, j,
  * gaugeconfiguration_su3.f90:1591-1615 (3x3 complex column-major) */
 int nrq_gauge_plaquette(const nrq_gauge *h, int i, int j, int64_t idx, double P[18]);

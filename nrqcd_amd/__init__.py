@@ -124,6 +124,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_gauge_gauss_max": (I, [P, P]),
         "nrq_gauge_mean_plaquette": (I, [P, P]),
         "nrq_gauge_observables": (I, [P, P]),
+        "nrq_gauge_log_observables": (I, [P]),
+        "nrq_gauge_log_read": (I, [P, P, I64, P]),
         "nrq_gauge_plaquette": (I, [P, I, I, I64, P]),
         "nrq_gauge_reunitarise": (I, [P]),
         "nrq_gauge_set_reunitarise_interval": (I, [P, I64]),
@@ -419,6 +421,27 @@ class GaugeConfiguration:
         out = np.zeros(4)
         _check(library().nrq_gauge_observables(self._h, _ptr(out)))
         return {"energy": out[0], "gauss_ref": out[1], "gauss_max": out[2], "plaquette": out[3]}
+
+    def log_observables(self) -> None:
+        """Append the observables of the current configuration to the
+        device-side log without waiting for the GPU."""
+        _check(library().nrq_gauge_log_observables(self._h))
+
+    def read_log(self, max_rows: int | None = None) -> np.ndarray:
+        """Oldest logged rows (energy, gauss_ref, gauss_max, plaquette),
+        removed from the log (all of them by default); waits for the GPU."""
+        parts, left = [], max_rows
+        while left is None or left > 0:
+            cap = 4096 if left is None else min(int(left), 4096)
+            out = np.zeros((cap, 4))
+            rows = ctypes.c_int64()
+            _check(library().nrq_gauge_log_read(self._h, _ptr(out), cap, ctypes.byref(rows)))
+            parts.append(out[: rows.value])
+            if left is not None:
+                left -= rows.value
+            if rows.value < cap:
+                break
+        return np.concatenate(parts) if parts else np.zeros((0, 4))
 
     def GetEnergy(self) -> float:
         return self.observables()["energy"]

@@ -256,18 +256,29 @@ __device__ __forceinline__ double wave_max(double v)
 constexpr int NOBS = 5; // E-field energy, magnetic energy, sum Re Tr P/3, gauss_ref sum, gauss max
 
 // Per-site observables in FP64 (gaugeconfiguration_su3.f90:1480-1531,
-// 724-813, 1769-1817): one pass computes everything; partials[q][block].
+// 724-813, 1769-1817), partials[q][block].  A block is OBS_SITES consecutive
+// sites times the three directions: thread (lane, i) takes direction i of
+// site base + lane, i.e. its electric energy and reference Gauss term, its
+// part of G(x) and one plaquette plane ((0,1), (1,2), (0,2) for i = 0, 1, 2);
+// the three parts of G(x) meet in LDS and are added in direction order
+// before the maximum is taken.  One thread per site would hold three
+// directions' links at once (over 256 registers, one wave per SIMD).
+constexpr int OBS_SITES = 64;
+constexpr int OBS_THREADS = 3 * OBS_SITES;
 template <typename R>
-__global__ __launch_bounds__(REDUCE_THREADS) void k_obs(const cplx<R> *__restrict__ u, const R *__restrict__ ef, geom g,
-                                                        double *__restrict__ partials)
+__global__ __launch_bounds__(OBS_THREADS) void k_obs(const cplx<R> *__restrict__ u, const R *__restrict__ ef, geom g,
+                                                     double *__restrict__ partials)
 {
     double acc[NOBS] = {0, 0, 0, 0, 0};
     const double a0 = g.a[0];
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < g.vol; s += (int64_t)gridDim.x * blockDim.x) {
-        const lsite p = make_site(g, s);
-        mat3<double> gsum;
-        set_zero(gsum);
-        for (int i = 0; i < 3; ++i) {
+    const int lane = threadIdx.x & 63, i = threadIdx.x >> 6;
+    __shared__ cplx<double> gpart[3][9][OBS_SITES];
+    const int pi = i == 2 ? 0 : i, pj = i == 0 ? 1 : 2; // plaquette plane of this thread
+    for (int64_t base = (int64_t)blockIdx.x * OBS_SITES; base < g.vol; base += (int64_t)gridDim.x * OBS_SITES) {
+        const int64_t s = base + lane;
+        const bool in = s < g.vol;
+        if (in) {
+            const lsite p = make_site(g, s);
             const double ai = g.a[i + 1];
             double ed[8], en[8];
             const int64_t sm = site_dn(g, p, i).s;
@@ -298,34 +309,40 @@ __global__ __launch_bounds__(REDUCE_THREADS) void k_obs(const cplx<R> *__restric
                 two_im_trace_gen(d, tt);
                 for (int a = 0; a < 8; ++a) acc[3] += fabs(tt[a]);
             }
-            for (int m = 0; m < 9; ++m) gsum.m[m] = gsum.m[m] + (1.0 / ai) * dm.m[m];
-            // magnetic part: plaquettes P_ij, j > i
-            for (int j = i + 1; j < 3; ++j) {
-                const int64_t xi = site_up(g, p, i).s, xj = site_up(g, p, j).s;
-                mat3<double> l1, l2, l3, l4;
-                {
-                    mat3<R> a1 = load_link(u, g.vol, i, s), a2 = load_link(u, g.vol, j, xi);
-                    mat3<R> a3 = load_link(u, g.vol, i, xj), a4 = load_link(u, g.vol, j, s);
-                    for (int m = 0; m < 9; ++m) {
-                        l1.m[m] = {(double)a1.m[m].re, (double)a1.m[m].im};
-                        l2.m[m] = {(double)a2.m[m].re, (double)a2.m[m].im};
-                        l3.m[m] = {(double)a3.m[m].re, (double)a3.m[m].im};
-                        l4.m[m] = {(double)a4.m[m].re, (double)a4.m[m].im};
-                    }
+            for (int m = 0; m < 9; ++m) gpart[i][m][lane] = (1.0 / ai) * dm.m[m];
+            // magnetic part: plaquette P_{pi pj}
+            const int64_t xi = site_up(g, p, pi).s, xj = site_up(g, p, pj).s;
+            mat3<double> l1, l2, l3, l4;
+            {
+                mat3<R> a1 = load_link(u, g.vol, pi, s), a2 = load_link(u, g.vol, pj, xi);
+                mat3<R> a3 = load_link(u, g.vol, pi, xj), a4 = load_link(u, g.vol, pj, s);
+                for (int m = 0; m < 9; ++m) {
+                    l1.m[m] = {(double)a1.m[m].re, (double)a1.m[m].im};
+                    l2.m[m] = {(double)a2.m[m].re, (double)a2.m[m].im};
+                    l3.m[m] = {(double)a3.m[m].re, (double)a3.m[m].im};
+                    l4.m[m] = {(double)a4.m[m].re, (double)a4.m[m].im};
                 }
-                mat3<double> p = mul_nd(mul_nd(mul(l1, l2), l3), l4);
-                const double tr = trace(p).re;
-                acc[1] += 2.0 * 3.0 * (1.0 - tr / 3.0) / ai / g.a[j + 1];
-                acc[2] += tr / 3.0;
             }
+            mat3<double> pl = mul_nd(mul_nd(mul(l1, l2), l3), l4);
+            const double tr = trace(pl).re;
+            acc[1] += 2.0 * 3.0 * (1.0 - tr / 3.0) / g.a[pi + 1] / g.a[pj + 1];
+            acc[2] += tr / 3.0;
         }
-        for (int m = 0; m < 9; ++m) gsum.m[m] = (1.0 / a0) * gsum.m[m];
-        double tt[8];
-        two_re_trace_gen(gsum, tt);
-        for (int a = 0; a < 8; ++a) acc[4] = fmax(acc[4], fabs(tt[a]));
+        __syncthreads();
+        if (in && i == 0) {
+            mat3<double> gsum;
+            for (int m = 0; m < 9; ++m) {
+                const cplx<double> v = gpart[0][m][lane] + gpart[1][m][lane];
+                gsum.m[m] = (1.0 / a0) * (v + gpart[2][m][lane]);
+            }
+            double tt[8];
+            two_re_trace_gen(gsum, tt);
+            for (int a = 0; a < 8; ++a) acc[4] = fmax(acc[4], fabs(tt[a]));
+        }
+        __syncthreads();
     }
-    __shared__ double red[NOBS][REDUCE_THREADS / 64];
-    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    __shared__ double red[NOBS][OBS_THREADS / 64];
+    const int wv = threadIdx.x >> 6;
     for (int q = 0; q < NOBS; ++q) {
         double v = q == 4 ? wave_max(acc[q]) : wave_sum(acc[q]);
         if (lane == 0) red[q][wv] = v;
@@ -334,21 +351,36 @@ __global__ __launch_bounds__(REDUCE_THREADS) void k_obs(const cplx<R> *__restric
     if (threadIdx.x < NOBS) {
         const int q = threadIdx.x;
         double v = red[q][0];
-        for (int w = 1; w < REDUCE_THREADS / 64; ++w) v = q == 4 ? fmax(v, red[q][w]) : v + red[q][w];
+        for (int w = 1; w < OBS_THREADS / 64; ++w) v = q == 4 ? fmax(v, red[q][w]) : v + red[q][w];
         partials[(int64_t)q * gridDim.x + blockIdx.x] = v;
     }
 }
 
-__global__ void k_fold(const double *__restrict__ partials, int nblocks, int nq, int max_q, double *__restrict__ out)
+// Second stage of the deterministic reductions: block q (REDUCE_THREADS
+// threads) folds the nblocks partials of quantity q (a maximum for
+// q == max_q, else a sum) in a fixed order: strided per-thread sums, then the
+// waves, then the waves' results in order.
+__global__ __launch_bounds__(REDUCE_THREADS) void k_fold(const double *__restrict__ partials, int nblocks, int nq,
+                                                         int max_q, double *__restrict__ out)
 {
-    const int q = threadIdx.x;
+    const int q = blockIdx.x;
     if (q >= nq) return;
-    double v = partials[(int64_t)q * nblocks];
-    for (int b = 1; b < nblocks; ++b) {
+    const bool mx = q == max_q;
+    double v = mx ? -INFINITY : 0.0;
+    for (int b = threadIdx.x; b < nblocks; b += blockDim.x) {
         const double w = partials[(int64_t)q * nblocks + b];
-        v = q == max_q ? fmax(v, w) : v + w;
+        v = mx ? fmax(v, w) : v + w;
     }
-    out[q] = v;
+    v = mx ? wave_max(v) : wave_sum(v);
+    __shared__ double red[REDUCE_THREADS / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = v;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int w = 1; w < (int)(blockDim.x >> 6); ++w) t = mx ? fmax(t, red[w]) : t + red[w];
+        out[q] = t;
+    }
 }
 
 // Projects U onto SU(3) in FP64: normalise row 1, orthonormalise row 2
@@ -441,8 +473,11 @@ static void release(nrq_gauge *h)
     if (h->e) (void)hipFree(h->e);
     if (h->partials) (void)hipFree(h->partials);
     if (h->result) (void)hipHostFree(h->result);
+    if (h->dlog) (void)hipFree(h->dlog);
     h->u[0] = h->u[1] = h->e = h->partials = nullptr;
     h->result = nullptr;
+    h->dlog = nullptr;
+    h->log_cap = h->log_n = 0;
 }
 
 static int allocate(nrq_gauge *h, int precision)
@@ -464,20 +499,37 @@ static int allocate(nrq_gauge *h, int precision)
     return NRQ_OK;
 }
 
-int gauge_obs(const nrq_gauge *h, double out[NOBS])
+// The observable sums of the current configuration into dres (device, NOBS
+// doubles), enqueued on the stream.
+static int gauge_obs_enqueue(const nrq_gauge *h, double *dres)
 {
     double *part = (double *)h->partials;
-    double *dres = part + NOBS * REDUCE_BLOCKS;
     const geom g = lat().g;
     if (h->precision == 64)
-        hipLaunchKernelGGL(k_obs<double>, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, stream(),
+        hipLaunchKernelGGL(k_obs<double>, dim3(REDUCE_BLOCKS), dim3(OBS_THREADS), 0, stream(),
                            (const cplx<double> *)h->u[h->cur], (const double *)h->e, g, part);
     else
-        hipLaunchKernelGGL(k_obs<float>, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, stream(),
+        hipLaunchKernelGGL(k_obs<float>, dim3(REDUCE_BLOCKS), dim3(OBS_THREADS), 0, stream(),
                            (const cplx<float> *)h->u[h->cur], (const float *)h->e, g, part);
     NRQ_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_fold, dim3(1), dim3(64), 0, stream(), part, REDUCE_BLOCKS, NOBS, 4, dres);
+    hipLaunchKernelGGL(k_fold, dim3(NOBS), dim3(REDUCE_THREADS), 0, stream(), part, REDUCE_BLOCKS, NOBS, 4, dres);
     NRQ_HIP(hipGetLastError());
+    return NRQ_OK;
+}
+
+// energy, gauss_ref, gauss_max, plaquette from the raw sums
+static void obs_row(const double *o, int64_t vol, double out[4])
+{
+    out[0] = o[0] + o[1];
+    out[1] = o[3];
+    out[2] = o[4];
+    out[3] = o[2] / (3.0 * (double)vol);
+}
+
+int gauge_obs(const nrq_gauge *h, double out[NOBS])
+{
+    double *dres = (double *)h->partials + NOBS * REDUCE_BLOCKS;
+    if (int r = gauge_obs_enqueue(h, dres)) return r;
     NRQ_HIP(hipMemcpyAsync(h->result, dres, NOBS * sizeof(double), hipMemcpyDeviceToHost, stream()));
     NRQ_HIP(hipStreamSynchronize(stream()));
     for (int q = 0; q < NOBS; ++q) out[q] = h->result[q];
@@ -1037,10 +1089,54 @@ extern "C" int nrq_gauge_observables(const nrq_gauge *h, double out[4])
     if (!out) return fail(NRQ_ERR_ARG, "null argument");
     double o[NOBS];
     if (int r = gauge_obs(h, o)) return r;
-    out[0] = o[0] + o[1];
-    out[1] = o[3];
-    out[2] = o[4];
-    out[3] = o[2] / (3.0 * (double)h->vol);
+    obs_row(o, h->vol, out);
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_log_observables(nrq_gauge *h)
+{
+    NEED_GAUGE(h);
+    if (h->log_n == h->log_cap) {
+        // grow by doubling (at least 64 rows), keeping the rows held
+        const int64_t cap = h->log_cap ? 2 * h->log_cap : 64;
+        double *d = nullptr;
+        if (hipMalloc((void **)&d, sizeof(double) * NOBS * (size_t)cap) != hipSuccess) {
+            (void)hipGetLastError();
+            return fail(NRQ_ERR_ALLOC, "gauge: observables log of %lld rows", (long long)cap);
+        }
+        if (h->log_n)
+            NRQ_HIP(hipMemcpyAsync(d, h->dlog, sizeof(double) * NOBS * (size_t)h->log_n, hipMemcpyDeviceToDevice, stream()));
+        NRQ_HIP(hipStreamSynchronize(stream()));
+        if (h->dlog) (void)hipFree(h->dlog);
+        h->dlog = d;
+        h->log_cap = cap;
+    }
+    if (int r = gauge_obs_enqueue(h, h->dlog + NOBS * h->log_n)) return r;
+    ++h->log_n;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_gauge_log_read(nrq_gauge *h, double *out, int64_t max_rows, int64_t *rows)
+{
+    NEED_GAUGE(h);
+    if (!rows || (max_rows > 0 && !out) || max_rows < 0) return fail(NRQ_ERR_ARG, "null or negative argument");
+    const int64_t n = std::min(h->log_n, max_rows);
+    *rows = n;
+    if (n == 0) return NRQ_OK;
+    std::vector<double> raw((size_t)NOBS * (size_t)n);
+    NRQ_HIP(hipMemcpyAsync(raw.data(), h->dlog, sizeof(double) * raw.size(), hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    for (int64_t r = 0; r < n; ++r) obs_row(raw.data() + NOBS * r, h->vol, out + 4 * r);
+    const int64_t left = h->log_n - n;
+    if (left > 0) {
+        // the rows not returned move to the front (non-overlapping when left <= n; else in n-row pieces)
+        for (int64_t k = 0; k < left; k += n) {
+            const int64_t m = std::min(n, left - k);
+            NRQ_HIP(hipMemcpyAsync(h->dlog + NOBS * k, h->dlog + NOBS * (n + k), sizeof(double) * NOBS * (size_t)m,
+                                   hipMemcpyDeviceToDevice, stream()));
+        }
+    }
+    h->log_n = left;
     return NRQ_OK;
 }
 

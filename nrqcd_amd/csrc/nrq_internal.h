@@ -72,6 +72,8 @@ struct nrq_gauge {
     double *result = nullptr; // pinned host result slots
     int64_t steps_since_reunit = 0;
     int64_t reunit_interval = 0; // reunitarise after this many steps (0: never)
+    double *dlog = nullptr;       // observables log on the device, rows of 5 raw sums
+    int64_t log_cap = 0, log_n = 0; // rows allocated, rows held
 };
 
 struct nrq_heavy {

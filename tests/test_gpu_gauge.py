@@ -112,6 +112,30 @@ def test_time_reversal_on_gpu(nrq, oracle, lattice):
     assert np.max(np.abs(Eg - E)) < 1e-11
 
 
+def test_observables_log_equals_synchronous_observables(nrq, lattice):
+    """The device-side log holds exactly what nrq_gauge_observables returns
+    at the same points (same kernels), across two growths of the log and a
+    partial read."""
+    ext = (6, 5, 4)
+    _say("observables log 6x5x4, 150 rows")
+    lattice(ext, SPACING_DT005)
+    nrq.random_init(3)
+    g = nrq.GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    want, got = [], []
+    for k in range(150):
+        g.Update(1.0, 1 + k % 3)
+        g.log_observables()
+        o = g.observables()
+        want.append([o["energy"], o["gauss_ref"], o["gauss_max"], o["plaquette"]])
+        if k == 40:
+            got.append(g.read_log(7))
+            assert got[-1].shape == (7, 4)
+    got.append(g.read_log())
+    assert np.array_equal(np.concatenate(got), np.array(want))
+    assert g.read_log().shape == (0, 4)
+
+
 def test_observables_match_oracle(nrq, oracle, lattice):
     ext = (8, 6, 4)
     sp = (0.05, 1.0, 0.9, 1.1)
