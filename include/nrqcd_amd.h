@@ -190,9 +190,11 @@ int nrq_gauge_observables(const nrq_gauge *h, double out[4]);
  * 1722-1737, 1769-1817) keeps the leapfrog queue full.  nrq_gauge_log_read
  * waits for the GPU and moves the oldest min(held, max_rows) rows to out (4
  * doubles per row, as nrq_gauge_observables), sets *rows and drops them from
- * the log.  The log grows as needed (device memory, 40 B per row); it is not
- * copied by nrq_gauge_copy and is freed with the handle.  No reference
- * counterpart beyond the quantities themselves. */
+ * the log.  The log grows as needed (device memory, 40 B per row) and is
+ * freed with the handle.  nrq_gauge_copy does not copy it; a copy that has to
+ * reallocate its destination (other precision or lattice) clears the
+ * destination's log.  No reference counterpart beyond the quantities
+ * themselves. */
 int nrq_gauge_log_observables(nrq_gauge *h);
 int nrq_gauge_log_read(nrq_gauge *h, double *out, int64_t max_rows, int64_t *rows);
 /* GaugeConfiguration%GetPlaquette_G(i, j, LatticeIndex) for spatial i, j,
