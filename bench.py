@@ -64,25 +64,34 @@ def _measured_traffic(size: int, shape: dict):
 
 def _cpu_baseline(size: int, seconds: float) -> dict:
     """Oracle leapfrog (OpenMP FP64 C restatement) on a bounded sample of the
-    same workload: the step count is chosen to fill about `seconds`."""
+    same workload, energy, Gauss law and plaquette every LOG_EVERY steps: the
+    number of 10-step chunks is chosen to fill about `seconds`."""
     from oracle.pyoracle import Oracle
 
     orc = Oracle()
     orc.lattice_init((size, size, size), (0.05, 1.0, 1.0, 1.0))
     U, E = orc.box_init(2, 0.5, 1.0, 1.0)
+
+    def chunk():
+        orc.leapfrog(U, E, 1.0, LOG_EVERY)
+        return (orc.energy(U, E), orc.gauss_ref(U, E), orc.gauss_max(U, E), orc.mean_plaquette(U))
+
     t0 = time.perf_counter()
-    orc.leapfrog(U, E, 1.0, 1)  # warm-up and calibration
+    chunk()  # warm-up and calibration
     one = time.perf_counter() - t0
-    steps = max(1, int(seconds / max(one, 1e-6)))
+    chunks = max(1, int(seconds / max(one, 1e-6)))
     t0 = time.perf_counter()
-    orc.leapfrog(U, E, 1.0, steps)
+    for _ in range(chunks):
+        chunk()
     dt = time.perf_counter() - t0
+    steps = chunks * LOG_EVERY
     return {
         "value": 3 * size ** 3 * steps / dt,
         "unit": "link-updates/s",
         "cores": orc.num_threads(),
         "kind": "port",
-        "sample": f"oracle leapfrog FP64, {size}^3, {steps} steps after 1 warm-up step ({dt:.1f} s)",
+        "sample": f"oracle leapfrog FP64 with energy, Gauss law and plaquette every {LOG_EVERY} steps, {size}^3, "
+                  f"{steps} steps after one warm-up chunk ({dt:.1f} s)",
     }
 
 
