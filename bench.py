@@ -106,7 +106,7 @@ def main() -> None:
     ap.add_argument("--cpu-seconds", type=float, default=12.0)
     args = ap.parse_args()
     if args.gpus != 1:
-        print(f"bench.py: this build runs on one GPU only (asked for {args.gpus}); the lattice is not "
+        print(f"bench.py: runs on one GPU only (asked for {args.gpus}); the lattice is not "
               "sharded across GPUs", file=sys.stderr)
         sys.exit(2)
 

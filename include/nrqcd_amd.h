@@ -35,7 +35,7 @@ extern "C" {
 #define NRQ_ERR_ARG (-1)          /* invalid argument */
 #define NRQ_ERR_STATE (-2)        /* module not initialised / already initialised */
 #define NRQ_ERR_HIP (-3)          /* HIP runtime or hipFFT failure */
-#define NRQ_ERR_UNSUPPORTED (-4)  /* feature outside this build (e.g. c2..c4 != 0) */
+#define NRQ_ERR_UNSUPPORTED (-4)  /* unsupported request (unknown solver kind, CNN on an x extent not a multiple of 16) */
 #define NRQ_ERR_NOCONV (-5)       /* Krylov solver hit maxit before the tolerance */
 #define NRQ_ERR_ALLOC (-6)        /* device or host allocation failed */
 

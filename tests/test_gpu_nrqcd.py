@@ -192,8 +192,8 @@ SOLVERS = ["GMRES_MIXED", "CHEBYSHEV_FP64", "CHEBYSHEV_MIXED"]
 @pytest.mark.parametrize("tol", [1e-12, 1e-10])
 def test_fast_solvers_match_oracle(nrq, oracle, cn, tol, kind):
     """Mixed-precision GMRES and the Chebyshev solvers: the FP64 true
-    residual is the judge, so each result agrees with the FP64 oracle to the
-    same bar as the all-FP64 GMRES."""
+    residual decides convergence, so each result agrees with the FP64 oracle
+    to the same bar as the all-FP64 GMRES."""
     ext = (8, 6, 4)
     sp = (0.05, 1.0, 0.9, 1.1)
     _say(f"CN {kind} tol {tol}")
