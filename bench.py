@@ -29,6 +29,14 @@ sys.path.insert(0, str(ROOT))
 
 METRIC = "link-updates/s & CN steps/s at FP64 rel.res 1e-10: learned+mixed vs plain"
 LOG_EVERY = 10  # steps between observables (energy, Gauss law, plaquette)
+# Algorithmic HBM bytes per site of one mixed-Chebyshev CN step with 12 live
+# fields when every column takes the planned 6 Chebyshev steps (two FP32
+# corrections of 4 and 2 steps): right-hand side with y, x0 (FP64) and K psi
+# (FP32) 2448; FP32 link copy 648; K pass 792; Chebyshev launches 792, 1080,
+# 2232 (adds the correction to x0 in FP64) and 1656 (adds the second to psi);
+# two FP64 true-residual passes 2 x 1872.
+CN_MIXED_CHEB_BYTES = 2448.0 + 648 + 792 + 792 + 1080 + 2232 + 1656 + 2 * 1872
+CN_MIXED_CHEB_ITS = 6.0
 # Algorithmic HBM bytes per site of the leapfrog launches, FP64: the fused
 # E + link pass reads and writes U and E of the three directions once; an
 # Update of n steps is one links-only pass, n - 1 fused passes and one E-only
@@ -193,6 +201,13 @@ def main() -> None:
     copy_gbs = nrq.bandwidth_probe(1 << 30, 20)
 
     cpu = None
+    # HBM roofline of the fastest CN step, from its fixed launch sequence
+    mc = cn["mixed_chebyshev"]
+    cn_roof = None
+    if mc["iterations_per_column_solve"] == CN_MIXED_CHEB_ITS:
+        gbs = CN_MIXED_CHEB_BYTES * V / (mc["ms_per_step"] * 1e-3) / 1e9
+        cn_roof = {"solver": "mixed_chebyshev", "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                   "frac": gbs / HBM_PEAK_GBS, "bytes_per_step": CN_MIXED_CHEB_BYTES * V}
     if not args.no_cpu:
         cpu = _cpu_baseline(L, args.cpu_seconds)
 
@@ -220,6 +235,7 @@ def main() -> None:
         "cn_steps_per_s_plain": cn["plain_fp64_gmres"]["cn_steps_per_s"],
         "cn": cn,
         "cn_speedup_vs_plain": {k: v["cn_steps_per_s"] / cn["plain_fp64_gmres"]["cn_steps_per_s"] for k, v in cn.items()},
+        "cn_roofline": cn_roof,
         "logged_rows": int(log.shape[0]),
         "energy_drift_rel": (obs1["energy"] - obs0["energy"]) / obs0["energy"],
         "gauss_max_drift": obs1["gauss_max"] - obs0["gauss_max"],
