@@ -64,6 +64,35 @@ def test_leapfrog_10_steps_fp64(nrq, oracle, lattice, ext):
     assert _rel(Eg, E) < 1e-12
 
 
+def test_bench_workload_64cubed_matches_oracle(nrq, oracle, lattice):
+    """The benchmarked configuration itself: 64^3, spacings (0.05, 1, 1, 1),
+    box IC Qs = 0.5 from ranlxd seed 2.  The GPU's initial condition equals
+    the oracle's (1e-13 absolute), 10 leapfrog steps from the same state
+    agree to 1e-12 relative, and so do the observables the run logs."""
+    ext = (64, 64, 64)
+    _say("bench workload 64^3 seed 2: IC, 10 steps, observables")
+    lattice(ext, SPACING_DT005)
+    oracle.lattice_init(ext, SPACING_DT005)
+    U, E = oracle.box_init(2, 0.5, 1.0, 1.0)
+    nrq.random_init(2)
+    g = nrq.GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    Ug, Eg = g.download()
+    assert np.max(np.abs(Ug - U)) < 1e-13
+    assert np.max(np.abs(Eg - E)) < 1e-13
+    g.upload(U, E)
+    g.Update(1.0, 10)
+    g.log_observables()
+    oracle.leapfrog(U, E, 1.0, 10)
+    Ug, Eg = g.download()
+    assert _rel(Ug, U) < 1e-12
+    assert _rel(Eg, E) < 1e-12
+    row = g.read_log()[0]
+    assert row[0] == pytest.approx(oracle.energy(U, E), rel=1e-12)
+    assert row[2] == pytest.approx(oracle.gauss_max(U, E), rel=1e-12)
+    assert row[3] == pytest.approx(oracle.mean_plaquette(U), rel=1e-12)
+
+
 def test_leapfrog_single_calls_match_batched(nrq, oracle, lattice):
     ext = (8, 8, 8)
     _say("leapfrog single vs batched 8^3")
