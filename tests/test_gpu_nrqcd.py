@@ -173,6 +173,42 @@ def test_cn_norm_conservation_and_reversibility(nrq, cn):
     assert np.max(np.abs(A1 - A0)) < 1e-10
 
 
+def test_bench_workload_64cubed_cn_properties(nrq, cn):
+    """At the benchmarked size (64^3, box IC seed 2 evolved 20 steps, point
+    sources, Ma = 2, FP64 true residual 1e-10) with the fastest solver,
+    mixed-precision Chebyshev: both norms stay 1 to the solver tolerance over
+    3 steps, the steps match the plain FP64 GMRES within the tolerance, and 3
+    backward steps (StepWidth -1, the exact inverse of a CN step) return the
+    point sources."""
+    ext = (64, 64, 64)
+    _say("CN 64^3 mixed Chebyshev: norms, against GMRES, reversibility")
+    cn(ext, (0.05, 1.0, 1.0, 1.0), tol=1e-10)
+    nrq.random_init(2)
+    g = nrq.GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    g.Update(1.0, 20)
+    fields = {}
+    for kind in (nrq.NRQ_SOLVER_GMRES_FP64, nrq.NRQ_SOLVER_CHEBYSHEV_MIXED):
+        nrq.nrqcd_set_solver(kind, 30, 10000)
+        h = nrq.NRQCDField()
+        h.InitSinglePoint(1, 1)
+        for _ in range(3):
+            st = h.Update(g, 2.0, C1)
+            assert st.max_true_relres <= 1e-10
+        nq, na = h.norms()
+        assert abs(nq - 1.0) < 1e-9 and abs(na - 1.0) < 1e-9
+        fields[kind] = (h, h.download())
+    (_, (Qg, Ag)), (h, (Qc, Ac)) = fields[nrq.NRQ_SOLVER_GMRES_FP64], fields[nrq.NRQ_SOLVER_CHEBYSHEV_MIXED]
+    assert np.max(np.abs(Qc - Qg)) < 1e-9 and np.max(np.abs(Ac - Ag)) < 1e-9
+    for _ in range(3):
+        h.Update(g, 2.0, C1, -1.0)
+    Q1, A1 = h.download()
+    h0 = nrq.NRQCDField()
+    h0.InitSinglePoint(1, 1)
+    Q0, A0 = h0.download()
+    assert np.max(np.abs(Q1 - Q0)) < 1e-9 and np.max(np.abs(A1 - A0)) < 1e-9
+
+
 def test_init_single_dof_matches_reference_indexing(nrq, lattice):
     lattice((4, 4, 4), SP)
     h = nrq.NRQCDField()
