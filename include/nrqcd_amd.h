@@ -221,6 +221,22 @@ int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval);
  * counterpart. */
 int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling, int *streaming,
                            int *two_row);
+/* The autotuner's candidate shapes: their number, the parameters of shape k
+ * (as nrq_gauge_launch_shape) and whether it applies to the current lattice.
+ * The tuner times only the shapes whose one fused pass agrees with the
+ * default shape's (1e-11 FP64, 5e-5 FP32; two-row shapes rebuild a row and
+ * agree to rounding) and reports any other on stderr.  No reference
+ * counterpart. */
+int nrq_leapfrog_shape_count(int *n);
+int nrq_leapfrog_shape_info(int k, int *block_sites, int *xcd_remap, int *tiling, int *streaming, int *two_row,
+                            int *applies);
+/* Pins launch shape k (0 .. count-1) for every later leapfrog call, or k = -1
+ * for the autotuner's choice (the default; NRQ_LF_SHAPE=k in the environment
+ * sets the initial value).  A leapfrog call fails with NRQ_ERR_UNSUPPORTED
+ * while a pinned shape does not apply to the lattice.  No reference
+ * counterpart. */
+int nrq_leapfrog_set_shape(int k);
+int nrq_leapfrog_get_shape(int *k);
 /* GaugeConfiguration%CoulombGaugefixing(Tolerance_, alpha_, MaxIterations_),
  * gaugeconfiguration_su3.f90:2269-2497 (the reference's defaults are 1e-8,
  * 0.08 and 100): Davies' Fourier-accelerated gauge fixing of the links.  E is
@@ -270,6 +286,24 @@ int nrq_cn_finalize(void);
 /* Solver selection: kind NRQ_SOLVER_*, GMRES restart length (PETSc default
  * 30) and iteration cap (PETSc default maxits 1e4, nrqcd.f90:803-804). */
 int nrq_cn_set_solver(int kind, int restart, int64_t maxit);
+/* Launch shape of the CN stencil kernels: colour-vector fields per thread
+ * sharing each link load in FP64 (nc64) and FP32 (nc32) arithmetic (1, 2 or
+ * 4), the site-block tiling (0: consecutive sites, 1: z-sweep tiles of
+ * x-rows, 2: z-sweep 16-wide tiles) and the XCD remap (1 or 8).  Every shape
+ * computes the same per-field sums in the same order.  NRQ_CN_SHAPE=
+ * "nc64,nc32,tiling,xcd" in the environment sets the initial value (default
+ * 2,4,0,8).  No reference counterpart. */
+int nrq_cn_set_shape(int nc64, int nc32, int tiling, int xcd_remap);
+int nrq_cn_get_shape(int *nc64, int *nc32, int *tiling, int *xcd_remap);
+/* Column solves that NRQ_SOLVER_FGMRES_CNN handed to plain GMRES since the
+ * library was loaded: the network made the Arnoldi step non-finite, a
+ * restart cycle reduced the residual less than tenfold, or four cycles did
+ * not converge.  The plain solve continues from the iterate reached, so the
+ * answer is the same; only the work changes.  (PETSc's Jacobi preconditioner
+ * scales by the constant diagonal of this operator, which leaves GMRES
+ * unchanged.)  No reference counterpart (nrqcd.f90:786-804 configures
+ * PETSc's preconditioner). */
+int nrq_cn_fallbacks(int64_t *count);
 
 /* NRQCDField() constructor / Destructor, nrqcd.f90:234-260 */
 int nrq_heavy_create(nrq_heavy **h);
@@ -290,7 +324,10 @@ int nrq_heavy_download(const nrq_heavy *h, double *quark, double *antiq);
 /* NRQCDField%Update(GaugeConf, Mass, WilsonCoeffs, [StepWidth]) ->
  * Update_CrankNicholson, nrqcd.f90:1053-1168: rhs = S psi, solve
  * S^dagger psi' = rhs with S = 1 - i (StepWidth a0/2) H, H = -c1 D^2/2M; the
- * antiquark uses -M, conj(c).  Only c1 is supported (c2..c4 must be 0).
+ * antiquark uses -M, conj(c).  With c = (c1, 0, 0, 0) the kinetic operator
+ * H = -c1 D^2/2M is solved by the selected solver; any non-zero c2..c4 adds
+ * the reference's magnetic, E-derivative and spin-orbit terms
+ * (nrqcd.f90:1244-1381), solved by FP64 GMRES on spin-coupled columns.
  * stats may be NULL. */
 int nrq_heavy_update(nrq_heavy *h, const nrq_gauge *g, double mass, const double c[8],
                      double stepwidth, nrq_solve_stats *stats);

@@ -130,6 +130,13 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_gauge_reunitarise": (I, [P]),
         "nrq_gauge_set_reunitarise_interval": (I, [P, I64]),
         "nrq_gauge_launch_shape": (I, [P, P, P, P, P, P]),
+        "nrq_leapfrog_shape_count": (I, [P]),
+        "nrq_leapfrog_shape_info": (I, [I, P, P, P, P, P, P]),
+        "nrq_leapfrog_set_shape": (I, [I]),
+        "nrq_leapfrog_get_shape": (I, [P]),
+        "nrq_cn_set_shape": (I, [I, I, I, I]),
+        "nrq_cn_get_shape": (I, [P, P, P, P]),
+        "nrq_cn_fallbacks": (I, [P]),
         "nrq_gauge_coulomb_gaugefix": (I, [P, D, D, I64, P, P]),
         "nrq_gauge_transverse_correlators": (I, [P, P, P]),
         "nrq_cn_init": (I, [D, P, D, D]),
@@ -412,8 +419,7 @@ class GaugeConfiguration:
         bs, xcd, tiling, nt, rec = (ctypes.c_int() for _ in range(5))
         _check(library().nrq_gauge_launch_shape(self._h, ctypes.byref(bs), ctypes.byref(xcd), ctypes.byref(tiling),
                                                 ctypes.byref(nt), ctypes.byref(rec)))
-        return {"threads_per_block": bs.value, "xcd_remap": xcd.value,
-                "tiling": ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles")[tiling.value],
+        return {"threads_per_block": bs.value, "xcd_remap": xcd.value, "tiling": _TILINGS[tiling.value],
                 "streaming": bool(nt.value), "two_row_neighbours": bool(rec.value)}
 
     # -- observables -----------------------------------------------------
@@ -498,6 +504,54 @@ class GaugeConfiguration:
 # ---------------------------------------------------------------------------
 # nrqcd
 # ---------------------------------------------------------------------------
+_TILINGS = ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles")
+
+
+def leapfrog_shapes() -> list:
+    """The autotuner's candidate launch shapes, with whether each applies to
+    the current lattice."""
+    n = ctypes.c_int()
+    _check(library().nrq_leapfrog_shape_count(ctypes.byref(n)))
+    out = []
+    for k in range(n.value):
+        v = [ctypes.c_int() for _ in range(6)]
+        _check(library().nrq_leapfrog_shape_info(k, *[ctypes.byref(x) for x in v]))
+        out.append({"threads_per_block": v[0].value, "xcd_remap": v[1].value, "tiling": _TILINGS[v[2].value],
+                    "streaming": bool(v[3].value), "two_row_neighbours": bool(v[4].value),
+                    "applies": bool(v[5].value)})
+    return out
+
+
+def leapfrog_set_shape(k: int) -> None:
+    """Pin launch shape k for every later leapfrog call (-1: autotuner)."""
+    _check(library().nrq_leapfrog_set_shape(int(k)))
+
+
+def leapfrog_get_shape() -> int:
+    k = ctypes.c_int()
+    _check(library().nrq_leapfrog_get_shape(ctypes.byref(k)))
+    return k.value
+
+
+def cn_set_shape(nc64: int, nc32: int, tiling: int = 0, xcd_remap: int = 8) -> None:
+    """Launch shape of the CN stencil kernels (fields per thread in FP64 and
+    FP32, site tiling, XCD remap)."""
+    _check(library().nrq_cn_set_shape(nc64, nc32, tiling, xcd_remap))
+
+
+def cn_get_shape() -> tuple:
+    v = [ctypes.c_int() for _ in range(4)]
+    _check(library().nrq_cn_get_shape(*[ctypes.byref(x) for x in v]))
+    return tuple(x.value for x in v)
+
+
+def cn_fallbacks() -> int:
+    """Column solves the learned preconditioner handed to plain GMRES."""
+    n = ctypes.c_int64()
+    _check(library().nrq_cn_fallbacks(ctypes.byref(n)))
+    return n.value
+
+
 def leapfrog_batch(configs, StepWidth: float = 1.0, nsteps: int = 1) -> None:
     """Advance an ensemble of GaugeConfiguration objects (one storage
     precision) by ``nsteps`` leapfrog steps, one launch per pass over all of

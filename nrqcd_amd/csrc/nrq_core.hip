@@ -222,6 +222,7 @@ extern "C" int nrq_lattice_init(const int64_t ext[3], const double spacing[4])
 extern "C" int nrq_lattice_finalize(void)
 {
     if (!lat().initialised) return fail(NRQ_ERR_STATE, "lattice: not initialised");
+    gauge_forget_lattice();
     lat().initialised = false;
     return NRQ_OK;
 }

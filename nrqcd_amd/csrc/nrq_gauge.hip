@@ -556,7 +556,7 @@ constexpr int LF_DEFAULT = 11; // {256, 8, 0, 0, 0}: used without tuning, valid 
 
 struct lf_choice {
     int precision;
-    int64_t vol;
+    int n[3]; // lattice extents the choice was tuned on
     int variant;
 };
 static std::vector<lf_choice> g_lf_choice;
@@ -638,40 +638,115 @@ template <typename R> static lf_batch<R> one(const cplx<R> *uin, cplx<R> *uout, 
     return b;
 }
 
+// Largest |a[k] - b[k]| over n reals, per block into part[blockIdx.x].
+template <typename R>
+__global__ __launch_bounds__(REDUCE_THREADS) void k_maxdiff(const R *__restrict__ a, const R *__restrict__ b, int64_t n,
+                                                            double *__restrict__ part)
+{
+    double m = 0;
+    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+        m = fmax(m, fabs((double)a[k] - (double)b[k]));
+    m = wave_max(m);
+    __shared__ double red[REDUCE_THREADS / 64];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    if (lane == 0) red[wv] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double t = red[0];
+        for (int w = 1; w < REDUCE_THREADS / 64; ++w) t = fmax(t, red[w]);
+        part[blockIdx.x] = t;
+    }
+}
+
+// Agreement a shape must show with LF_DEFAULT after one fused pass from the
+// same state: links (entries of modulus <= 1) to `tol`, E to tol * max(1, |E|).
+// Two-row shapes rebuild the third row of every neighbour link, so they agree
+// to rounding, not bit for bit.
+template <typename R> static double shape_tolerance() { return sizeof(R) == 8 ? 1e-11 : 5e-5; }
+
+// max |a - b| over n reals of type R (device arrays) into *out (host)
+template <typename R> static int max_diff(const R *a, const R *b, int64_t n, double *part, double *out)
+{
+    hipLaunchKernelGGL(k_maxdiff<R>, dim3(REDUCE_BLOCKS), dim3(REDUCE_THREADS), 0, stream(), a, b, n, part);
+    NRQ_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_fold, dim3(1), dim3(REDUCE_THREADS), 0, stream(), part, REDUCE_BLOCKS, 1, 0, part + REDUCE_BLOCKS);
+    NRQ_HIP(hipGetLastError());
+    NRQ_HIP(hipMemcpyAsync(out, part + REDUCE_BLOCKS, sizeof(double), hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    return NRQ_OK;
+}
+
 // Times every launch shape on scratch copies of the configuration (the
-// handle's own state is not touched) and returns the fastest.
+// handle's own state is not touched) and returns the fastest of those whose
+// one-pass result agrees with LF_DEFAULT's (shape_tolerance); a shape that
+// does not agree is reported on stderr and never chosen.
 template <typename R> static int tune_leapfrog(const nrq_gauge *h, R dt, int *best)
 {
     const geom g = lat().g;
-    const size_t lb = (size_t)3 * 9 * (size_t)g.vol * sizeof(cplx<R>), eb = (size_t)3 * 8 * (size_t)g.vol * sizeof(R);
-    cplx<R> *u0 = nullptr, *u1 = nullptr;
-    R *ef = nullptr;
-    *best = 0;
-    if (hipMalloc(&u0, lb) != hipSuccess || hipMalloc(&u1, lb) != hipSuccess || hipMalloc(&ef, eb) != hipSuccess) {
-        if (u0) (void)hipFree(u0);
-        if (u1) (void)hipFree(u1);
+    const int64_t nl = (int64_t)3 * 9 * 2 * g.vol, ne = (int64_t)3 * 8 * g.vol; // reals
+    const size_t lb = (size_t)nl * sizeof(R), eb = (size_t)ne * sizeof(R);
+    *best = LF_DEFAULT;
+    // u0, e0: the state; ua, ea: LF_DEFAULT's pass (then timing scratch); ub, eb: the shape's pass
+    void *buf[6] = {};
+    const size_t sz[6] = {lb, lb, lb, eb, eb, eb};
+    double *part = nullptr;
+    bool ok = hipMalloc(&part, sizeof(double) * (REDUCE_BLOCKS + 1)) == hipSuccess;
+    for (int k = 0; k < 6 && ok; ++k) ok = hipMalloc(&buf[k], sz[k]) == hipSuccess;
+    auto release_all = [&]() {
+        for (void *b : buf)
+            if (b) (void)hipFree(b);
+        if (part) (void)hipFree(part);
+    };
+    if (!ok) {
+        release_all();
         (void)hipGetLastError();
         return NRQ_OK; // no room to tune: keep the default shape
     }
+    cplx<R> *u0 = (cplx<R> *)buf[0], *ua = (cplx<R> *)buf[1], *ub = (cplx<R> *)buf[2];
+    R *e0 = (R *)buf[3], *ea = (R *)buf[4], *eb_ = (R *)buf[5];
     NRQ_HIP(hipMemcpyAsync(u0, h->u[h->cur], lb, hipMemcpyDeviceToDevice, stream()));
-    NRQ_HIP(hipMemcpyAsync(ef, h->e, eb, hipMemcpyDeviceToDevice, stream()));
-    hipEvent_t e0, e1;
-    NRQ_HIP(hipEventCreate(&e0));
-    NRQ_HIP(hipEventCreate(&e1));
+    NRQ_HIP(hipMemcpyAsync(e0, h->e, eb, hipMemcpyDeviceToDevice, stream()));
+    NRQ_HIP(hipMemcpyAsync(ea, e0, eb, hipMemcpyDeviceToDevice, stream()));
+    launch_fused<R>(LF_DEFAULT, one<R>(u0, ua, ea), 1, g, dt);
+    NRQ_HIP(hipGetLastError());
+    // the scale of E: max |E - 0| after LF_DEFAULT's pass
+    double emax = 0;
+    NRQ_HIP(hipMemsetAsync(eb_, 0, eb, stream()));
+    if (int r = max_diff<R>(ea, eb_, ne, part, &emax)) return release_all(), r;
+    const double tol = shape_tolerance<R>();
+    std::vector<char> valid(LF_NVARIANTS, 0);
+    for (int v = 0; v < LF_NVARIANTS; ++v) {
+        if (!variant_ok(v, g)) continue;
+        NRQ_HIP(hipMemcpyAsync(eb_, e0, eb, hipMemcpyDeviceToDevice, stream()));
+        launch_fused<R>(v, one<R>(u0, ub, eb_), 1, g, dt);
+        NRQ_HIP(hipGetLastError());
+        double du = 0, de = 0;
+        if (int r = max_diff<R>((const R *)ua, (const R *)ub, nl, part, &du)) return release_all(), r;
+        if (int r = max_diff<R>(ea, eb_, ne, part, &de)) return release_all(), r;
+        valid[v] = du <= tol && de <= tol * std::max(1.0, emax);
+        if (!valid[v])
+            fprintf(stderr,
+                    "nrqcd_amd autotune: launch shape %d rejected: one pass differs from shape %d by %.3e (links), "
+                    "%.3e (E)\n",
+                    v, LF_DEFAULT, du, de);
+    }
+    hipEvent_t t0, t1;
+    NRQ_HIP(hipEventCreate(&t0));
+    NRQ_HIP(hipEventCreate(&t1));
     // two passes over all shapes, each shape scored by its faster pass, so
     // that one slow sample (clock ramp, another process) does not decide
     const int reps = g.vol >= (1 << 20) ? 6 : (g.vol >= (1 << 18) ? 16 : 40);
     std::vector<float> best_of(LF_NVARIANTS, 1e30f);
     for (int pass = 0; pass < 2; ++pass)
         for (int v = 0; v < LF_NVARIANTS; ++v) {
-            if (!variant_ok(v, g)) continue;
-            launch_fused<R>(v, one<R>(u0, u1, ef), 1, g, dt); // warm-up
-            NRQ_HIP(hipEventRecord(e0, stream()));
-            for (int k = 0; k < reps; ++k) launch_fused<R>(v, (k & 1) ? one<R>(u1, u0, ef) : one<R>(u0, u1, ef), 1, g, dt);
-            NRQ_HIP(hipEventRecord(e1, stream()));
-            NRQ_HIP(hipEventSynchronize(e1));
+            if (!valid[v]) continue;
+            launch_fused<R>(v, one<R>(u0, ua, eb_), 1, g, dt); // warm-up
+            NRQ_HIP(hipEventRecord(t0, stream()));
+            for (int k = 0; k < reps; ++k) launch_fused<R>(v, (k & 1) ? one<R>(ua, u0, eb_) : one<R>(u0, ua, eb_), 1, g, dt);
+            NRQ_HIP(hipEventRecord(t1, stream()));
+            NRQ_HIP(hipEventSynchronize(t1));
             float ms = 0;
-            NRQ_HIP(hipEventElapsedTime(&ms, e0, e1));
+            NRQ_HIP(hipEventElapsedTime(&ms, t0, t1));
             best_of[v] = std::min(best_of[v], ms);
             if (getenv("NRQ_AUTOTUNE_VERBOSE"))
                 fprintf(stderr,
@@ -679,31 +754,33 @@ template <typename R> static int tune_leapfrog(const nrq_gauge *h, R dt, int *be
                         (int)(8 * sizeof(R)), (long long)g.vol, v, LF_VARIANTS[v].bs, LF_VARIANTS[v].xcd, LF_VARIANTS[v].mode,
                         LF_VARIANTS[v].nt, LF_VARIANTS[v].rec, 1e3 * ms / reps);
         }
-    *best = LF_DEFAULT;
     for (int v = 0; v < LF_NVARIANTS; ++v)
-        if (best_of[v] < best_of[*best]) *best = v;
-    (void)hipEventDestroy(e0);
-    (void)hipEventDestroy(e1);
-    (void)hipFree(u0);
-    (void)hipFree(u1);
-    (void)hipFree(ef);
+        if (valid[v] && best_of[v] < best_of[*best]) *best = v;
+    (void)hipEventDestroy(t0);
+    (void)hipEventDestroy(t1);
+    release_all();
     return NRQ_OK;
 }
 
-// NRQ_LF_SHAPE=<n> pins launch shape n of LF_VARIANTS (where the lattice
-// allows it) instead of tuning, e.g. to profile one shape.
-static int forced_shape()
+// Launch shape pinned for every leapfrog call (-1: the autotuner's choice).
+// NRQ_LF_SHAPE=<n> in the environment gives the initial value (used where
+// the lattice allows shape n); nrq_leapfrog_set_shape changes it.
+static int g_pinned = -2; // -2: environment not read yet
+static bool g_pinned_by_env = false;
+static int pinned_shape()
 {
-    static int forced = -2;
-    if (forced == -2) {
-        forced = -1;
+    if (g_pinned == -2) {
+        g_pinned = -1;
         if (const char *e = getenv("NRQ_LF_SHAPE")) {
             char *end = nullptr;
             const long v = strtol(e, &end, 10);
-            if (end != e && *end == '\0' && v >= 0 && v < LF_NVARIANTS) forced = (int)v;
+            if (end != e && *end == '\0' && v >= 0 && v < LF_NVARIANTS) {
+                g_pinned = (int)v;
+                g_pinned_by_env = true;
+            }
         }
     }
-    return forced;
+    return g_pinned;
 }
 
 template <typename R> static int leapfrog_variant(const nrq_gauge *h, R dt, int *variant)
@@ -712,20 +789,26 @@ template <typename R> static int leapfrog_variant(const nrq_gauge *h, R dt, int 
         const char *env = getenv("NRQ_AUTOTUNE");
         g_autotune = (env && env[0] == '0') ? 0 : 1;
     }
-    const int f = forced_shape();
-    if (f >= 0 && variant_ok(f, lat().g)) {
-        *variant = f;
-        return NRQ_OK;
+    const geom &g = lat().g;
+    const int f = pinned_shape();
+    if (f >= 0) {
+        if (variant_ok(f, g)) {
+            *variant = f;
+            return NRQ_OK;
+        }
+        if (!g_pinned_by_env)
+            return fail(NRQ_ERR_UNSUPPORTED, "leapfrog: pinned launch shape %d does not apply to a %dx%dx%d lattice", f,
+                        g.n0, g.n1, g.n2);
     }
     *variant = LF_DEFAULT;
     if (!g_autotune) return NRQ_OK;
     for (const lf_choice &c : g_lf_choice)
-        if (c.precision == h->precision && c.vol == h->vol) {
+        if (c.precision == h->precision && c.n[0] == g.n0 && c.n[1] == g.n1 && c.n[2] == g.n2) {
             *variant = c.variant;
             return NRQ_OK;
         }
     if (int r = tune_leapfrog<R>(h, dt, variant)) return r;
-    g_lf_choice.push_back({h->precision, h->vol, *variant});
+    g_lf_choice.push_back({h->precision, {g.n0, g.n1, g.n2}, *variant});
     return NRQ_OK;
 }
 
@@ -781,9 +864,16 @@ template <typename R> static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, 
         issue();
         NRQ_HIP(hipGetLastError());
     } else {
+        // the captured launches carry the extents, the spacings (staple
+        // coefficients), the shape and the buffers by value
         std::vector<const void *> key = {(const void *)(intptr_t)g.n0, (const void *)(intptr_t)g.n1,
                                          (const void *)(intptr_t)g.n2, (const void *)(intptr_t)v,
                                          (const void *)(intptr_t)sizeof(R)};
+        for (int k = 0; k < 4; ++k) {
+            uint64_t bits;
+            memcpy(&bits, &g.a[k], sizeof bits);
+            key.push_back((const void *)(uintptr_t)bits);
+        }
         for (int i = 0; i < nb; ++i) {
             key.push_back(hs[i]->u[cur[i]]);
             key.push_back(hs[i]->u[cur[i] ^ 1]);
@@ -820,6 +910,14 @@ template <typename R> static int leapfrog_t(nrq_gauge *h, double sw, int64_t n)
 {
     nrq_gauge *hs[1] = {h};
     return leapfrog_batch_t<R>(hs, 1, sw, n, false);
+}
+
+// Drops the captured leapfrog graphs (their launches carry the lattice by
+// value); called when the lattice module is finalised.
+void gauge_forget_lattice()
+{
+    for (lf_graph &c : g_lf_graphs) (void)hipGraphExecDestroy(c.exec);
+    g_lf_graphs.clear();
 }
 
 int leapfrog_choice(const nrq_gauge *h, int *bs, int *xcd, int *mode, int *nt, int *rec)
@@ -1221,4 +1319,42 @@ extern "C" int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int 
     NEED_GAUGE(h);
     if (!block_sites || !xcd_remap || !tiling || !streaming || !two_row) return fail(NRQ_ERR_ARG, "null argument");
     return leapfrog_choice(h, block_sites, xcd_remap, tiling, streaming, two_row);
+}
+
+extern "C" int nrq_leapfrog_shape_count(int *n)
+{
+    if (!n) return fail(NRQ_ERR_ARG, "null argument");
+    *n = LF_NVARIANTS;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_leapfrog_shape_info(int k, int *block_sites, int *xcd_remap, int *tiling, int *streaming, int *two_row,
+                                       int *applies)
+{
+    if (k < 0 || k >= LF_NVARIANTS) return fail(NRQ_ERR_ARG, "launch shape %d out of range", k);
+    if (!block_sites || !xcd_remap || !tiling || !streaming || !two_row || !applies) return fail(NRQ_ERR_ARG, "null argument");
+    const lf_variant &v = LF_VARIANTS[k];
+    *block_sites = v.bs;
+    *xcd_remap = v.xcd;
+    *tiling = v.mode;
+    *streaming = v.nt;
+    *two_row = v.rec;
+    *applies = lat().initialised ? (variant_ok(k, lat().g) ? 1 : 0) : 0;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_leapfrog_set_shape(int k)
+{
+    if (k < -1 || k >= LF_NVARIANTS) return fail(NRQ_ERR_ARG, "launch shape %d out of range", k);
+    (void)pinned_shape(); // read the environment once, so that it cannot override this call later
+    g_pinned = k;
+    g_pinned_by_env = false;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_leapfrog_get_shape(int *k)
+{
+    if (!k) return fail(NRQ_ERR_ARG, "null argument");
+    *k = pinned_shape();
+    return NRQ_OK;
 }

@@ -53,6 +53,10 @@ int transverse_project(const cplx<double> *fld, double scale, double *out);
 int ham_cn_update(nrq_heavy *const *hs, const double *mass, int n, const nrq_gauge *gf, const double c[8],
                   double stepwidth, int restart, int64_t maxit, double tol, nrq_solve_stats *stats);
 
+// Drops state that carries the lattice by value (captured leapfrog graphs);
+// called by nrq_lattice_finalize.
+void gauge_forget_lattice();
+
 struct scratch_buf {
     void *ptr = nullptr;
     size_t bytes = 0;

@@ -674,30 +674,33 @@ static int upload_active(const std::vector<int> &act) { return upload_ints(g_ws.
 
 // Launch shape of the stencil kernels: slots per thread (NC, sharing each
 // link load) for FP64 and for FP32 arithmetic, and the site-block mapping of
-// make_lf_map.  NRQ_CN_SHAPE="nc64,nc32,mode,xcd" overrides the defaults.
+// make_lf_map.  NRQ_CN_SHAPE="nc64,nc32,mode,xcd" in the environment sets
+// the initial value, nrq_cn_set_shape changes it.  Every shape computes the
+// same sums in the same order per slot.
 struct cn_shape {
     int nc64 = 2, nc32 = 4, mode = 0, xcd = 8;
 };
+static bool cn_nc_ok(int v) { return v == 1 || v == 2 || v == 4; }
 static cn_shape read_shape()
 {
     cn_shape sh;
     if (const char *e = getenv("NRQ_CN_SHAPE")) {
         int a = 0, b = 0, c = 0, d = 0;
         if (sscanf(e, "%d,%d,%d,%d", &a, &b, &c, &d) == 4) {
-            auto ok = [](int v) { return v == 1 || v == 2 || v == 4; };
-            if (ok(a)) sh.nc64 = a;
-            if (ok(b)) sh.nc32 = b;
+            if (cn_nc_ok(a)) sh.nc64 = a;
+            if (cn_nc_ok(b)) sh.nc32 = b;
             if (c >= 0 && c <= 2) sh.mode = c;
             if (d == 1 || d == 8) sh.xcd = d;
         }
     }
     return sh;
 }
-static const cn_shape &shape()
+static cn_shape &shape_mut()
 {
-    static const cn_shape sh = read_shape();
+    static cn_shape sh = read_shape();
     return sh;
 }
+static const cn_shape &shape() { return shape_mut(); }
 template <typename T> static int nc_of() { return sizeof(T) == 8 ? shape().nc64 : shape().nc32; }
 
 // stencil launch map for nf slots, nc per thread
@@ -826,6 +829,9 @@ int krylov_norms2(const cplx<double> *x, int nf, double *out)
     return NRQ_OK;
 }
 
+// column solves handed from FGMRES to plain GMRES since the library loaded
+static int64_t g_fallbacks = 0;
+
 // ---------------------------------------------------------------------------
 // GMRES(m) core: solves (1 + beta_r K) x_r = b_r for the slots with act[r],
 // from the current x (x = 0 when zero_guess), until the T-precision true
@@ -833,13 +839,19 @@ int krylov_norms2(const cplx<double> *x, int nf, double *out)
 // rel_out[r] receives the last true relative residual.  beta must already be
 // on the device.
 // ---------------------------------------------------------------------------
+// With net (FGMRES, T = double): z_j = M(v_j), w = A z_j, x += Z y.  A slot
+// whose network output makes the Arnoldi step non-finite, whose cycle reduces
+// its residual by less than FGMRES_STALL, or that has not converged after
+// FGMRES_CYCLES cycles leaves the solve with x intact and fallen[r] = 1, for
+// the caller to finish with plain GMRES.
+constexpr double FGMRES_STALL = 0.1;
+constexpr int FGMRES_CYCLES = 4;
 template <typename T, typename RL>
 static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x, cplx<T> *basis, cplx<T> *t,
                       cplx<T> *res, bool zero_guess, const double *bnorm, std::vector<int> act, double tol, int m,
                       int64_t maxit, std::vector<int64_t> &its, double *rel_out, int64_t &cycles,
-                      nrq_cnn *net = nullptr, cplx<T> *zbasis = nullptr)
+                      nrq_cnn *net = nullptr, cplx<T> *zbasis = nullptr, std::vector<int> *fallen = nullptr)
 {
-    // with net (FGMRES, T = double): z_j = M(v_j), w = A z_j, x += Z y
     typedef std::complex<double> zc;
     const int64_t n = 3 * g_ws.vol;
     const size_t fbytes = sizeof(cplx<T>) * (size_t)n;
@@ -865,13 +877,15 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
     std::vector<std::vector<zc>> H(R, std::vector<zc>((size_t)(m + 1) * m)), G(R, std::vector<zc>(m + 1)),
         SN(R, std::vector<zc>(m)), Y(R, std::vector<zc>(m));
     std::vector<std::vector<double>> CS(R, std::vector<double>(m));
-    std::vector<int> jused(R, 0), cycle(R);
+    std::vector<int> jused(R, 0), cycle(R), ncycles(R, 0);
+    std::vector<double> rn_start(R, 0.0);
     double *restart_n2 = g_ws.nrm + workspace::NRM_RESTART;
     while (true) {
         bool any = false;
         for (int r = 0; r < R; ++r) any = any || act[r];
         if (!any) break;
         ++cycles;
+        rn_start = rn;
         std::vector<double> rn2(R);
         for (int r = 0; r < R; ++r) {
             rn2[r] = act[r] ? rn[r] * rn[r] : 0.0;
@@ -918,6 +932,18 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
             if (int e = nrm_to_host()) return e;
             for (int r = 0; r < R; ++r) {
                 if (!cycle[r]) continue;
+                if (net) {
+                    // the network's direction broke the Arnoldi step: end the
+                    // slot's cycle before column j (x keeps columns < j)
+                    bool fin = std::isfinite(g_ws.nbuf[r]);
+                    for (int i = 0; i < nv && fin; ++i)
+                        fin = std::isfinite(g_ws.hbuf[r * nv + i].re) && std::isfinite(g_ws.hbuf[r * nv + i].im);
+                    if (!fin) {
+                        cycle[r] = 0;
+                        (*fallen)[r] = 1;
+                        continue;
+                    }
+                }
                 ++its[r];
                 zc *Hc = H[r].data() + (size_t)(m + 1) * j;
                 for (int i = 0; i < nv; ++i) Hc[i] = zc(g_ws.hbuf[r * nv + i].re, g_ws.hbuf[r * nv + i].im);
@@ -957,7 +983,15 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
                 for (int k = i + 1; k < jj; ++k) sacc -= H[r][(size_t)(m + 1) * k + i] * Y[r][k];
                 Y[r][i] = sacc / H[r][(size_t)(m + 1) * i + i];
             }
-            for (int i = 0; i < jj; ++i) g_ws.hbuf[r * (m + 1) + i] = {Y[r][i].real(), Y[r][i].imag()};
+            if (net) {
+                bool fin = true;
+                for (int i = 0; i < jj && fin; ++i) fin = std::isfinite(Y[r][i].real()) && std::isfinite(Y[r][i].imag());
+                if (!fin) {
+                    nu[r] = 0; // keep x
+                    (*fallen)[r] = 1;
+                }
+            }
+            for (int i = 0; i < nu[r]; ++i) g_ws.hbuf[r * (m + 1) + i] = {Y[r][i].real(), Y[r][i].imag()};
         }
         if (int e = upload_active(act)) return e;
         NRQ_HIP(hipMemcpyAsync(g_ws.ydev, g_ws.hbuf, sizeof(cplx<double>) * R * (m + 1), hipMemcpyHostToDevice, stream()));
@@ -975,7 +1009,14 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
             if (!act[r]) continue;
             rn[r] = std::sqrt(std::max(g_ws.nbuf[r], 0.0));
             rel_out[r] = bnorm[r] > 0 ? rn[r] / bnorm[r] : 0.0;
-            if (rn[r] <= tol * bnorm[r] || its[r] >= maxit) act[r] = 0;
+            if (rn[r] <= tol * bnorm[r] || its[r] >= maxit) {
+                act[r] = 0;
+                continue;
+            }
+            if (net && ((*fallen)[r] || !(rn[r] <= FGMRES_STALL * rn_start[r]) || ++ncycles[r] >= FGMRES_CYCLES)) {
+                act[r] = 0;
+                (*fallen)[r] = 1;
+            }
         }
     }
     return NRQ_OK;
@@ -1333,9 +1374,21 @@ static int cn_step_t(const cplx<RL> *u, int R, const std::vector<zc> &alpha, con
             nrq_cnn *net = cn_preconditioner();
             if (!net) return fail(NRQ_ERR_STATE, "CN: FGMRES needs a network (nrq_cn_set_preconditioner)");
             if (int e = reserve_zbasis()) return e;
+            std::vector<int> fallen(R, 0);
             status = gmres_core<double, RL>(u, R, g_ws.y, g_ws.x, (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res,
                                             !p.extrapolate, bn.data(), act, p.tol, p.restart, p.maxit, its, rel.data(),
-                                            cycles, net, g_ws.zbasis);
+                                            cycles, net, g_ws.zbasis, &fallen);
+            int nfall = 0;
+            for (int f : fallen) nfall += f;
+            if (!status && nfall) {
+                // the network did not help these slots: plain GMRES (the
+                // operator's diagonal is constant, so Jacobi scaling leaves
+                // GMRES unchanged) from the iterate reached so far
+                g_fallbacks += nfall;
+                status = gmres_core<double, RL>(u, R, g_ws.y, g_ws.x, (cplx<double> *)g_ws.basis, g_ws.t, g_ws.res,
+                                                false, bn.data(), fallen, p.tol, p.restart, p.maxit, its, rel.data(),
+                                                cycles);
+            }
         } else
             status = gmres_mixed<RL>(u, R, bn, p, act, rel, its, cycles);
     }
@@ -1388,3 +1441,36 @@ int krylov_cn_step(const nrq_gauge *gf, cplx<double> *const *fields, int R, cons
 }
 
 } // namespace nrq
+
+using namespace nrq;
+
+extern "C" int nrq_cn_set_shape(int nc64, int nc32, int tiling, int xcd_remap)
+{
+    if (!cn_nc_ok(nc64) || !cn_nc_ok(nc32)) return fail(NRQ_ERR_ARG, "CN shape: slots per thread must be 1, 2 or 4");
+    if (tiling < 0 || tiling > 2) return fail(NRQ_ERR_ARG, "CN shape: tiling %d out of range", tiling);
+    if (xcd_remap != 1 && xcd_remap != 8) return fail(NRQ_ERR_ARG, "CN shape: XCD remap must be 1 or 8");
+    cn_shape &sh = shape_mut();
+    sh.nc64 = nc64;
+    sh.nc32 = nc32;
+    sh.mode = tiling;
+    sh.xcd = xcd_remap;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_cn_get_shape(int *nc64, int *nc32, int *tiling, int *xcd_remap)
+{
+    if (!nc64 || !nc32 || !tiling || !xcd_remap) return fail(NRQ_ERR_ARG, "null argument");
+    const cn_shape &sh = shape();
+    *nc64 = sh.nc64;
+    *nc32 = sh.nc32;
+    *tiling = sh.mode;
+    *xcd_remap = sh.xcd;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_cn_fallbacks(int64_t *count)
+{
+    if (!count) return fail(NRQ_ERR_ARG, "null argument");
+    *count = g_fallbacks;
+    return NRQ_OK;
+}

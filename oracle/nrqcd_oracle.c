@@ -49,6 +49,10 @@ static int64_t VOL = 0;
 static double SP[4] = {1, 1, 1, 1};
 static const double ZERO_TOL = 1e-12; /* tolerances.f90:23,26 */
 
+/* planted mistake (see nrqcd_oracle.h); 0 plants nothing */
+static int PLANT = 0;
+void orc_set_plant(int which) { PLANT = which; }
+
 int orc_lattice_init(const int64_t ext[3], const double spacing[4])
 {
     for (int i = 0; i < 3; ++i) {
@@ -540,12 +544,18 @@ int orc_box_init(double qs, double amplitude, double coupling, zc *links, double
         double f = box_occupation(pn, qs, amplitude, coupling);
         double pa = sqrt(f / pn * 2.0);
         zc pe = I * sqrt(f * pn * 2.0);
+        if (PLANT == 3) pa *= sqrt(2.0);
+        if (PLANT == 4) {
+            pa = sqrt(f * pn * 2.0);
+            pe = I * sqrt(f / pn * 2.0);
+        }
+        if (PLANT == 6) pe = sqrt(f * pn * 2.0);
         for (int a = 0; a < 8; ++a) {
             zc r1 = r[x + V * a], r2 = r[x + V * (a + 8)];
             for (int i = 0; i < 3; ++i) {
                 zc mode = r1 * eps[i] + r2 * eps[i + 3];
                 af[x + V * (a + 8 * i)] = pa * mode;
-                ef[x + V * (a + 8 * i)] = pe * mode;
+                ef[x + V * (a + 8 * i)] = pe * (PLANT == 5 ? r2 * eps[i] + r1 * eps[i + 3] : mode);
             }
         }
     }
@@ -622,7 +632,8 @@ void orc_update_efield(const zc *links, double *efield, double stepwidth)
                 zc su[9], sd[9];
                 ustaple(links, i, k, x, su);
                 dstaple(links, i, k, x, sd);
-                double coef = stepwidth * (SP[0] / SP[k]) * (SP[0] / SP[k]);
+                const int kc = PLANT == 1 ? i : k;
+                double coef = stepwidth * (SP[0] / SP[kc]) * (SP[0] / SP[kc]);
                 for (int m = 0; m < 9; ++m) sum[m] += coef * (su[m] + sd[m]);
             }
             zc w[9];
@@ -685,7 +696,7 @@ double orc_energy(const zc *links, const double *efield)
                 zc P[9];
                 orc_plaquette(links, i, j, x, P);
                 double tr = creal(P[0] + P[4] + P[8]);
-                s += 2.0 * 3.0 * (1.0 - tr / 3.0) / SP[i] / SP[j];
+                s += 2.0 * 3.0 * (1.0 - tr / 3.0) / SP[i] / SP[PLANT == 2 ? i : j];
             }
         }
         site[x - 1] = s;

@@ -96,6 +96,16 @@ zc      orc_corr_o8_3s1_gconjg(const zc *quark, const zc *antiq);
 
 int     orc_num_threads(void);
 
+/* Planted mistakes, for the tests that show a check fails when the oracle
+ * is wrong (0, the default, plants nothing; every other use leaves it 0):
+ *   1  staple coefficient (a0/a_i)^2 instead of (a0/a_k)^2
+ *   2  magnetic energy divided by a_i a_i instead of a_i a_j
+ *   3  box IC: A amplitude sqrt(2) too large
+ *   4  box IC: |p| powers of the A and E amplitudes swapped
+ *   5  box IC: E takes the noise of the two polarisations swapped
+ *   6  box IC: E amplitude real (the factor i dropped) */
+void    orc_set_plant(int which);
+
 #ifdef __cplusplus
 }
 #endif

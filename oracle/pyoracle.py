@@ -64,7 +64,7 @@ class Oracle:
             ("orc_mean_plaquette", D, [P]), ("orc_heavy_init_point", None, [P, I64]),
             ("orc_apply_system", None, [P, P, P, D, _Z, D, I]),
             ("orc_cn_update", I, [P, P, P, D, P, D, D, I, I64, P]), ("orc_prop_norm", D, [P]),
-            ("orc_num_threads", I, []),
+            ("orc_num_threads", I, []), ("orc_set_plant", None, [I]),
         ]:
             f = getattr(L, name)
             f.restype = res
@@ -258,6 +258,11 @@ class Oracle:
 
     def num_threads(self) -> int:
         return self.lib.orc_num_threads()
+
+    def set_plant(self, which: int) -> None:
+        """Plant mistake `which` (nrqcd_oracle.h) into the oracle; 0 removes it.
+        Only the tests that show a check can fail use this."""
+        self.lib.orc_set_plant(which)
 
 
 class RefModules:

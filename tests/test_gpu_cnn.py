@@ -152,3 +152,42 @@ def test_training_and_gain(nrq, oracle, lattice):
         nrq.nrqcd_set_preconditioner(None)
     finally:
         nrq.nrqcd_finalize()
+
+
+@pytest.mark.parametrize("scale", [1e4, 30.0])
+def test_fgmres_bad_network_falls_back_to_gmres(nrq, oracle, lattice, scale):
+    """A network that does not precondition (weights far too large, so that
+    N(v) swamps v in z = v + N(v)) must not change the
+    answer: FGMRES hands the column solves it cannot finish to plain GMRES,
+    which continues from the iterate reached and meets the oracle's CN step
+    within 1e-10.  With the far too large weights the solves do fall back."""
+    ext = (16, 6, 5)
+    _say(f"FGMRES with a bad network (weights x {scale})")
+    lattice(ext, SP)
+    nrq.nrqcd_init(2.0, C1, 1.0, 1e-12)
+    try:
+        oracle.lattice_init(ext, SP)
+        U, E = oracle.box_init(8, 0.5, 1.0, 1.0)
+        g = nrq.GaugeConfiguration()
+        g.upload(U, E)
+        net = nrq.PreconditionerNet(5)
+        net.weights = net.weights * np.float32(scale)
+        nrq.nrqcd_set_preconditioner(net)
+        nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_FGMRES_CNN, 20, 10000)
+        before = nrq.cn_fallbacks()
+        h = nrq.NRQCDField()
+        h.InitSinglePoint(1, 17)
+        st = h.Update(g, 2.0, C1)
+        fell = nrq.cn_fallbacks() - before
+        _say(f"column solves handed to GMRES: {fell} of {st.solves}")
+        assert st.max_true_relres <= 1e-12
+        Q, A = oracle.point_prop(1), oracle.point_prop(17)
+        oracle.cn_update(U, Q, A, 2.0, C1, 1.0, 1e-13)
+        Qg, Ag = h.download()
+        assert np.max(np.abs(Qg - Q)) < 1e-10 * np.max(np.abs(Q))
+        assert np.max(np.abs(Ag - A)) < 1e-10 * np.max(np.abs(A))
+        if scale > 1e3:
+            assert fell > 0
+        nrq.nrqcd_set_preconditioner(None)
+    finally:
+        nrq.nrqcd_finalize()
