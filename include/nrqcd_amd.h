@@ -167,6 +167,16 @@ int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps);
  * interval is honoured.  No reference counterpart: the reference evolves
  * one configuration per run (src/main.f90:2572-2620 loops over runs). */
 int nrq_gauge_leapfrog_batch(nrq_gauge *const *h, int n, double stepwidth, int64_t nsteps);
+/* Kernel timer of the fused leapfrog pass (measurement only, no reference
+ * counterpart).  After nrq_leapfrog_timer_start(max_spans), each leapfrog
+ * call that is not graph-captured records a HIP event pair on the library
+ * stream around its run of fused launches (the E update of one step fused
+ * with the link update of the next), up to max_spans runs.
+ * nrq_leapfrog_timer_stop synchronises and returns the summed event time,
+ * the fused launches and (site, configuration) steps those runs hold, and
+ * the fused launches of runs past the last span (not timed). */
+int nrq_leapfrog_timer_start(int64_t max_spans);
+int nrq_leapfrog_timer_stop(double *fused_ms, int64_t *launches, int64_t *site_steps, int64_t *skipped);
 /* GaugeConfiguration%GetEnergy(), gaugeconfiguration_su3.f90:1769-1817 */
 int nrq_gauge_energy(const nrq_gauge *h, double *energy);
 /* GaugeConfiguration%GetDeviationFromGaussLaw(), gaugeconfiguration_su3.f90:
@@ -211,7 +221,10 @@ int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval);
 /* Launch shape the leapfrog autotuner chose for this configuration's
  * precision and lattice size: threads per block, XCD remap factor (1 = none),
  * tiling (0: consecutive sites, 1: z-sweep tiles of full x-rows, 2: z-sweep
- * tiles 16 sites wide), streaming (1: nontemporal loads/stores of E and the
+ * tiles 16 sites wide, 3 and 4: LDS-staged 16 x 8 (FP32) or 16 x 4 (FP64)
+ * tiles over runs of 16 or 8 z planes, neighbour links read once per run and
+ * shared through LDS; extents multiples of 16, 8 and the run), streaming
+ * (1: nontemporal loads/stores of E and the
  * new links) and two_row (1: neighbour links read as two rows, the third
  * rebuilt as the conjugate cross product, exact for SU(3)).  The first
  * leapfrog call per (precision, size) times every shape on scratch copies;
@@ -222,7 +235,8 @@ int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval);
 int nrq_gauge_launch_shape(const nrq_gauge *h, int *block_sites, int *xcd_remap, int *tiling, int *streaming,
                            int *two_row);
 /* The autotuner's candidate shapes: their number, the parameters of shape k
- * (as nrq_gauge_launch_shape) and whether it applies to the current lattice.
+ * (as nrq_gauge_launch_shape) and whether it applies to the current lattice
+ * (bit 0: with FP64 storage, bit 1: with FP32 storage).
  * The tuner times only the shapes whose one fused pass agrees with the
  * default shape's (1e-11 FP64, 5e-5 FP32; two-row shapes rebuild a row and
  * agree to rounding) and reports any other on stderr.  No reference
@@ -275,6 +289,8 @@ typedef struct {
                                         iterative refinement */
 #define NRQ_SOLVER_FGMRES_CNN 4      /* FGMRES(m) in FP64, right-preconditioned by
                                         the learned network (nrq_cn_set_preconditioner) */
+#define NRQ_SOLVER_FGMRES_CNN_MIXED 5 /* FP32 FGMRES(m) corrections with the learned
+                                         network inside FP64 iterative refinement */
 
 /* nrqcd::InitModule(Mass, WilsonCoeffs(4), StepWidth, kspTol),
  * nrqcd.f90:167-192 with KSPSetTolerances(rtol=kspTol), nrqcd.f90:803-804.
@@ -368,11 +384,15 @@ int nrq_w2t(double *data, int64_t n, double dt);
 
 /* ---- learned preconditioner ------------------------------------------
  * No reference counterpart (the reference uses PETSc's ILU(0)/block-Jacobi,
- * nrqcd.f90:786-804).  A 4-layer 3-D convolutional network N acting on three
- * colour-vector fields at a time (18 real channels per site), 32 channels
+ * nrqcd.f90:786-804).  A 4-layer 3-D convolutional network N, 32 channels
  * wide, 3^3 periodic kernels, ReLU, no biases, BF16 weights and activations
- * with FP32 accumulation on matrix cores.  FGMRES uses z = v + N(v) as its
- * right preconditioner.  The lattice x extent must be a multiple of 16. */
+ * with FP32 accumulation on matrix cores.  Its 18 real input channels per
+ * site are one colour-vector field v (3 colours x re/im) and its covariant
+ * features beta K v and (beta K)^2 v, with K the covariant Laplacian of the
+ * CN operator 1 + beta K: the inputs transform like v under gauge
+ * transformations and do not depend on the mass.  FGMRES uses
+ * z = v + N(v, beta K v, (beta K)^2 v) (output channels 0..5) as its right
+ * preconditioner.  The lattice x extent must be a multiple of 16. */
 typedef struct nrq_cnn nrq_cnn;
 #define NRQ_CNN_NWEIGHTS (4 * 27 * 32 * 32) /* [layer][tap][co][ci], tap = 9(dz+1)+3(dy+1)+(dx+1);
                                                channels past 18 of layer 1 inputs and
@@ -387,13 +407,19 @@ int nrq_cnn_forward(nrq_cnn *net, const float *in, float *out);
 /* loss = sum (N(in) - target)^2 / norm and its gradient with respect to the
  * NRQ_CNN_NWEIGHTS weights (host arrays, [site][18] layouts) */
 int nrq_cnn_loss_grad(nrq_cnn *net, const float *in, const float *target, double norm, double *loss, float *grad);
+/* nrq_cnn_loss_grad that also returns the network's own BF16 values (as
+ * floats, [layer][site][32]): acts[l] the input of layer l, deltas[l] the
+ * loss gradient at the output of layer l, from which that layer's weight
+ * gradient is formed.  Either may be NULL.  For checking the kernels. */
+int nrq_cnn_loss_grad_trace(nrq_cnn *net, const float *in, const float *target, double norm, double *loss, float *grad,
+                            float *acts, float *deltas);
 /* Adds training pairs from `nsolves` plain FP64 GMRES solves (true relative
  * residual 1e-12) of CN systems on g at `mass` and `stepwidth` (c1 = 1):
- * each solve takes 12 random
- * Gaussian fields psi (6 quark, 6 antiquark), forms y = S psi and solves
- * S^+ x = y; every three fields give one pair r = y/|y|, target
- * (x - y)/|y| (the error of the zero guess minus the residual).  All pairs of
- * one network come from one lattice size. */
+ * each solve takes 12 random Gaussian fields psi (6 quark, 6 antiquark),
+ * forms y = S psi and solves S^+ x = y; every field gives one pair: input
+ * [y, beta K y, (beta K)^2 y] / |y|, target (x - y) / |y| (the error of the
+ * zero guess minus the residual) in channels 0..5 and zero in 6..17.  All
+ * pairs of one network come from one lattice size. */
 int nrq_cnn_harvest(nrq_cnn *net, const nrq_gauge *g, double mass, double stepwidth, int nsolves, uint64_t seed);
 int nrq_cnn_dataset_size(const nrq_cnn *net, int64_t *count);
 /* `epochs` passes of Adam (learning rate lr, one pair per step, order

@@ -38,6 +38,7 @@ NRQ_SOLVER_GMRES_MIXED = 1
 NRQ_SOLVER_CHEBYSHEV_FP64 = 2
 NRQ_SOLVER_CHEBYSHEV_MIXED = 3
 NRQ_SOLVER_FGMRES_CNN = 4
+NRQ_SOLVER_FGMRES_CNN_MIXED = 5
 NRQ_CNN_NWEIGHTS = 4 * 27 * 32 * 32
 
 _ERRORS = {
@@ -96,6 +97,8 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_event_elapsed": (I, [I, I, P]),
         "nrq_synchronize": (I, []),
         "nrq_bandwidth_probe": (I, [I64, I, P]),
+        "nrq_leapfrog_timer_start": (I, [I64]),
+        "nrq_leapfrog_timer_stop": (I, [P, P, P, P]),
         "nrq_lattice_init": (I, [P, P]),
         "nrq_lattice_finalize": (I, []),
         "nrq_lattice_size": (I, [P]),
@@ -157,6 +160,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_cnn_set_weights": (I, [P, P]),
         "nrq_cnn_forward": (I, [P, P, P]),
         "nrq_cnn_loss_grad": (I, [P, P, P, D, P, P]),
+        "nrq_cnn_loss_grad_trace": (I, [P, P, P, D, P, P, P, P]),
         "nrq_cnn_harvest": (I, [P, P, D, D, I, ctypes.c_uint64]),
         "nrq_cnn_dataset_size": (I, [P, P]),
         "nrq_cnn_train": (I, [P, I, D, ctypes.c_uint64, P]),
@@ -201,6 +205,20 @@ def event_elapsed_ms(start: int, stop: int) -> float:
 
 def synchronize() -> None:
     _check(library().nrq_synchronize())
+
+
+def leapfrog_timer_start(max_spans: int = 1024) -> None:
+    """Start timing the fused leapfrog launches (one HIP event pair per
+    leapfrog call, on the library stream)."""
+    _check(library().nrq_leapfrog_timer_start(max_spans))
+
+
+def leapfrog_timer_stop() -> dict:
+    """Stop the fused-launch timer: summed event time (ms), fused launches,
+    (site, configuration) steps they hold, and untimed launches."""
+    ms, la, ss, sk = ctypes.c_double(), ctypes.c_int64(), ctypes.c_int64(), ctypes.c_int64()
+    _check(library().nrq_leapfrog_timer_stop(ctypes.byref(ms), ctypes.byref(la), ctypes.byref(ss), ctypes.byref(sk)))
+    return {"ms": ms.value, "launches": la.value, "site_steps": ss.value, "skipped": sk.value}
 
 
 def bandwidth_probe(nbytes: int = 1 << 30, reps: int = 20) -> float:
@@ -504,12 +522,14 @@ class GaugeConfiguration:
 # ---------------------------------------------------------------------------
 # nrqcd
 # ---------------------------------------------------------------------------
-_TILINGS = ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles")
+_TILINGS = ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles", "LDS tiles, 16-plane runs",
+            "LDS tiles, 8-plane runs")
 
 
 def leapfrog_shapes() -> list:
     """The autotuner's candidate launch shapes, with whether each applies to
-    the current lattice."""
+    the current lattice with FP64 storage ("applies") and with FP32 storage
+    ("applies_fp32")."""
     n = ctypes.c_int()
     _check(library().nrq_leapfrog_shape_count(ctypes.byref(n)))
     out = []
@@ -518,7 +538,7 @@ def leapfrog_shapes() -> list:
         _check(library().nrq_leapfrog_shape_info(k, *[ctypes.byref(x) for x in v]))
         out.append({"threads_per_block": v[0].value, "xcd_remap": v[1].value, "tiling": _TILINGS[v[2].value],
                     "streaming": bool(v[3].value), "two_row_neighbours": bool(v[4].value),
-                    "applies": bool(v[5].value)})
+                    "applies": bool(v[5].value & 1), "applies_fp32": bool(v[5].value & 2)})
     return out
 
 
@@ -713,6 +733,20 @@ class PreconditionerNet:
         grad = np.zeros(NRQ_CNN_NWEIGHTS, np.float32)
         _check(library().nrq_cnn_loss_grad(self._h, _ptr(inp), _ptr(target), norm, ctypes.byref(loss), _ptr(grad)))
         return loss.value, grad
+
+    def loss_grad_trace(self, inp: np.ndarray, target: np.ndarray, norm: float = 1.0):
+        """loss_grad with the network's own BF16 layer inputs and output
+        gradients: (loss, grad, acts[4][V][32], deltas[4][V][32])."""
+        inp = np.ascontiguousarray(inp, dtype=np.float32)
+        target = np.ascontiguousarray(target, dtype=np.float32)
+        v = inp.shape[0]
+        loss = ctypes.c_double()
+        grad = np.zeros(NRQ_CNN_NWEIGHTS, np.float32)
+        acts = np.zeros((4, v, 32), np.float32)
+        deltas = np.zeros((4, v, 32), np.float32)
+        _check(library().nrq_cnn_loss_grad_trace(self._h, _ptr(inp), _ptr(target), norm, ctypes.byref(loss), _ptr(grad),
+                                                 _ptr(acts), _ptr(deltas)))
+        return loss.value, grad, acts, deltas
 
     def harvest(self, GaugeConf, mass: float, nsolves: int, seed: int = 1, StepWidth: float = 1.0) -> None:
         """Training pairs from plain GMRES solves of CN systems at this mass."""

@@ -230,15 +230,15 @@ extern "C" int nrq_cn_finalize(void)
 extern "C" int nrq_cn_set_solver(int kind, int restart, int64_t maxit)
 {
     if (!g_cn.initialised) return fail(NRQ_ERR_STATE, "nrqcd: not initialised");
-    if (kind < NRQ_SOLVER_GMRES_FP64 || kind > NRQ_SOLVER_FGMRES_CNN)
+    if (kind < NRQ_SOLVER_GMRES_FP64 || kind > NRQ_SOLVER_FGMRES_CNN_MIXED)
         return fail(NRQ_ERR_UNSUPPORTED, "nrqcd: solver kind %d not available", kind);
     if (restart < 1 || restart > 200 || maxit < 1) return fail(NRQ_ERR_ARG, "nrqcd: bad restart or maxit");
     static const krylov_kind kinds[] = {KRYLOV_GMRES_FP64, KRYLOV_GMRES_MIXED, KRYLOV_CHEB_FP64, KRYLOV_CHEB_MIXED,
-                                        KRYLOV_FGMRES_CNN};
+                                        KRYLOV_FGMRES_CNN, KRYLOV_FGMRES_CNN_MIXED};
     g_cn.solver.kind = kinds[kind];
     // the fast solvers start from the first-order guess; plain GMRES and
     // FGMRES start from zero (PETSc's default), so that FGMRES differs from
-    // GMRES only by the network
+    // GMRES only by the network (and the mixed FGMRES from the mixed GMRES)
     g_cn.solver.extrapolate = kind != NRQ_SOLVER_GMRES_FP64 && kind != NRQ_SOLVER_FGMRES_CNN;
     g_cn.solver.restart = restart;
     g_cn.solver.maxit = maxit;
@@ -480,6 +480,14 @@ extern "C" int nrq_heavy_update_batch(nrq_heavy *const *h, const double *mass, i
     return heavy_update_n(h, mass, n, gf, c, stepwidth, stats);
 }
 
+// Blocks of one k_corr_traces pass: one site per thread, at most
+// CORR_BLOCKS_MAX blocks (the kernel strides over larger lattices).
+constexpr int CORR_BLOCKS_MAX = 4096;
+static int corr_blocks(int64_t vol)
+{
+    return (int)std::min<int64_t>(CORR_BLOCKS_MAX, std::max<int64_t>(1, (vol + REDUCE_THREADS - 1) / REDUCE_THREADS));
+}
+
 // Brings h->obs up to date with the field (one k_corr_traces pass when the
 // field changed since the last evaluation).
 static int heavy_observables(const nrq_heavy *hc)
@@ -490,8 +498,8 @@ static int heavy_observables(const nrq_heavy *hc)
     if (int r = krylov_reserve(h->vol, g_cn.initialised ? g_cn.solver.restart : 30, 0)) return r;
     // one site per thread up to 4096 blocks (the partials, 16 x 4096 doubles,
     // and the 16 sums fit the reduction scratch)
-    static_assert((size_t)CORR_SUMS * 4096 + CORR_SUMS <= KRYLOV_REDUCE_DOUBLES, "correlator partials");
-    const int nb = (int)std::min<int64_t>(4096, std::max<int64_t>(1, (h->vol + REDUCE_THREADS - 1) / REDUCE_THREADS));
+    static_assert((size_t)CORR_SUMS * CORR_BLOCKS_MAX + CORR_SUMS <= KRYLOV_REDUCE_DOUBLES, "correlator partials");
+    const int nb = corr_blocks(h->vol);
     double *part = krylov_reduce_scratch();
     double *res = part + (size_t)CORR_SUMS * nb;
     hipLaunchKernelGGL(k_corr_traces, dim3(nb), dim3(REDUCE_THREADS), 0, stream(), (const cplx<double> *)h->psi,
@@ -515,13 +523,11 @@ extern "C" int nrq_heavy_norms(const nrq_heavy *h, double *nq, double *na)
     return NRQ_OK;
 }
 
-// The plain (gconjg = false) or _gconjg correlators from the trace sums.
-static int heavy_correlators(const nrq_heavy *h, bool gconjg, double out[8])
+// The plain (gconjg = false) or _gconjg correlators from the CORR_SUMS trace
+// sums t.
+static void correlators_from_sums(const double *t, bool gconjg, double out[8])
 {
-    if (!out) return fail(NRQ_ERR_ARG, "null argument");
-    if (int r = heavy_observables(h)) return r;
     using zc = std::complex<double>;
-    const double *t = h->obs;
     const int o = gconjg ? 8 : 0;
     const zc t0(t[o], t[o + 1]), tx(t[2], t[3]);
     const zc tc(t[gconjg ? 10 : 4], t[gconjg ? 11 : 5]);
@@ -533,6 +539,13 @@ static int heavy_correlators(const nrq_heavy *h, bool gconjg, double out[8])
         out[2 * k] = c[k].real();
         out[2 * k + 1] = c[k].imag();
     }
+}
+
+static int heavy_correlators(const nrq_heavy *h, bool gconjg, double out[8])
+{
+    if (!out) return fail(NRQ_ERR_ARG, "null argument");
+    if (int r = heavy_observables(h)) return r;
+    correlators_from_sums(h->obs, gconjg, out);
     return NRQ_OK;
 }
 

@@ -6,10 +6,14 @@ struct nrq_cnn;
 
 namespace nrq {
 
-// z_r = v_r + N(v_r) for the R slot fields v (3V complex doubles each,
-// colour major) at v[r], results at z[r]: the network acts on groups of three
-// slots (18 real channels per site).  Needs n0 % 16 == 0.
-int cnn_precondition(nrq_cnn *net, cplx<double> *const *v, cplx<double> *const *z, int R);
+// z_r = v_r + N(v_r, f1_r, f2_r) for the slots r < R with act[r] (all when
+// act is null): v_r, f1_r = beta_r K v_r, f2_r = beta_r K f1_r are colour
+// fields of 3V complex numbers (colour major), z_r receives the result.  The
+// network takes one colour field with its two features per pass (18 real
+// channels per site).  Needs n0 % 16 == 0.
+template <typename T>
+int cnn_precondition(nrq_cnn *net, const cplx<T> *const *v, const cplx<T> *const *f1, const cplx<T> *const *f2,
+                     cplx<T> *const *z, int R, const int *act);
 
 // the network registered with nrq_cn_set_preconditioner (may be null)
 nrq_cnn *cn_preconditioner();

@@ -244,58 +244,60 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
     wt[(int64_t)l * WL + ((int64_t)(26 - tap) * CH + ci) * CH + co] = f2bf(w[i]);
 }
 
-// BF16 network input from three slot fields (complex colour-major, 3V each):
-// channel 6 j + 2 c + {re, im} of slot j, colour c; channels >= 18 zero.
-__global__ void k_pack(const cplx<double> *__restrict__ v0, const cplx<double> *__restrict__ v1,
-                       const cplx<double> *__restrict__ v2, bf16 *__restrict__ act, int64_t vol)
+// BF16 network input of one colour field v (complex colour-major, 3V) and
+// its features f1 = beta K v, f2 = beta K f1: channel 6 q + 2 c + {re, im}
+// of feature q (v, f1, f2) and colour c; channels >= 18 zero.
+template <typename T>
+__global__ void k_pack(const cplx<T> *__restrict__ v, const cplx<T> *__restrict__ f1, const cplx<T> *__restrict__ f2,
+                       bf16 *__restrict__ act, int64_t vol)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x) {
-        const cplx<double> *v[3] = {v0, v1, v2};
-        for (int j = 0; j < 3; ++j)
+        const cplx<T> *f[3] = {v, f1, f2};
+        for (int q = 0; q < 3; ++q)
             for (int c = 0; c < 3; ++c) {
-                const cplx<double> a = v[j] ? v[j][(int64_t)c * vol + s] : cplx<double>{0.0, 0.0};
-                act[s * CH + 6 * j + 2 * c] = f2bf((float)a.re);
-                act[s * CH + 6 * j + 2 * c + 1] = f2bf((float)a.im);
+                const cplx<T> a = f[q][(int64_t)c * vol + s];
+                act[s * CH + 6 * q + 2 * c] = f2bf((float)a.re);
+                act[s * CH + 6 * q + 2 * c + 1] = f2bf((float)a.im);
             }
         for (int c = CIN; c < CH; ++c) act[s * CH + c] = 0;
     }
 }
 
-// z_j = v_j + N(v)_j (FP64) for the three slots of a group
-__global__ void k_unpack(const float *__restrict__ o, const cplx<double> *__restrict__ v0, const cplx<double> *__restrict__ v1,
-                         const cplx<double> *__restrict__ v2, cplx<double> *__restrict__ z0, cplx<double> *__restrict__ z1,
-                         cplx<double> *__restrict__ z2, int64_t vol)
+// z = v + N(v) from output channels 0..5 (colour c at 2 c, 2 c + 1)
+template <typename T>
+__global__ void k_unpack(const float *__restrict__ o, const cplx<T> *__restrict__ v, cplx<T> *__restrict__ z, int64_t vol)
 {
-    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x) {
-        const cplx<double> *v[3] = {v0, v1, v2};
-        cplx<double> *z[3] = {z0, z1, z2};
-        for (int j = 0; j < 3; ++j) {
-            if (!z[j]) continue;
-            for (int c = 0; c < 3; ++c) {
-                const int64_t at = (int64_t)c * vol + s;
-                const cplx<double> a = v[j][at];
-                z[j][at] = {a.re + (double)o[s * CH + 6 * j + 2 * c], a.im + (double)o[s * CH + 6 * j + 2 * c + 1]};
-            }
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x)
+        for (int c = 0; c < 3; ++c) {
+            const int64_t at = (int64_t)c * vol + s;
+            const cplx<T> a = v[at];
+            z[at] = {(T)(a.re + (T)o[s * CH + 2 * c]), (T)(a.im + (T)o[s * CH + 2 * c + 1])};
         }
-    }
 }
 
-// Training sample from three right-hand sides y and solutions x of a CN
-// solve: input r = y / |y| (BF16), target t = (x - y) / |y| (FP32 [V][18]).
-__global__ void k_make_sample(const cplx<double> *__restrict__ y, const cplx<double> *__restrict__ x, double inv,
+// Training sample of one column of a CN solve: right-hand side y, solution
+// x = A^-1 y and the features f1 = beta K y, f2 = beta K f1; input
+// [y, f1, f2] / |y| (BF16), target (x - y) / |y| in channels 0..5 and zero in
+// 6..17 (FP32 [V][18]).
+__global__ void k_make_sample(const cplx<double> *__restrict__ y, const cplx<double> *__restrict__ x,
+                              const cplx<double> *__restrict__ f1, const cplx<double> *__restrict__ f2, double inv,
                               bf16 *__restrict__ act, float *__restrict__ tgt, int64_t vol)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x) {
-        for (int j = 0; j < 3; ++j)
+        const cplx<double> *f[3] = {y, f1, f2};
+        for (int q = 0; q < 3; ++q)
             for (int c = 0; c < 3; ++c) {
-                const int64_t at = (int64_t)j * 3 * vol + (int64_t)c * vol + s;
-                const cplx<double> a = y[at], b = x[at];
-                act[s * CH + 6 * j + 2 * c] = f2bf((float)(a.re * inv));
-                act[s * CH + 6 * j + 2 * c + 1] = f2bf((float)(a.im * inv));
-                tgt[s * CIN + 6 * j + 2 * c] = (float)((b.re - a.re) * inv);
-                tgt[s * CIN + 6 * j + 2 * c + 1] = (float)((b.im - a.im) * inv);
+                const cplx<double> a = f[q][(int64_t)c * vol + s];
+                act[s * CH + 6 * q + 2 * c] = f2bf((float)(a.re * inv));
+                act[s * CH + 6 * q + 2 * c + 1] = f2bf((float)(a.im * inv));
             }
         for (int c = CIN; c < CH; ++c) act[s * CH + c] = 0;
+        for (int c = 0; c < 3; ++c) {
+            const int64_t at = (int64_t)c * vol + s;
+            tgt[s * CIN + 2 * c] = (float)((x[at].re - y[at].re) * inv);
+            tgt[s * CIN + 2 * c + 1] = (float)((x[at].im - y[at].im) * inv);
+        }
+        for (int c = 6; c < CIN; ++c) tgt[s * CIN + c] = 0.f;
     }
 }
 
@@ -402,8 +404,10 @@ static int forward(nrq_cnn *n)
 }
 
 // loss of the current output against tgt (added to n->loss) and the weight
-// gradient into n->grad
-static int backward(nrq_cnn *n, const float *tgt, double norm)
+// gradient into n->grad; with dtrace (host, NL x V x 32 bf16), the
+// gradient at the output of each layer l (the d of its weight gradient) is
+// copied to dtrace[l]
+static int backward(nrq_cnn *n, const float *tgt, double norm, bf16 *dtrace = nullptr)
 {
     const geom &g = n->g;
     const int nblk = (int)((g.vol + DW_SITES - 1) / DW_SITES);
@@ -411,6 +415,9 @@ static int backward(nrq_cnn *n, const float *tgt, double norm)
     hipLaunchKernelGGL(k_sum, dim3(1), dim3(64), 0, stream(), (const double *)n->lpart, 512, n->loss);
     bf16 *d = n->d0, *dn = n->d1;
     for (int l = NL - 1; l >= 0; --l) {
+        if (dtrace)
+            NRQ_HIP(hipMemcpyAsync(dtrace + (size_t)l * g.vol * CH, d, sizeof(bf16) * (size_t)g.vol * CH,
+                                   hipMemcpyDeviceToHost, stream()));
         hipLaunchKernelGGL(k_dw, dim3(nblk), dim3(256), 0, stream(), (const bf16 *)d, (const bf16 *)n->act[l], n->part, g);
         hipLaunchKernelGGL(k_dw_fold, dim3((WL + 255) / 256), dim3(256), 0, stream(), (const float *)n->part, nblk,
                            n->grad + (int64_t)l * WL);
@@ -434,25 +441,25 @@ static int upload_weights(nrq_cnn *n)
     return NRQ_OK;
 }
 
-int cnn_precondition(nrq_cnn *net, cplx<double> *const *v, cplx<double> *const *z, int R)
+template <typename T>
+int cnn_precondition(nrq_cnn *net, const cplx<T> *const *v, const cplx<T> *const *f1, const cplx<T> *const *f2,
+                     cplx<T> *const *z, int R, const int *act)
 {
     const geom g = lat().g;
     if (int e = ensure_geom(net, g)) return e;
-    for (int r0 = 0; r0 < R; r0 += 3) {
-        const cplx<double> *vi[3] = {nullptr, nullptr, nullptr};
-        cplx<double> *zi[3] = {nullptr, nullptr, nullptr};
-        for (int j = 0; j < 3 && r0 + j < R; ++j) {
-            vi[j] = v[r0 + j];
-            zi[j] = z[r0 + j];
-        }
-        hipLaunchKernelGGL(k_pack, dim3(1024), dim3(256), 0, stream(), vi[0], vi[1], vi[2], net->act[0], g.vol);
+    for (int r = 0; r < R; ++r) {
+        if (act && !act[r]) continue;
+        hipLaunchKernelGGL(k_pack<T>, dim3(1024), dim3(256), 0, stream(), v[r], f1[r], f2[r], net->act[0], g.vol);
         if (int e = forward(net)) return e;
-        hipLaunchKernelGGL(k_unpack, dim3(1024), dim3(256), 0, stream(), (const float *)net->out, vi[0], vi[1] ? vi[1] : vi[0],
-                           vi[2] ? vi[2] : vi[0], zi[0], zi[1], zi[2], g.vol);
+        hipLaunchKernelGGL(k_unpack<T>, dim3(1024), dim3(256), 0, stream(), (const float *)net->out, v[r], z[r], g.vol);
         NRQ_HIP(hipGetLastError());
     }
     return NRQ_OK;
 }
+template int cnn_precondition<double>(nrq_cnn *, const cplx<double> *const *, const cplx<double> *const *,
+                                      const cplx<double> *const *, cplx<double> *const *, int, const int *);
+template int cnn_precondition<float>(nrq_cnn *, const cplx<float> *const *, const cplx<float> *const *,
+                                     const cplx<float> *const *, cplx<float> *const *, int, const int *);
 
 } // namespace nrq
 
@@ -567,6 +574,12 @@ extern "C" int nrq_cnn_forward(nrq_cnn *n, const float *in, float *out)
 
 extern "C" int nrq_cnn_loss_grad(nrq_cnn *n, const float *in, const float *target, double norm, double *loss, float *grad)
 {
+    return nrq_cnn_loss_grad_trace(n, in, target, norm, loss, grad, nullptr, nullptr);
+}
+
+extern "C" int nrq_cnn_loss_grad_trace(nrq_cnn *n, const float *in, const float *target, double norm, double *loss,
+                                       float *grad, float *acts, float *deltas)
+{
     NEED_NET(n);
     if (!lat().initialised) return fail(NRQ_ERR_STATE, "lattice: not initialised");
     if (!in || !target || !loss || !grad || !(norm > 0)) return fail(NRQ_ERR_ARG, "cnn: bad argument");
@@ -580,12 +593,21 @@ extern "C" int nrq_cnn_loss_grad(nrq_cnn *n, const float *in, const float *targe
     NRQ_HIP(hipMemcpyAsync(n->act[0], a.data(), a.size() * sizeof(bf16), hipMemcpyHostToDevice, stream()));
     NRQ_HIP(hipMemcpyAsync(t, target, sizeof(float) * (size_t)g.vol * CIN, hipMemcpyHostToDevice, stream()));
     NRQ_HIP(hipMemsetAsync(n->loss, 0, sizeof(double), stream()));
+    const size_t lay = (size_t)g.vol * CH;
+    std::vector<bf16> at(acts ? NL * lay : 0), dt(deltas ? NL * lay : 0);
     int e = forward(n);
-    if (!e) e = backward(n, t, norm);
+    if (!e && acts)
+        for (int l = 0; l < NL && !e; ++l)
+            if (hipMemcpyAsync(at.data() + l * lay, n->act[l], sizeof(bf16) * lay, hipMemcpyDeviceToHost, stream()) !=
+                hipSuccess)
+                e = fail(NRQ_ERR_HIP, "cnn: activation copy failed");
+    if (!e) e = backward(n, t, norm, deltas ? dt.data() : nullptr);
     if (!e) {
         NRQ_HIP(hipMemcpyAsync(grad, n->grad, sizeof(float) * NW, hipMemcpyDeviceToHost, stream()));
         NRQ_HIP(hipMemcpyAsync(loss, n->loss, sizeof(double), hipMemcpyDeviceToHost, stream()));
         NRQ_HIP(hipStreamSynchronize(stream()));
+        for (size_t k = 0; k < at.size(); ++k) acts[k] = bf2f(at[k]);
+        for (size_t k = 0; k < dt.size(); ++k) deltas[k] = bf2f(dt[k]);
     }
     (void)hipFree(t);
     return e;
@@ -624,11 +646,14 @@ extern "C" int nrq_cnn_harvest(nrq_cnn *n, const nrq_gauge *gf, double mass, dou
         nrq_solve_stats st = {0, 0, 0, 0.0};
         status = krylov_cn_step(gf, fields, R, alpha, beta, p, &st);
         if (status) break;
-        // right-hand sides y of the solve (slot order) and the solutions in psi
+        // right-hand sides y of the solve (slot order), their features
+        // beta K y and (beta K)^2 y, and the solutions in psi
         const cplx<double> *y = krylov_last_rhs();
+        const cplx<double> *f1 = nullptr, *f2 = nullptr;
+        status = krylov_rhs_features(gf, R, &f1, &f2);
         double yn2[R];
-        status = krylov_norms2(y, R, yn2);
-        for (int q = 0; q < R / 3 && !status; ++q) {
+        if (!status) status = krylov_norms2(y, R, yn2);
+        for (int q = 0; q < R && !status; ++q) {
             cnn_sample smp;
             if (hipMalloc(&smp.act, sizeof(bf16) * (size_t)g.vol * CH) != hipSuccess ||
                 hipMalloc(&smp.tgt, sizeof(float) * (size_t)g.vol * CIN) != hipSuccess) {
@@ -637,9 +662,10 @@ extern "C" int nrq_cnn_harvest(nrq_cnn *n, const nrq_gauge *gf, double mass, dou
                 status = fail(NRQ_ERR_ALLOC, "cnn: training set allocation failed");
                 break;
             }
-            const double nrm = std::sqrt(yn2[3 * q] + yn2[3 * q + 1] + yn2[3 * q + 2]);
-            hipLaunchKernelGGL(k_make_sample, dim3(1024), dim3(256), 0, stream(), y + (int64_t)3 * q * fs,
-                               (const cplx<double> *)psi + (int64_t)3 * q * fs, 1.0 / nrm, smp.act, smp.tgt, g.vol);
+            const int64_t off = (int64_t)q * fs;
+            hipLaunchKernelGGL(k_make_sample, dim3(1024), dim3(256), 0, stream(), y + off,
+                               (const cplx<double> *)psi + off, f1 + off, f2 + off, 1.0 / std::sqrt(yn2[q]), smp.act,
+                               smp.tgt, g.vol);
             n->data.push_back(smp);
         }
     }

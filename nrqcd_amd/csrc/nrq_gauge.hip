@@ -229,6 +229,122 @@ __global__ __launch_bounds__(BS) void k_fused(lf_batch<R> bt, geom g, R dt, lf_m
     store_link_out<NT>(uout, g.vol, i, s, evolve_link(ui, e, dt));
 }
 
+// The fused pass with the neighbour links staged in LDS.  A block owns a
+// TX x TY tile of (x, y) and ZC consecutive z planes of one configuration
+// (blockIdx.y); thread (site of the tile, direction i) works on (x, y, z, i)
+// for z in its run.  LDS holds a ring of three z planes of links, each the
+// tile with a one-site halo in x and y, all three directions and nine
+// elements ([slot][dir * 9 + e][halo site]); planes z - 1, z, z + 1 serve
+// every staple of plane z (the 13 links of (x, i) lie in them).  The next
+// plane streams into registers while plane z is computed and replaces plane
+// z - 1 afterwards, so each link is read from HBM once per run of ZC planes
+// (plus the x/y halo, from L2, and two planes per run).  Per (site, dir) the
+// arithmetic is that of k_fused with full neighbour links.  The lattice
+// extents must be multiples of TX, TY and ZC.
+template <typename R, int TX, int TY, int ZC, bool NT>
+__global__ __launch_bounds__(TX *TY * 3) void k_fused_tile(lf_batch<R> bt, geom g, R dt, int xcd)
+{
+    constexpr int HX = TX + 2, HY = TY + 2, HS = HX * HY;
+    constexpr int NTH = TX * TY * 3;
+    constexpr int PL = 27 * HS;                 // complex numbers per plane
+    constexpr int PER = (PL + NTH - 1) / NTH;   // per thread and plane
+    __shared__ cplx<R> lds[3 * PL];
+    const cplx<R> *__restrict__ uin = bt.uin[blockIdx.y];
+    cplx<R> *__restrict__ uout = bt.uout[blockIdx.y];
+    R *__restrict__ ef = bt.ef[blockIdx.y];
+    const int ntx = g.n0 / TX, nty = g.n1 / TY;
+    const unsigned total = (unsigned)ntx * (unsigned)nty * (unsigned)(g.n2 / ZC);
+    const unsigned lb = xcd_block(total, xcd);
+    const int bx = (int)(lb % (unsigned)ntx), by = (int)((lb / (unsigned)ntx) % (unsigned)nty);
+    const int bz = (int)(lb / ((unsigned)ntx * (unsigned)nty));
+    const int x0 = bx * TX, y0 = by * TY, z0 = bz * ZC;
+    const int t = threadIdx.x;
+    const int64_t vol = g.vol;
+    const int64_t plane = (int64_t)g.n0 * g.n1;
+
+    // global offset (site) of halo element q of a plane, and its LDS slot index
+    cplx<R> pre[PER];
+    auto gload = [&](int z) {
+        const int zz = z < 0 ? z + g.n2 : (z >= g.n2 ? z - g.n2 : z);
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int q = t + k * NTH;
+            if (q < PL) {
+                const int hs = q % HS, de = q / HS;
+                const int hx = hs % HX, hy = hs / HX;
+                int gx = x0 - 1 + hx, gy = y0 - 1 + hy;
+                gx = gx < 0 ? gx + g.n0 : (gx >= g.n0 ? gx - g.n0 : gx);
+                gy = gy < 0 ? gy + g.n1 : (gy >= g.n1 ? gy - g.n1 : gy);
+                pre[k] = uin[(int64_t)de * vol + gx + (int64_t)g.n0 * gy + plane * zz];
+            }
+        }
+    };
+    auto sstore = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < PER; ++k) {
+            const int q = t + k * NTH;
+            if (q < PL) lds[slot * PL + q] = pre[k];
+        }
+    };
+    gload(z0 - 1);
+    sstore(0);
+    gload(z0);
+    sstore(1);
+    gload(z0 + 1);
+    sstore(2);
+    __syncthreads();
+
+    const int i = t / (TX * TY);
+    const int r = t - i * (TX * TY);
+    const int tx = r % TX, ty = r / TX;
+    // link of direction d at offset (dx, dy) in the plane held by `slot`
+    auto lnk = [&](int slot, int d, int dx, int dy) {
+        mat3<R> m;
+        const cplx<R> *b = lds + slot * PL + d * 9 * HS + (tx + 1 + dx) + HX * (ty + 1 + dy);
+#pragma unroll
+        for (int e = 0; e < 9; ++e) m.m[e] = b[e * HS];
+        return m;
+    };
+    for (int kz = 0; kz < ZC; ++kz) {
+        const int z = z0 + kz;
+        const bool more = kz + 1 < ZC;
+        if (more) gload(z + 2);
+        const int sm = kz % 3, s0 = (kz + 1) % 3, sp = (kz + 2) % 3;
+        // staples, k in increasing order as in the reference's sum
+        mat3<R> sum;
+        set_zero(sum);
+        for (int k = 0; k < 3; ++k) {
+            if (k == i) continue;
+            const R coef = dt * R((g.a[0] / g.a[k + 1]) * (g.a[0] / g.a[k + 1]));
+            // offsets of +i and +k: (dx, dy) in the plane, or the next plane for direction 2
+            const int ix = i == 0, iy = i == 1, kx = k == 0, ky = k == 1;
+            const int si = i == 2 ? sp : s0; // plane of x + i
+            const int sk = k == 2 ? sp : s0; // plane of x + k
+            const int smk = k == 2 ? sm : s0; // plane of x - k
+            // plane of x + i - k
+            const int sik = i == 2 ? (k == 2 ? s0 : sp) : (k == 2 ? sm : s0);
+            mat3<R> su = mul_nd(lnk(si, k, ix, iy), lnk(sk, i, kx, ky));
+            su = mul_nd(su, lnk(s0, k, 0, 0));
+            mat3<R> sd = mul_dd(lnk(sik, k, ix - kx, iy - ky), lnk(smk, i, -kx, -ky));
+            sd = mul(sd, lnk(smk, k, -kx, -ky));
+#pragma unroll
+            for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * (su.m[m] + sd.m[m]);
+        }
+        const int64_t s = (x0 + tx) + (int64_t)g.n0 * (y0 + ty) + plane * z;
+        R e[8];
+#pragma unroll
+        for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * vol + s);
+        const mat3<R> ui = lnk(s0, i, 0, 0);
+        apply_force(ui, sum, e);
+#pragma unroll
+        for (int a = 0; a < 8; ++a) st_e<NT>(ef + ((int64_t)i * 8 + a) * vol + s, e[a]);
+        store_link_out<NT>(uout, vol, i, s, evolve_link(ui, e, dt));
+        __syncthreads(); // plane z - 1 is no longer read
+        if (more) sstore(sm);
+        __syncthreads();
+    }
+}
+
 template <typename R> __global__ void k_cold(cplx<R> *u, R *ef, int64_t vol)
 {
     const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -547,10 +663,14 @@ struct lf_variant {
     int nt;  // streaming loads/stores of E and of the new links
     int rec; // neighbour links read as two rows (third rebuilt)
 };
+// Modes 3 and 4 are the LDS-tiled fused pass (k_fused_tile, 16 x 8 tiles in
+// FP32 and 16 x 4 in FP64, runs of 16 or 8 z planes; bs is its thread count
+// in FP32); their first and last half steps use the consecutive-site map.
 static const lf_variant LF_VARIANTS[] = {
     {256, 8, 1, 1, 1}, {128, 8, 1, 1, 1}, {64, 8, 1, 1, 1}, {256, 8, 0, 1, 1}, {128, 8, 0, 1, 1}, {64, 8, 0, 1, 1},
     {256, 8, 2, 1, 1}, {256, 8, 2, 0, 1}, {256, 8, 0, 0, 1}, {64, 8, 0, 0, 1},  {256, 8, 1, 1, 0}, {256, 8, 0, 0, 0},
-    {256, 8, 2, 0, 0}, {64, 8, 0, 0, 0},  {256, 1, 0, 0, 1}};
+    {256, 8, 2, 0, 0}, {64, 8, 0, 0, 0},  {256, 1, 0, 0, 1}, {384, 8, 3, 0, 0}, {384, 8, 3, 1, 0}, {384, 8, 4, 0, 0},
+    {384, 8, 4, 1, 0}};
 constexpr int LF_NVARIANTS = sizeof(LF_VARIANTS) / sizeof(LF_VARIANTS[0]);
 constexpr int LF_DEFAULT = 11; // {256, 8, 0, 0, 0}: used without tuning, valid on every lattice
 
@@ -562,11 +682,16 @@ struct lf_choice {
 static std::vector<lf_choice> g_lf_choice;
 static int g_autotune = -1; // -1: unset (env NRQ_AUTOTUNE, default on)
 
-static bool variant_ok(int variant, const geom &g)
+// Whether shape `variant` applies to lattice g at storage precision prec
+// (64 or 32).  The LDS tiles are FP32 only: a 16 x 4 FP64 tile fits LDS, but
+// at one wavefront per SIMD its prefetch registers do not fit.
+static bool variant_ok(int variant, const geom &g, int prec)
 {
     const lf_variant v = LF_VARIANTS[variant];
     if (v.mode == 1) return g.n0 <= v.bs;
     if (v.mode == 2) return g.n0 >= 16;
+    if (v.mode >= 3)
+        return prec == 32 && g.n0 % 16 == 0 && g.n1 % 8 == 0 && g.n2 % (v.mode == 3 ? 16 : 8) == 0;
     return true;
 }
 
@@ -592,7 +717,17 @@ static bool variant_ok(int variant, const geom &g)
 static lf_map make_map(int variant, const geom &g)
 {
     const lf_variant v = LF_VARIANTS[variant];
+    if (v.mode >= 3) return make_lf_map(g, 256, v.xcd, 0, 3);
     return make_lf_map(g, v.bs, v.xcd, v.mode, 3);
+}
+
+template <typename R, int ZC, bool NTV>
+static void launch_tile(const lf_batch<R> &bt, int nb, const geom &g, R dt, int xcd)
+{
+    constexpr int TX = 16, TY = 8;
+    const unsigned total = (unsigned)(g.n0 / TX) * (unsigned)(g.n1 / TY) * (unsigned)(g.n2 / ZC);
+    hipLaunchKernelGGL((k_fused_tile<R, TX, TY, ZC, NTV>), dim3(total, (unsigned)nb), dim3(TX * TY * 3), 0, stream(), bt,
+                       g, dt, xcd);
 }
 
 template <typename R> static void launch_links(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
@@ -619,6 +754,14 @@ template <typename R> static void launch_efield(int variant, const lf_batch<R> &
 
 template <typename R> static void launch_fused(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
 {
+    const lf_variant tv = LF_VARIANTS[variant];
+    if constexpr (sizeof(R) == 4) if (tv.mode >= 3) {
+        if (tv.mode == 3)
+            tv.nt ? launch_tile<R, 16, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 16, false>(bt, nb, g, dt, tv.xcd);
+        else
+            tv.nt ? launch_tile<R, 8, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 8, false>(bt, nb, g, dt, tv.xcd);
+        return;
+    }
     const lf_map m = make_map(variant, g);
     const dim3 grd((unsigned)(m.ngroup * m.nsb), (unsigned)nb);
     switch (m.bs) {
@@ -716,7 +859,7 @@ template <typename R> static int tune_leapfrog(const nrq_gauge *h, R dt, int *be
     const double tol = shape_tolerance<R>();
     std::vector<char> valid(LF_NVARIANTS, 0);
     for (int v = 0; v < LF_NVARIANTS; ++v) {
-        if (!variant_ok(v, g)) continue;
+        if (!variant_ok(v, g, (int)(8 * sizeof(R)))) continue;
         NRQ_HIP(hipMemcpyAsync(eb_, e0, eb, hipMemcpyDeviceToDevice, stream()));
         launch_fused<R>(v, one<R>(u0, ub, eb_), 1, g, dt);
         NRQ_HIP(hipGetLastError());
@@ -792,7 +935,7 @@ template <typename R> static int leapfrog_variant(const nrq_gauge *h, R dt, int 
     const geom &g = lat().g;
     const int f = pinned_shape();
     if (f >= 0) {
-        if (variant_ok(f, g)) {
+        if (variant_ok(f, g, h->precision)) {
             *variant = f;
             return NRQ_OK;
         }
@@ -826,6 +969,17 @@ struct lf_graph {
 static std::vector<lf_graph> g_lf_graphs;
 constexpr size_t LF_GRAPH_CACHE = 8;
 
+// Span timer of the fused leapfrog launches (measurement only): while it is
+// on, every leapfrog call that is not graph-captured records one event pair
+// on the library stream around its run of fused launches, into the next of
+// the reserved spans (runs past the last span are counted as skipped).
+struct lf_timer {
+    std::vector<hipEvent_t> ev; // start, stop per span
+    int64_t used = 0, launches = 0, site_steps = 0, skipped = 0;
+    bool on = false;
+};
+static lf_timer g_lf_timer;
+
 template <typename R> static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, double sw, int64_t n, bool graph)
 {
     const geom g = lat().g;
@@ -844,24 +998,31 @@ template <typename R> static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, 
     std::vector<int> cur(nb);
     for (int i = 0; i < nb; ++i) cur[i] = hs[i]->cur;
     const int64_t flips = sw > 0 ? n - 1 : n;
-    auto issue = [&]() {
+    auto issue = [&](bool timed) {
         std::vector<int> c = cur;
-        if (sw > 0) {
-            launch_links<R>(v, table(c, false), nb, g, dt);
-            for (int64_t k = 1; k < n; ++k) {
-                launch_fused<R>(v, table(c, true), nb, g, dt);
-                for (int &x : c) x ^= 1;
-            }
-            launch_efield<R>(v, table(c, false), nb, g, dt);
-        } else {
-            for (int64_t k = 0; k < n; ++k) {
-                launch_fused<R>(v, table(c, true), nb, g, dt);
-                for (int &x : c) x ^= 1;
-            }
+        const int64_t nfused = sw > 0 ? n - 1 : n;
+        lf_timer &tm = g_lf_timer;
+        bool rec = false;
+        if (timed && tm.on && nfused > 0) {
+            rec = tm.used < (int64_t)tm.ev.size() / 2;
+            if (!rec) tm.skipped += nfused;
         }
+        if (sw > 0) launch_links<R>(v, table(c, false), nb, g, dt);
+        if (rec) (void)hipEventRecord(tm.ev[2 * tm.used], stream());
+        for (int64_t k = 0; k < nfused; ++k) {
+            launch_fused<R>(v, table(c, true), nb, g, dt);
+            for (int &x : c) x ^= 1;
+        }
+        if (rec) {
+            (void)hipEventRecord(tm.ev[2 * tm.used + 1], stream());
+            ++tm.used;
+            tm.launches += nfused;
+            tm.site_steps += nfused * nb * g.vol;
+        }
+        if (sw > 0) launch_efield<R>(v, table(c, false), nb, g, dt);
     };
     if (!graph || n < 2) {
-        issue();
+        issue(true);
         NRQ_HIP(hipGetLastError());
     } else {
         // the captured launches carry the extents, the spacings (staple
@@ -885,7 +1046,7 @@ template <typename R> static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, 
         if (!exec) {
             hipGraph_t gr = nullptr;
             NRQ_HIP(hipStreamBeginCapture(stream(), hipStreamCaptureModeThreadLocal));
-            issue();
+            issue(false);
             const hipError_t le = hipGetLastError();
             const hipError_t ce = hipStreamEndCapture(stream(), &gr);
             NRQ_HIP(le);
@@ -1139,6 +1300,40 @@ extern "C" int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps
     return NRQ_OK;
 }
 
+extern "C" int nrq_leapfrog_timer_start(int64_t max_spans)
+{
+    if (max_spans < 1 || max_spans > 1000000) return fail(NRQ_ERR_ARG, "leapfrog timer: bad span count %lld", (long long)max_spans);
+    if (int r = ensure_device()) return r;
+    lf_timer &tm = g_lf_timer;
+    while ((int64_t)tm.ev.size() < 2 * max_spans) {
+        hipEvent_t e = nullptr;
+        NRQ_HIP(hipEventCreate(&e));
+        tm.ev.push_back(e);
+    }
+    tm.used = tm.launches = tm.site_steps = tm.skipped = 0;
+    tm.on = true;
+    return NRQ_OK;
+}
+
+extern "C" int nrq_leapfrog_timer_stop(double *fused_ms, int64_t *launches, int64_t *site_steps, int64_t *skipped)
+{
+    lf_timer &tm = g_lf_timer;
+    if (!tm.on) return fail(NRQ_ERR_STATE, "leapfrog timer: not started");
+    tm.on = false;
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    double ms = 0;
+    for (int64_t k = 0; k < tm.used; ++k) {
+        float t = 0;
+        NRQ_HIP(hipEventElapsedTime(&t, tm.ev[2 * k], tm.ev[2 * k + 1]));
+        ms += t;
+    }
+    if (fused_ms) *fused_ms = ms;
+    if (launches) *launches = tm.launches;
+    if (site_steps) *site_steps = tm.site_steps;
+    if (skipped) *skipped = tm.skipped;
+    return NRQ_OK;
+}
+
 extern "C" int nrq_gauge_leapfrog_batch(nrq_gauge *const *h, int n, double stepwidth, int64_t nsteps)
 {
     if (!h || n < 1) return fail(NRQ_ERR_ARG, "leapfrog batch: no configurations");
@@ -1339,7 +1534,7 @@ extern "C" int nrq_leapfrog_shape_info(int k, int *block_sites, int *xcd_remap, 
     *tiling = v.mode;
     *streaming = v.nt;
     *two_row = v.rec;
-    *applies = lat().initialised ? (variant_ok(k, lat().g) ? 1 : 0) : 0;
+    *applies = lat().initialised ? (variant_ok(k, lat().g, 64) ? 1 : 0) | (variant_ok(k, lat().g, 32) ? 2 : 0) : 0;
     return NRQ_OK;
 }
 

@@ -21,6 +21,7 @@ enum krylov_kind {
     KRYLOV_CHEB_FP64 = 2,   // Chebyshev iteration on the known spectral segment, FP64
     KRYLOV_CHEB_MIXED = 3,  // FP32 Chebyshev corrections inside FP64 iterative refinement
     KRYLOV_FGMRES_CNN = 4,  // FGMRES(m) in FP64, right-preconditioned by the learned network
+    KRYLOV_FGMRES_CNN_MIXED = 5, // FP32 FGMRES(m) with the network inside FP64 iterative refinement
 };
 
 struct krylov_params {
@@ -48,6 +49,10 @@ void krylov_release();
 const cplx<double> *krylov_last_rhs();
 // squared norms of nf consecutive FP64 fields into out[nf] (host)
 int krylov_norms2(const cplx<double> *x, int nf, double *out);
+// Features of the right-hand sides y of the last krylov_cn_step on gf:
+// f1_r = beta_r K y_r and f2_r = beta_r K f1_r (FP64, slot-major like y),
+// valid until the next solve.
+int krylov_rhs_features(const nrq_gauge *gf, int R, const cplx<double> **f1, const cplx<double> **f2);
 // partial sums for the correlator kernel: device scratch of
 // KRYLOV_REDUCE_DOUBLES doubles
 constexpr size_t KRYLOV_REDUCE_DOUBLES = (size_t)NSLOTS * 256 * 8;

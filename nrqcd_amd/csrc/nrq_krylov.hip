@@ -10,7 +10,13 @@
 //  * KRYLOV_GMRES_MIXED: FP64 iterative refinement around FP32 GMRES(m)
 //    corrections.
 //  * KRYLOV_FGMRES_CNN: flexible GMRES(m) in FP64 whose right preconditioner
-//    is the learned network of nrq_cnn.hip (z = v + N(v)).
+//    is the learned network of nrq_cnn.hip, z = v + N(v, beta K v,
+//    (beta K)^2 v): the network sees each direction with its two covariant
+//    features (K is the covariant Laplacian, so the inputs transform like v
+//    under gauge transformations, and the beta scaling makes them
+//    mass-independent).
+//  * KRYLOV_FGMRES_CNN_MIXED: FP64 iterative refinement around FP32 FGMRES(m)
+//    corrections with the same network.
 //  * KRYLOV_CHEB_FP64 / KRYLOV_CHEB_MIXED: Chebyshev iteration on the known
 //    spectral segment of A_r, in FP64 or as FP32 corrections inside FP64
 //    iterative refinement.  Step counts are planned from a priori bounds, so
@@ -573,6 +579,7 @@ struct workspace {
     cplx<float> *y32 = nullptr, *x32 = nullptr, *t32 = nullptr, *res32 = nullptr;
     cplx<float> *links32 = nullptr;    // FP32 copy of the links for the inner solves
     cplx<double> *beta = nullptr, *cpar = nullptr, *hdev = nullptr, *ydev = nullptr, *cpart = nullptr;
+    cplx<double> *negone = nullptr;    // -1 per slot: k_stencil with it as gco gives beta K in
     cplx<double> *cheb = nullptr;      // Chebyshev coefficients [iteration][slot][2]
     double *rpart = nullptr, *spart = nullptr, *nrm = nullptr;
     int *active = nullptr, *nused = nullptr;
@@ -589,7 +596,7 @@ struct workspace {
     void release()
     {
         void *dev[] = {basis, y, x, t, res, y32, x32, t32, res32, links32, beta, cpar, hdev, ydev, cpart,
-                       rpart, spart, nrm, active, nused, cheb};
+                       rpart, spart, nrm, active, nused, cheb, negone};
         for (void *q : dev)
             if (q) (void)hipFree(q);
         if (hbuf) (void)hipHostFree(hbuf);
@@ -619,6 +626,7 @@ struct workspace {
                   dm((void **)&x32, f32) && dm((void **)&t32, f32) && dm((void **)&res32, f32) &&
                   dm((void **)&links32, (size_t)27 * (size_t)v * sizeof(cplx<float>)) &&
                   dm((void **)&beta, sizeof(cplx<double>) * NSLOTS) && dm((void **)&cpar, sizeof(cplx<double>) * 2 * NSLOTS) &&
+                  dm((void **)&negone, sizeof(cplx<double>) * NSLOTS) &&
                   dm((void **)&cheb, sizeof(cplx<double>) * 2 * NSLOTS * CHEB_MAX) &&
                   dm((void **)&hdev, sizeof(cplx<double>) * NSLOTS * (mm + 1)) &&
                   dm((void **)&ydev, sizeof(cplx<double>) * NSLOTS * (mm + 1)) &&
@@ -632,6 +640,14 @@ struct workspace {
             (void)hipGetLastError();
             release();
             return fail(NRQ_ERR_ALLOC, "CN: workspace allocation failed (V=%lld, m=%d, %d slots)", (long long)v, mm, rc);
+        }
+        {
+            std::vector<cplx<double>> m1(NSLOTS, cplx<double>{-1.0, 0.0});
+            if (hipMemcpy(negone, m1.data(), sizeof(cplx<double>) * NSLOTS, hipMemcpyHostToDevice) != hipSuccess) {
+                (void)hipGetLastError();
+                release();
+                return fail(NRQ_ERR_HIP, "CN: workspace initialisation failed");
+            }
         }
         vol = v;
         m = mm;
@@ -829,6 +845,30 @@ int krylov_norms2(const cplx<double> *x, int nf, double *out)
     return NRQ_OK;
 }
 
+// f1_r = beta_r K in_r, f2_r = beta_r K f1_r for the active slots of R
+template <typename RL, typename T>
+static int bk_features(const cplx<RL> *u, const slots<T> &in, const slots<T> &f1, const slots<T> &f2, int R)
+{
+    launch_stencil<RL, T>(u, in, f1, R, g_ws.negone);
+    launch_stencil<RL, T>(u, f1, f2, R, g_ws.negone);
+    NRQ_HIP(hipGetLastError());
+    return NRQ_OK;
+}
+
+int krylov_rhs_features(const nrq_gauge *gf, int R, const cplx<double> **f1, const cplx<double> **f2)
+{
+    if (R <= 0 || R > g_ws.rcap) return fail(NRQ_ERR_ARG, "CN: %d features for a workspace of %d slots", R, g_ws.rcap);
+    std::vector<int> all(R, 1);
+    if (int e = upload_active(all)) return e;
+    const slots<double> y = ws_slots(g_ws.y, R), a = ws_slots(g_ws.t, R), b = ws_slots(g_ws.res, R);
+    const int e = gf->precision == 64 ? bk_features<double, double>((const cplx<double> *)gauge_links(gf), y, a, b, R)
+                                      : bk_features<float, double>((const cplx<float> *)gauge_links(gf), y, a, b, R);
+    if (e) return e;
+    *f1 = g_ws.t;
+    *f2 = g_ws.res;
+    return NRQ_OK;
+}
+
 // column solves handed from FGMRES to plain GMRES since the library loaded
 static int64_t g_fallbacks = 0;
 
@@ -839,7 +879,7 @@ static int64_t g_fallbacks = 0;
 // rel_out[r] receives the last true relative residual.  beta must already be
 // on the device.
 // ---------------------------------------------------------------------------
-// With net (FGMRES, T = double): z_j = M(v_j), w = A z_j, x += Z y.  A slot
+// With net (FGMRES): z_j = M(v_j), w = A z_j, x += Z y.  A slot
 // whose network output makes the Arnoldi step non-finite, whose cycle reduces
 // its residual by less than FGMRES_STALL, or that has not converged after
 // FGMRES_CYCLES cycles leaves the solve with x intact and fallen[r] = 1, for
@@ -906,13 +946,20 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
             cplx<T> *vj = basis + (size_t)j * R * n;
             cplx<T> *w = basis + (size_t)(j + 1) * R * n;
             if (net) {
+                // z_j = M(v_j) from v_j and its features beta K v_j (in t) and
+                // (beta K)^2 v_j (in res; the next cycle's residual is
+                // recomputed after this loop)
                 cplx<T> *zj = zbasis + (size_t)j * R * n;
-                cplx<double> *vp[NSLOTS], *zp[NSLOTS];
+                if (int e = bk_features<RL, T>(links, ws_slots(vj, R), ws_slots(t, R), ws_slots(res, R), R)) return e;
+                const cplx<T> *vp[NSLOTS], *f1p[NSLOTS], *f2p[NSLOTS];
+                cplx<T> *zp[NSLOTS];
                 for (int r = 0; r < R; ++r) {
-                    vp[r] = (cplx<double> *)(vj + (size_t)r * n);
-                    zp[r] = (cplx<double> *)(zj + (size_t)r * n);
+                    vp[r] = vj + (size_t)r * n;
+                    f1p[r] = t + (size_t)r * n;
+                    f2p[r] = res + (size_t)r * n;
+                    zp[r] = zj + (size_t)r * n;
                 }
-                if (int e = cnn_precondition(net, vp, zp, R)) return e;
+                if (int e = cnn_precondition<T>(net, vp, f1p, f2p, zp, R, cycle.data())) return e;
                 if (int e = apply_t<RL, T>(links, zj, w, R)) return e;
             } else if (int e = apply_t<RL, T>(links, vj, w, R)) {
                 return e;
@@ -1272,10 +1319,11 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
 }
 
 // FP64 iterative refinement around FP32 GMRES corrections, from the guess in
-// g_ws.x.
+// g_ws.x; with net, the corrections are FP32 FGMRES with the network (slots
+// the network does not help finish their correction with plain FP32 GMRES).
 template <typename RL>
 static int gmres_mixed(const cplx<RL> *u, int R, std::vector<double> &bn, const krylov_params &p, std::vector<int> &act,
-                       std::vector<double> &rel, std::vector<int64_t> &its, int64_t &cycles)
+                       std::vector<double> &rel, std::vector<int64_t> &its, int64_t &cycles, nrq_cnn *net = nullptr)
 {
     const int64_t n = 3 * g_ws.vol;
     const cplx<float> *l32;
@@ -1313,10 +1361,21 @@ static int gmres_mixed(const cplx<RL> *u, int R, std::vector<double> &bn, const 
                            (const double *)rnorm2, g_ws.active);
         std::vector<double> inner_rel(R, 0.0);
         int64_t inner_cycles = 0;
+        const double itol = std::max(p.inner_tol, std::max(p.tol / 1e-3, 2e-7));
+        std::vector<int> fallen(R, 0);
         if (int e = gmres_core<float, float>(l32, R, g_ws.y32, g_ws.x32, (cplx<float> *)g_ws.basis, g_ws.t32, g_ws.res32, true,
-                                             one.data(), act, std::max(p.inner_tol, std::max(p.tol / 1e-3, 2e-7)), p.restart,
-                                             p.maxit, its, inner_rel.data(), inner_cycles))
+                                             one.data(), act, itol, p.restart, p.maxit, its, inner_rel.data(), inner_cycles,
+                                             net, (cplx<float> *)g_ws.zbasis, net ? &fallen : nullptr))
             return e;
+        int nfall = 0;
+        for (int f : fallen) nfall += f;
+        if (nfall) {
+            g_fallbacks += nfall;
+            if (int e = gmres_core<float, float>(l32, R, g_ws.y32, g_ws.x32, (cplx<float> *)g_ws.basis, g_ws.t32, g_ws.res32,
+                                                 false, one.data(), fallen, itol, p.restart, p.maxit, its, inner_rel.data(),
+                                                 inner_cycles))
+                return e;
+        }
         // gmres_core left the slot masks of the inner solve on the device;
         // the refinement updates every slot that is still active here
         if (int e = upload_active(act)) return e;
@@ -1389,6 +1448,11 @@ static int cn_step_t(const cplx<RL> *u, int R, const std::vector<zc> &alpha, con
                                                 false, bn.data(), fallen, p.tol, p.restart, p.maxit, its, rel.data(),
                                                 cycles);
             }
+        } else if (p.kind == KRYLOV_FGMRES_CNN_MIXED) {
+            nrq_cnn *net = cn_preconditioner();
+            if (!net) return fail(NRQ_ERR_STATE, "CN: FGMRES needs a network (nrq_cn_set_preconditioner)");
+            if (int e = reserve_zbasis()) return e;
+            status = gmres_mixed<RL>(u, R, bn, p, act, rel, its, cycles, net);
         } else
             status = gmres_mixed<RL>(u, R, bn, p, act, rel, its, cycles);
     }

@@ -62,6 +62,47 @@ def test_loss_gradient_matches_oracle(nrq, lattice):
         assert _rel(g4[l], r4[l]) < 5e-2, l
 
 
+def test_each_layer_from_the_networks_own_values(nrq, lattice):
+    """Layer by layer from the GPU's own BF16 values, so that a rounding
+    branch taken differently in float64 cannot hide or fake an error: the
+    weight gradient of layer l is sum_s d_l(s) x act_l(s + tap) (float64 here)
+    within 1e-3 relative; each layer input is the BF16 ReLU of the
+    convolution of the previous one, each output gradient the BF16 ReLU-masked
+    transposed convolution of the next, and the last one 2 (N - t) / norm
+    (bars 1e-2: one BF16 rounding)."""
+    ext = (16, 3, 2)
+    _say("cnn layers from the GPU's own activations 16x3x2")
+    lattice(ext, SP)
+    rng = np.random.default_rng(6)
+    v = ext[0] * ext[1] * ext[2]
+    inp = rng.normal(size=(v, co.CIN)).astype(np.float32)
+    tgt = rng.normal(size=(v, co.CIN)).astype(np.float32)
+    w = co.random_weights(rng).astype(np.float32)
+    net = nrq.PreconditionerNet(1)
+    net.weights = w
+    loss, grad, acts, deltas = net.loss_grad_trace(inp, tgt, 3.0)
+    shp = (ext[2], ext[1], ext[0], co.CH)
+    A = [a.reshape(shp).astype(np.float64) for a in acts]
+    D = [d.reshape(shp).astype(np.float64) for d in deltas]
+    ws = [co.bf16(x) for x in co.weights_layers(w)]
+    first = np.zeros(shp)
+    first[..., :co.CIN] = co.bf16(inp).reshape(shp[:3] + (co.CIN,))
+    assert np.array_equal(A[0], first)
+    g4 = grad.reshape(co.NL, co.TAPS, co.CH, co.CH)
+    for l in range(co.NL):
+        ref = np.stack([np.einsum("zyxo,zyxi->oi", D[l], co._shift(A[l], *off)) for off in co.OFFS])
+        assert _rel(g4[l], ref) < 1e-3, l
+    for l in range(co.NL - 1):
+        assert _rel(A[l + 1], co.bf16(np.maximum(co.conv(A[l], ws[l]), 0.0))) < 1e-2, l
+    for l in range(co.NL - 1, 0, -1):
+        assert _rel(D[l - 1], co.bf16(np.where(A[l] > 0, co.conv_t(D[l], ws[l]), 0.0))) < 1e-2, l
+    out = co.conv(A[co.NL - 1], ws[co.NL - 1])[..., :co.CIN]
+    last = np.zeros(shp)
+    last[..., :co.CIN] = co.bf16(2.0 * (out - tgt.reshape(shp[:3] + (co.CIN,))) / 3.0)
+    assert _rel(D[co.NL - 1], last) < 1e-2
+    assert loss == pytest.approx(float(np.sum((out - tgt.reshape(out.shape)) ** 2) / 3.0), rel=1e-3)
+
+
 def _evolved(nrq, seed=7, steps=40):
     nrq.random_init(seed)
     g = nrq.GaugeConfiguration()
@@ -110,11 +151,11 @@ def test_training_and_gain(nrq, oracle, lattice):
     """Train on pairs from plain GMRES solves at Ma = 1, 2, 5 on an evolved
     configuration, then solve held-out masses 3 and 8 with FGMRES + the trained
     network and with GMRES alone.  Shows what the network gains: the loss
-    falls and both reach the same FP64 answer; the iteration counts are
-    printed.  The network sees no gauge links, while the off-diagonal part of
-    the CN operator is the covariant hop U psi(x +- i), so at dt = 0.05 a it
-    can only learn the scalar part of the correction and the iteration counts
-    stay equal; the bar allows up to one more iteration per column solve."""
+    falls, both reach the same FP64 answer, and FGMRES with the network takes
+    no more iterations per column solve than GMRES (the counts are printed;
+    the network sees each direction with its covariant features
+    beta K v and (beta K)^2 v, from which A^-1 v - v = -beta K v +
+    (beta K)^2 v - ... is within reach)."""
     ext = (16, 16, 8)
     _say("CNN training 16x16x8")
     lattice(ext, SP)
@@ -124,7 +165,7 @@ def test_training_and_gain(nrq, oracle, lattice):
         net = nrq.PreconditionerNet(3)
         for m in (1.0, 2.0, 5.0):
             net.harvest(g, m, 4, seed=int(10 * m))
-        assert net.dataset_size() == 3 * 4 * 4
+        assert net.dataset_size() == 3 * 4 * 12
         first = net.train(1, 1e-3, seed=1)
         last = net.train(15, 1e-3, seed=2)
         _say(f"training loss: after 1 epoch {first:.4g}, after 16 epochs {last:.4g}")
@@ -132,7 +173,9 @@ def test_training_and_gain(nrq, oracle, lattice):
         nrq.nrqcd_set_preconditioner(net)
         for m in (3.0, 8.0):
             out = {}
-            for name, kind in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64), ("fgmres_cnn", nrq.NRQ_SOLVER_FGMRES_CNN)):
+            for name, kind in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64), ("fgmres_cnn", nrq.NRQ_SOLVER_FGMRES_CNN),
+                               ("gmres_mixed", nrq.NRQ_SOLVER_GMRES_MIXED),
+                               ("fgmres_cnn_mixed", nrq.NRQ_SOLVER_FGMRES_CNN_MIXED)):
                 nrq.nrqcd_set_solver(kind, 20, 10000)
                 h = nrq.NRQCDField()
                 h.InitSinglePoint(100, 900)
@@ -143,12 +186,12 @@ def test_training_and_gain(nrq, oracle, lattice):
                     its += st.iterations
                     solves += st.solves
                 out[name] = (its / solves, h.download())
-            _say(f"Ma={m}: iterations per column solve: gmres {out['gmres'][0]:.2f}, "
-                 f"fgmres+cnn {out['fgmres_cnn'][0]:.2f}")
-            assert out["fgmres_cnn"][0] <= out["gmres"][0] + 1.0
-            for k in range(2):
-                a, b = out["fgmres_cnn"][1][k], out["gmres"][1][k]
-                assert np.max(np.abs(a - b)) < 1e-8 * np.max(np.abs(b))
+            _say(f"Ma={m}: iterations per column solve: " + ", ".join(f"{k} {v[0]:.2f}" for k, v in out.items()))
+            assert out["fgmres_cnn"][0] <= out["gmres"][0]
+            for name in ("fgmres_cnn", "gmres_mixed", "fgmres_cnn_mixed"):
+                for k in range(2):
+                    a, b = out[name][1][k], out["gmres"][1][k]
+                    assert np.max(np.abs(a - b)) < 1e-8 * np.max(np.abs(b)), name
         nrq.nrqcd_set_preconditioner(None)
     finally:
         nrq.nrqcd_finalize()
@@ -188,6 +231,50 @@ def test_fgmres_bad_network_falls_back_to_gmres(nrq, oracle, lattice, scale):
         assert np.max(np.abs(Ag - A)) < 1e-10 * np.max(np.abs(A))
         if scale > 1e3:
             assert fell > 0
+        nrq.nrqcd_set_preconditioner(None)
+    finally:
+        nrq.nrqcd_finalize()
+
+
+def test_learned_preconditioner_at_128cubed(nrq, lattice):
+    """The learned-preconditioner configuration at its size: the 128^3 seed-3
+    box initial condition evolved to t = 10/Qs (400 steps), pairs from plain
+    GMRES solves at Ma = 1, 2, 5, the network trained on the GPU; at the
+    held-out Ma = 3 and 8, FGMRES(20) with it (FP64, and FP32 inside FP64
+    refinement) reaches FP64 true residual 1e-10 with the answer of
+    GMRES(20), in no more iterations per column solve."""
+    ext = (128, 128, 128)
+    _say("learned preconditioner at 128^3: evolve, harvest, train")
+    lattice(ext, SP)
+    nrq.nrqcd_init(2.0, C1, 1.0, 1e-10)
+    try:
+        g = _evolved(nrq, seed=3, steps=400)
+        net = nrq.PreconditionerNet(3)
+        for m in (1.0, 2.0, 5.0):
+            net.harvest(g, m, 1, seed=int(10 * m))
+        assert net.dataset_size() == 36
+        first = net.train(1, 1e-3, seed=1)
+        last = net.train(4, 1e-3, seed=2)
+        _say(f"training loss: after 1 epoch {first:.4g}, after 5 epochs {last:.4g}")
+        assert last < first
+        nrq.nrqcd_set_preconditioner(net)
+        for m in (3.0, 8.0):
+            out = {}
+            for name, kind in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64), ("fgmres_cnn", nrq.NRQ_SOLVER_FGMRES_CNN),
+                               ("fgmres_cnn_mixed", nrq.NRQ_SOLVER_FGMRES_CNN_MIXED)):
+                nrq.nrqcd_set_solver(kind, 20, 10000)
+                h = nrq.NRQCDField()
+                h.InitSinglePoint(1, 1)
+                st = h.Update(g, m, C1)
+                assert st.max_true_relres <= 1e-10, name
+                out[name] = (st.iterations / st.solves, h.download())
+                h.Destructor()
+            _say(f"128^3 Ma={m}: iterations per column solve: " + ", ".join(f"{k} {v[0]:.2f}" for k, v in out.items()))
+            assert out["fgmres_cnn"][0] <= out["gmres"][0]
+            for name in ("fgmres_cnn", "fgmres_cnn_mixed"):
+                for k in range(2):
+                    a, b = out[name][1][k], out["gmres"][1][k]
+                    assert np.max(np.abs(a - b)) < 1e-9 * np.max(np.abs(b)), name
         nrq.nrqcd_set_preconditioner(None)
     finally:
         nrq.nrqcd_finalize()

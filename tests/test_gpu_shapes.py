@@ -57,24 +57,28 @@ def test_every_leapfrog_shape_matches_oracle_fp64(nrq, oracle, lattice, pinned, 
     assert tried >= 9
 
 
-def test_every_leapfrog_shape_matches_oracle_fp32(nrq, oracle, lattice, pinned):
-    ext = (16, 16, 16)
-    lattice(ext, SP)
-    oracle.lattice_init(ext, SP)
+@pytest.mark.parametrize("ext,sp", [((16, 16, 16), SP), ((32, 16, 24), (0.1, 1.0, 0.8, 1.3))])
+def test_every_leapfrog_shape_matches_oracle_fp32(nrq, oracle, lattice, pinned, ext, sp):
+    lattice(ext, sp)
+    oracle.lattice_init(ext, sp)
     U0, E0 = oracle.box_init(3, 0.5, 1.0, 1.0)
     U, E = U0.copy(order="F"), E0.copy(order="F")
     oracle.leapfrog(U, E, 1.0, 10)
     g = nrq.GaugeConfiguration(nrq.NRQ_PRECISION_FP32)
+    tried = []
     for k, sh in enumerate(nrq.leapfrog_shapes()):
-        if not sh["applies"]:
+        if not sh["applies_fp32"]:
             continue
-        _say(f"leapfrog shape {k} on {ext} fp32")
+        tried.append(k)
+        _say(f"leapfrog shape {k} on {ext} {sp} fp32")
         nrq.leapfrog_set_shape(k)
         g.upload(U0, E0)
         g.Update(1.0, 10)
         Ug, Eg = g.download()
         assert np.max(np.abs(Ug - U)) < 2e-5, k
         assert np.max(np.abs(Eg - E)) < 2e-5 * max(1.0, np.max(np.abs(E))), k
+    # the LDS-tiled shapes are among them
+    assert any(nrq.leapfrog_shapes()[k]["tiling"].startswith("LDS") for k in tried)
 
 
 def test_pinned_shape_that_does_not_apply_is_an_error(nrq, lattice, pinned):
