@@ -180,3 +180,27 @@ def test_batch_graph_follows_a_change_of_spacings(nrq, lattice, monkeypatch):
             assert np.array_equal(Ua, Ub) and np.array_equal(Ea, Eb), sp
         for x in ens + ref:
             x.Deallocate()
+
+
+def test_fused_launch_timer_counts_the_fused_passes(nrq, lattice):
+    """The kernel timer behind the benchmark's roofline: an n-step forward
+    Update is one links-only pass, n - 1 fused passes and one E-only pass, so
+    it times n - 1 fused launches; a backward Update is n fused launches;
+    runs past the reserved spans are counted as skipped; the timer must be
+    started before it is stopped."""
+    ext = (16, 16, 16)
+    lattice(ext, SP)
+    nrq.random_init(4)
+    g = nrq.GaugeConfiguration()
+    g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+    _say("fused-launch timer")
+    nrq.leapfrog_timer_start(2)
+    g.Update(1.0, 10)
+    g.Update(-1.0, 4)
+    g.Update(1.0, 5)  # past the two spans
+    t = nrq.leapfrog_timer_stop()
+    assert t["launches"] == 9 + 4 and t["skipped"] == 4
+    assert t["site_steps"] == 13 * 16 ** 3
+    assert t["ms"] > 0
+    with pytest.raises(nrq.NRQError):
+        nrq.leapfrog_timer_stop()
