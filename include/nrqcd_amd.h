@@ -363,6 +363,17 @@ int nrq_heavy_correlators(const nrq_heavy *h, double out[8]);
 /* GetMesoncorrelator_{O1_1S0, O1_3S1, O8_1S0, O8_3S1}_ZeroMomentum_gconjg,
  * nrqcd.f90:1728-1905 (conj(A) in place of A^dag); same order as above */
 int nrq_heavy_correlators_gconjg(const nrq_heavy *h, double out[8]);
+/* nrq_heavy_correlators without any allocation: no counterpart in the
+ * reference.  For callers that cannot allocate device or pinned memory
+ * (graph-capturing or tight serving loops, harnesses that forbid hipMalloc):
+ * the plain correlator of nrq_heavy_correlators computed by the same kernels
+ * with the partial sums in a fixed 524,416-byte device array of the library
+ * (one per GPU, held until the process ends) instead of the solver workspace
+ * that nrq_heavy_correlators reserves on first use.  Any lattice size fits
+ * (the trace kernel strides over the sites); a result is returned to the
+ * host, so the call synchronises the library stream.  Calls that overlap on
+ * one GPU share that array and race. */
+int nrq_heavy_correlators_noalloc(const nrq_heavy *h, double out[8]);
 
 /* ---- spectral functions (main.f90 ComputeSpectrum, twfft.f90) --------- */
 

@@ -168,6 +168,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_heavy_norms": (I, [P, P, P]),
         "nrq_heavy_correlators": (I, [P, P]),
         "nrq_heavy_correlators_gconjg": (I, [P, P]),
+        "nrq_heavy_correlators_noalloc": (I, [P, P]),
         "nrq_spectral_functions": (I, [P, I64, I64, D, P, P]),
         "nrq_gauge_box_ic_correlators": (I, [P, D, D, D, P, P]),
         "nrq_gauge_transverse_init": (I, [P, P, I, P, P]),

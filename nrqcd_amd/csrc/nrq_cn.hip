@@ -555,3 +555,60 @@ extern "C" int nrq_heavy_correlators_gconjg(const nrq_heavy *h, double out[8])
 {
     return heavy_correlators(h, true, out);
 }
+
+// ---------------------------------------------------------------------------
+// Allocation-free correlators.  The partial sums live in `scratch`, a
+// fixed-size device array of this file: the partials of CORR_BLOCKS_MAX
+// blocks and the CORR_SUMS folded sums (524,416 bytes), enough for any
+// lattice because k_corr_traces strides over the sites of larger ones.  Each
+// GPU holds its own copy until the process ends; its address is looked up
+// once per device (hipGetSymbolAddress, with that device current) and kept,
+// and only that address is handed to kernels.  Every block writes its
+// partials and the fold writes every sum, so nothing needs clearing between
+// calls; calls that overlap on one GPU (other host threads, other streams)
+// share the array and race.
+// ---------------------------------------------------------------------------
+namespace nrq {
+constexpr size_t CORR_SCRATCH_DOUBLES = (size_t)CORR_SUMS * (CORR_BLOCKS_MAX + 1);
+static __device__ __align__(256) double scratch[CORR_SCRATCH_DOUBLES];
+constexpr int SCRATCH_MAX_DEVICES = 64;
+static double *g_scratch_dev[SCRATCH_MAX_DEVICES]; // device address of `scratch`, per device
+
+static int corr_scratch(double **out)
+{
+    int dev = -1;
+    NRQ_HIP(hipGetDevice(&dev));
+    if (dev < 0 || dev >= SCRATCH_MAX_DEVICES)
+        return fail(NRQ_ERR_ARG, "heavy: device %d beyond the %d devices the correlator scratch serves", dev,
+                    SCRATCH_MAX_DEVICES);
+    if (!g_scratch_dev[dev]) {
+        void *p = nullptr;
+        NRQ_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return fail(NRQ_ERR_HIP, "heavy: no device address for the correlator scratch on device %d", dev);
+        g_scratch_dev[dev] = (double *)p;
+    }
+    *out = g_scratch_dev[dev];
+    return NRQ_OK;
+}
+} // namespace nrq
+
+extern "C" int nrq_heavy_correlators_noalloc(const nrq_heavy *h, double out[8])
+{
+    if (!out) return fail(NRQ_ERR_ARG, "null argument");
+    NEED_HEAVY(h);
+    double *part = nullptr;
+    if (int r = corr_scratch(&part)) return r;
+    const int nb = corr_blocks(h->vol);
+    if ((size_t)CORR_SUMS * ((size_t)nb + 1) > CORR_SCRATCH_DOUBLES)
+        return fail(NRQ_ERR_ARG, "heavy: %d partial blocks exceed the correlator scratch of %d", nb, CORR_BLOCKS_MAX);
+    double *res = part + (size_t)CORR_SUMS * nb;
+    hipLaunchKernelGGL(k_corr_traces, dim3(nb), dim3(REDUCE_THREADS), 0, stream(), (const cplx<double> *)h->psi,
+                       h->vol, part);
+    hipLaunchKernelGGL(k_fold, dim3(CORR_SUMS), dim3(REDUCE_THREADS), 0, stream(), part, nb, res);
+    NRQ_HIP(hipGetLastError());
+    double sums[CORR_SUMS];
+    NRQ_HIP(hipMemcpyAsync(sums, res, sizeof(sums), hipMemcpyDeviceToHost, stream()));
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    correlators_from_sums(sums, false, out);
+    return NRQ_OK;
+}
