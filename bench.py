@@ -50,6 +50,7 @@ C1 = [1, 0, 0, 0]
 LOG_EVERY = 50        # steps between logged observables (energy, Gauss law, plaquette)
 REUNIT_EVERY = 10     # FP32 storage: FP64 reunitarisation interval
 HBM_PEAK_GBS = 8000.0
+EPOCHS = 10           # training passes of the learned preconditioner
 # Algorithmic HBM bytes per site of one fused leapfrog launch (E update of one
 # step with the link update of the next): U (3 x 9 complex) and E (3 x 8
 # reals) read once and written once.
@@ -166,7 +167,7 @@ def _train(nrq, g_ic):
     """The learned-preconditioner data: the seed-3 initial condition evolved
     to t = 10/Qs (400 steps of 0.05 a), pairs from one plain GMRES solve of
     12 random fields at each of Ma = 1, 2, 5 (36 whole-lattice pairs, each
-    512 16^3 patches), 5 epochs of Adam on the GPU."""
+    512 16^3 patches), EPOCHS epochs of Adam on the GPU."""
     g = g_ic.copy()
     g.Update(1.0, 400)
     net = nrq.PreconditionerNet(3)
@@ -174,26 +175,31 @@ def _train(nrq, g_ic):
     for m in (1.0, 2.0, 5.0):
         net.harvest(g, m, 1, seed=int(10 * m))
     loss0 = net.train(1, 1e-3, seed=1)
-    loss1 = net.train(4, 1e-3, seed=2)
+    loss1 = net.train(EPOCHS - 1, 1e-3, seed=2)
     nrq.synchronize()
     info = {"configuration": f"seed {SEED} evolved 400 steps (t = 10/Qs)", "training_pairs": net.dataset_size(),
-            "patches_16cubed": net.dataset_size() * (nrq.lattice_size() // 4096), "epochs": 5,
+            "patches_16cubed": net.dataset_size() * (nrq.lattice_size() // 4096), "epochs": EPOCHS,
             "loss_first_epoch": loss0, "loss_last_epoch": loss1, "harvest_and_training_s": time.perf_counter() - t0}
     return net, g, info
 
 
-def _held_out(nrq, g, cn_steps: int) -> dict:
+def _held_out(nrq, g, net, cn_steps: int) -> dict:
     """FGMRES(20) with the trained network (FP64, and FP32 in FP64
-    refinement) against GMRES(20) and mixed GMRES(20) at the held-out
-    Ma = 3 and 8 on the evolved configuration (point sources, FP64 true
-    residual 1e-10)."""
+    refinement) against GMRES(20), mixed GMRES(20) and FGMRES(20) with the
+    untrained (zero) network, i.e. the second-order Neumann polynomial the
+    network corrects, at the held-out Ma = 3 and 8 on the evolved
+    configuration (point sources, FP64 true residual 1e-10)."""
+    zero = nrq.PreconditionerNet(3)
+    zero.weights = np.zeros(nrq.NRQ_CNN_NWEIGHTS, np.float32)
     out = {}
     for m in (3.0, 8.0):
         row = {}
-        for name, kind in (("jacobi_gmres", nrq.NRQ_SOLVER_GMRES_FP64),
-                           ("learned_fgmres", nrq.NRQ_SOLVER_FGMRES_CNN),
-                           ("mixed_gmres", nrq.NRQ_SOLVER_GMRES_MIXED),
-                           ("learned_mixed_fgmres", nrq.NRQ_SOLVER_FGMRES_CNN_MIXED)):
+        for name, kind, pre in (("jacobi_gmres", nrq.NRQ_SOLVER_GMRES_FP64, None),
+                                ("polynomial_fgmres", nrq.NRQ_SOLVER_FGMRES_CNN, zero),
+                                ("learned_fgmres", nrq.NRQ_SOLVER_FGMRES_CNN, net),
+                                ("mixed_gmres", nrq.NRQ_SOLVER_GMRES_MIXED, None),
+                                ("learned_mixed_fgmres", nrq.NRQ_SOLVER_FGMRES_CNN_MIXED, net)):
+            nrq.nrqcd_set_preconditioner(pre)
             before = nrq.cn_fallbacks()
             row[name] = _cn_leg(nrq, g, kind, 20, m, cn_steps)
             row[name]["fallbacks_to_gmres"] = nrq.cn_fallbacks() - before
@@ -202,6 +208,8 @@ def _held_out(nrq, g, cn_steps: int) -> dict:
             "cn_steps_per_s": row["learned_fgmres"]["cn_steps_per_s"] / base["cn_steps_per_s"],
             "iterations": row["learned_fgmres"]["iterations_per_column_solve"] / base["iterations_per_column_solve"]}
         out[str(m)] = row
+    nrq.nrqcd_set_preconditioner(net)
+    del zero
     return out
 
 
@@ -305,7 +313,7 @@ def main() -> None:
     cn_fp32_links = {name: _cn_leg(nrq, g32, kind, 30, 2.0, args.cn_steps) for name, kind in solvers
                      if "mixed" in name}
     if net is not None:
-        learned["held_out"] = _held_out(nrq, g_ev, args.cn_steps)
+        learned["held_out"] = _held_out(nrq, g_ev, net, args.cn_steps)
         nrq.nrqcd_set_preconditioner(None)
         del net, g_ev
     nrq.nrqcd_finalize()

@@ -402,8 +402,10 @@ int nrq_w2t(double *data, int64_t n, double dt);
  * features beta K v and (beta K)^2 v, with K the covariant Laplacian of the
  * CN operator 1 + beta K: the inputs transform like v under gauge
  * transformations and do not depend on the mass.  FGMRES uses
- * z = v + N(v, beta K v, (beta K)^2 v) (output channels 0..5) as its right
- * preconditioner.  The lattice x extent must be a multiple of 16. */
+ * z = v - beta K v + (beta K)^2 v + N(v, beta K v, (beta K)^2 v) (output
+ * channels 0..5) as its right preconditioner: the second-order Neumann
+ * polynomial of (1 + beta K)^-1 corrected by the network (with all weights 0,
+ * the polynomial alone).  The lattice x extent must be a multiple of 16. */
 typedef struct nrq_cnn nrq_cnn;
 #define NRQ_CNN_NWEIGHTS (4 * 27 * 32 * 32) /* [layer][tap][co][ci], tap = 9(dz+1)+3(dy+1)+(dx+1);
                                                channels past 18 of layer 1 inputs and
@@ -428,9 +430,10 @@ int nrq_cnn_loss_grad_trace(nrq_cnn *net, const float *in, const float *target, 
  * residual 1e-12) of CN systems on g at `mass` and `stepwidth` (c1 = 1):
  * each solve takes 12 random Gaussian fields psi (6 quark, 6 antiquark),
  * forms y = S psi and solves S^+ x = y; every field gives one pair: input
- * [y, beta K y, (beta K)^2 y] / |y|, target (x - y) / |y| (the error of the
- * zero guess minus the residual) in channels 0..5 and zero in 6..17.  All
- * pairs of one network come from one lattice size. */
+ * [y, beta K y, (beta K)^2 y] / |y|, target (x - (y - beta K y +
+ * (beta K)^2 y)) / |y| (what the Neumann polynomial misses of the zero guess's
+ * error) in channels 0..5 and zero in 6..17.  All pairs of one network come
+ * from one lattice size. */
 int nrq_cnn_harvest(nrq_cnn *net, const nrq_gauge *g, double mass, double stepwidth, int nsolves, uint64_t seed);
 int nrq_cnn_dataset_size(const nrq_cnn *net, int64_t *count);
 /* `epochs` passes of Adam (learning rate lr, one pair per step, order

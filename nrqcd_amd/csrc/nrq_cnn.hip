@@ -1,11 +1,14 @@
 // nrq_cnn.hip -- learned preconditioner for the CN systems: a 4-layer 3-D
 // convolutional network N (18 -> 32 -> 32 -> 32 -> 18 real channels, 3^3
-// periodic kernels, ReLU between layers, no biases) acting on three colour
-// vector fields at once (3 fields x 3 colours x re/im = 18 channels per site).
-// The preconditioner is z = v + N(v): the untrained network (all weights 0)
-// is the identity, and training fits N(r) to e - r on residual/error pairs
-// (r, e), r = A e, harvested from plain GMRES solves.  N is positively
-// homogeneous (ReLU, no biases), so it commutes with the scale of its input.
+// periodic kernels, ReLU between layers, no biases) acting on one colour
+// vector field v and its covariant features f1 = beta K v, f2 = beta K f1
+// (3 features x 3 colours x re/im = 18 channels per site) of the CN operator
+// A = 1 + beta K.  The preconditioner is z = v - f1 + f2 + N(v, f1, f2): the
+// second-order Neumann polynomial of A^-1 corrected by the network (all
+// weights 0: the polynomial alone), and training fits N(r, ...) to
+// e - (r - f1 + f2) on residual/error pairs (r, e), r = A e, harvested from
+// plain GMRES solves.  N is positively homogeneous (ReLU, no biases), so it
+// commutes with the scale of its input.
 //
 // Convolutions are implicit GEMMs on BF16 matrix cores with FP32
 // accumulation: a wavefront computes 16 consecutive sites x 32 output
@@ -263,22 +266,26 @@ __global__ void k_pack(const cplx<T> *__restrict__ v, const cplx<T> *__restrict_
     }
 }
 
-// z = v + N(v) from output channels 0..5 (colour c at 2 c, 2 c + 1)
+// z = v - f1 + f2 + N(v, f1, f2): the second-order Neumann polynomial of
+// (1 + beta K)^-1 plus the network's correction from output channels 0..5
+// (colour c at 2 c, 2 c + 1)
 template <typename T>
-__global__ void k_unpack(const float *__restrict__ o, const cplx<T> *__restrict__ v, cplx<T> *__restrict__ z, int64_t vol)
+__global__ void k_unpack(const float *__restrict__ o, const cplx<T> *__restrict__ v, const cplx<T> *__restrict__ f1,
+                         const cplx<T> *__restrict__ f2, cplx<T> *__restrict__ z, int64_t vol)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x)
         for (int c = 0; c < 3; ++c) {
             const int64_t at = (int64_t)c * vol + s;
-            const cplx<T> a = v[at];
-            z[at] = {(T)(a.re + (T)o[s * CH + 2 * c]), (T)(a.im + (T)o[s * CH + 2 * c + 1])};
+            const cplx<T> a = v[at], b = f1[at], d = f2[at];
+            z[at] = {(T)((a.re - b.re + d.re) + (T)o[s * CH + 2 * c]), (T)((a.im - b.im + d.im) + (T)o[s * CH + 2 * c + 1])};
         }
 }
 
 // Training sample of one column of a CN solve: right-hand side y, solution
 // x = A^-1 y and the features f1 = beta K y, f2 = beta K f1; input
-// [y, f1, f2] / |y| (BF16), target (x - y) / |y| in channels 0..5 and zero in
-// 6..17 (FP32 [V][18]).
+// [y, f1, f2] / |y| (BF16), target (x - (y - f1 + f2)) / |y| (what the
+// second-order Neumann polynomial misses) in channels 0..5 and zero in 6..17
+// (FP32 [V][18]).
 __global__ void k_make_sample(const cplx<double> *__restrict__ y, const cplx<double> *__restrict__ x,
                               const cplx<double> *__restrict__ f1, const cplx<double> *__restrict__ f2, double inv,
                               bf16 *__restrict__ act, float *__restrict__ tgt, int64_t vol)
@@ -294,8 +301,8 @@ __global__ void k_make_sample(const cplx<double> *__restrict__ y, const cplx<dou
         for (int c = CIN; c < CH; ++c) act[s * CH + c] = 0;
         for (int c = 0; c < 3; ++c) {
             const int64_t at = (int64_t)c * vol + s;
-            tgt[s * CIN + 2 * c] = (float)((x[at].re - y[at].re) * inv);
-            tgt[s * CIN + 2 * c + 1] = (float)((x[at].im - y[at].im) * inv);
+            tgt[s * CIN + 2 * c] = (float)((x[at].re - (y[at].re - f1[at].re + f2[at].re)) * inv);
+            tgt[s * CIN + 2 * c + 1] = (float)((x[at].im - (y[at].im - f1[at].im + f2[at].im)) * inv);
         }
         for (int c = 6; c < CIN; ++c) tgt[s * CIN + c] = 0.f;
     }
@@ -451,7 +458,8 @@ int cnn_precondition(nrq_cnn *net, const cplx<T> *const *v, const cplx<T> *const
         if (act && !act[r]) continue;
         hipLaunchKernelGGL(k_pack<T>, dim3(1024), dim3(256), 0, stream(), v[r], f1[r], f2[r], net->act[0], g.vol);
         if (int e = forward(net)) return e;
-        hipLaunchKernelGGL(k_unpack<T>, dim3(1024), dim3(256), 0, stream(), (const float *)net->out, v[r], z[r], g.vol);
+        hipLaunchKernelGGL(k_unpack<T>, dim3(1024), dim3(256), 0, stream(), (const float *)net->out, v[r], f1[r], f2[r],
+                           z[r], g.vol);
         NRQ_HIP(hipGetLastError());
     }
     return NRQ_OK;

@@ -10,11 +10,12 @@
 //  * KRYLOV_GMRES_MIXED: FP64 iterative refinement around FP32 GMRES(m)
 //    corrections.
 //  * KRYLOV_FGMRES_CNN: flexible GMRES(m) in FP64 whose right preconditioner
-//    is the learned network of nrq_cnn.hip, z = v + N(v, beta K v,
-//    (beta K)^2 v): the network sees each direction with its two covariant
-//    features (K is the covariant Laplacian, so the inputs transform like v
-//    under gauge transformations, and the beta scaling makes them
-//    mass-independent).
+//    is z = v - f1 + f2 + N(v, f1, f2) with f1 = beta K v, f2 = beta K f1:
+//    the second-order Neumann polynomial of A^-1 corrected by the learned
+//    network of nrq_cnn.hip, which sees each direction with its two
+//    covariant features (K is the covariant Laplacian, so the inputs
+//    transform like v under gauge transformations, and the beta scaling
+//    makes them mass-independent).
 //  * KRYLOV_FGMRES_CNN_MIXED: FP64 iterative refinement around FP32 FGMRES(m)
 //    corrections with the same network.
 //  * KRYLOV_CHEB_FP64 / KRYLOV_CHEB_MIXED: Chebyshev iteration on the known

@@ -111,11 +111,14 @@ def _evolved(nrq, seed=7, steps=40):
     return g
 
 
-def test_fgmres_identity_network_matches_gmres_and_oracle(nrq, oracle, lattice):
-    """With all weights zero the preconditioner is the identity: FGMRES takes
-    the same iterations as GMRES and both reproduce the oracle's CN step."""
+def test_fgmres_zero_network_is_the_neumann_polynomial(nrq, oracle, lattice):
+    """With all weights zero the preconditioner is the second-order Neumann
+    polynomial z = v - beta K v + (beta K)^2 v of (1 + beta K)^-1: FGMRES
+    then needs fewer iterations than GMRES (the preconditioned operator is
+    1 - (beta K)^3 + ..., |beta K| <= 0.075 here), and both reproduce the
+    oracle's CN step."""
     ext = (16, 6, 5)
-    _say("FGMRES identity network")
+    _say("FGMRES zero network")
     lattice(ext, SP)
     nrq.nrqcd_init(2.0, C1, 1.0, 1e-12)
     try:
@@ -141,7 +144,8 @@ def test_fgmres_identity_network_matches_gmres_and_oracle(nrq, oracle, lattice):
         for name in res:
             assert np.max(np.abs(res[name][0] - Q)) < 1e-10 * np.max(np.abs(Q)), name
             assert np.max(np.abs(res[name][1] - A)) < 1e-10 * np.max(np.abs(A)), name
-        assert its["fgmres"] == its["gmres"]
+        _say(f"iterations: gmres {its['gmres']}, fgmres with the polynomial {its['fgmres']}")
+        assert its["fgmres"] < its["gmres"]
         nrq.nrqcd_set_preconditioner(None)
     finally:
         nrq.nrqcd_finalize()
@@ -150,12 +154,11 @@ def test_fgmres_identity_network_matches_gmres_and_oracle(nrq, oracle, lattice):
 def test_training_and_gain(nrq, oracle, lattice):
     """Train on pairs from plain GMRES solves at Ma = 1, 2, 5 on an evolved
     configuration, then solve held-out masses 3 and 8 with FGMRES + the trained
-    network and with GMRES alone.  Shows what the network gains: the loss
-    falls, both reach the same FP64 answer, and FGMRES with the network takes
-    no more iterations per column solve than GMRES (the counts are printed;
-    the network sees each direction with its covariant features
-    beta K v and (beta K)^2 v, from which A^-1 v - v = -beta K v +
-    (beta K)^2 v - ... is within reach)."""
+    network, with FGMRES + the zero network (the Neumann polynomial alone)
+    and with GMRES.  Shows what the network gains: the loss falls, all reach
+    the same FP64 answer, and the trained network takes no more iterations
+    per column solve than the polynomial alone, which takes fewer than GMRES
+    (the counts are printed)."""
     ext = (16, 16, 8)
     _say("CNN training 16x16x8")
     lattice(ext, SP)
@@ -170,12 +173,16 @@ def test_training_and_gain(nrq, oracle, lattice):
         last = net.train(15, 1e-3, seed=2)
         _say(f"training loss: after 1 epoch {first:.4g}, after 16 epochs {last:.4g}")
         assert last < first
-        nrq.nrqcd_set_preconditioner(net)
+        zero = nrq.PreconditionerNet(3)
+        zero.weights = np.zeros(nrq.NRQ_CNN_NWEIGHTS, np.float32)
         for m in (3.0, 8.0):
             out = {}
-            for name, kind in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64), ("fgmres_cnn", nrq.NRQ_SOLVER_FGMRES_CNN),
-                               ("gmres_mixed", nrq.NRQ_SOLVER_GMRES_MIXED),
-                               ("fgmres_cnn_mixed", nrq.NRQ_SOLVER_FGMRES_CNN_MIXED)):
+            for name, kind, pre in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64, None),
+                                    ("fgmres_polynomial", nrq.NRQ_SOLVER_FGMRES_CNN, zero),
+                                    ("fgmres_cnn", nrq.NRQ_SOLVER_FGMRES_CNN, net),
+                                    ("gmres_mixed", nrq.NRQ_SOLVER_GMRES_MIXED, None),
+                                    ("fgmres_cnn_mixed", nrq.NRQ_SOLVER_FGMRES_CNN_MIXED, net)):
+                nrq.nrqcd_set_preconditioner(pre)
                 nrq.nrqcd_set_solver(kind, 20, 10000)
                 h = nrq.NRQCDField()
                 h.InitSinglePoint(100, 900)
@@ -187,8 +194,9 @@ def test_training_and_gain(nrq, oracle, lattice):
                     solves += st.solves
                 out[name] = (its / solves, h.download())
             _say(f"Ma={m}: iterations per column solve: " + ", ".join(f"{k} {v[0]:.2f}" for k, v in out.items()))
-            assert out["fgmres_cnn"][0] <= out["gmres"][0]
-            for name in ("fgmres_cnn", "gmres_mixed", "fgmres_cnn_mixed"):
+            assert out["fgmres_polynomial"][0] < out["gmres"][0]
+            assert out["fgmres_cnn"][0] <= out["fgmres_polynomial"][0]
+            for name in ("fgmres_polynomial", "fgmres_cnn", "gmres_mixed", "fgmres_cnn_mixed"):
                 for k in range(2):
                     a, b = out[name][1][k], out["gmres"][1][k]
                     assert np.max(np.abs(a - b)) < 1e-8 * np.max(np.abs(b)), name
@@ -242,7 +250,8 @@ def test_learned_preconditioner_at_128cubed(nrq, lattice):
     GMRES solves at Ma = 1, 2, 5, the network trained on the GPU; at the
     held-out Ma = 3 and 8, FGMRES(20) with it (FP64, and FP32 inside FP64
     refinement) reaches FP64 true residual 1e-10 with the answer of
-    GMRES(20), in no more iterations per column solve."""
+    GMRES(20), in fewer iterations per column solve (the iterations of the
+    Neumann polynomial alone are printed beside them)."""
     ext = (128, 128, 128)
     _say("learned preconditioner at 128^3: evolve, harvest, train")
     lattice(ext, SP)
@@ -257,11 +266,15 @@ def test_learned_preconditioner_at_128cubed(nrq, lattice):
         last = net.train(4, 1e-3, seed=2)
         _say(f"training loss: after 1 epoch {first:.4g}, after 5 epochs {last:.4g}")
         assert last < first
-        nrq.nrqcd_set_preconditioner(net)
+        zero = nrq.PreconditionerNet(3)
+        zero.weights = np.zeros(nrq.NRQ_CNN_NWEIGHTS, np.float32)
         for m in (3.0, 8.0):
             out = {}
-            for name, kind in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64), ("fgmres_cnn", nrq.NRQ_SOLVER_FGMRES_CNN),
-                               ("fgmres_cnn_mixed", nrq.NRQ_SOLVER_FGMRES_CNN_MIXED)):
+            for name, kind, pre in (("gmres", nrq.NRQ_SOLVER_GMRES_FP64, None),
+                                    ("fgmres_polynomial", nrq.NRQ_SOLVER_FGMRES_CNN, zero),
+                                    ("fgmres_cnn", nrq.NRQ_SOLVER_FGMRES_CNN, net),
+                                    ("fgmres_cnn_mixed", nrq.NRQ_SOLVER_FGMRES_CNN_MIXED, net)):
+                nrq.nrqcd_set_preconditioner(pre)
                 nrq.nrqcd_set_solver(kind, 20, 10000)
                 h = nrq.NRQCDField()
                 h.InitSinglePoint(1, 1)
@@ -270,8 +283,8 @@ def test_learned_preconditioner_at_128cubed(nrq, lattice):
                 out[name] = (st.iterations / st.solves, h.download())
                 h.Destructor()
             _say(f"128^3 Ma={m}: iterations per column solve: " + ", ".join(f"{k} {v[0]:.2f}" for k, v in out.items()))
-            assert out["fgmres_cnn"][0] <= out["gmres"][0]
-            for name in ("fgmres_cnn", "fgmres_cnn_mixed"):
+            assert out["fgmres_cnn"][0] < out["gmres"][0]
+            for name in ("fgmres_polynomial", "fgmres_cnn", "fgmres_cnn_mixed"):
                 for k in range(2):
                     a, b = out[name][1][k], out["gmres"][1][k]
                     assert np.max(np.abs(a - b)) < 1e-9 * np.max(np.abs(b)), name
