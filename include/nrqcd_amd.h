@@ -221,9 +221,9 @@ int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval);
 /* Launch shape the leapfrog autotuner chose for this configuration's
  * precision and lattice size: threads per block, XCD remap factor (1 = none),
  * tiling (0: consecutive sites, 1: z-sweep tiles of full x-rows, 2: z-sweep
- * tiles 16 sites wide, 3 and 4: LDS-staged 16 x 8 (FP32) or 16 x 4 (FP64)
- * tiles over runs of 16 or 8 z planes, neighbour links read once per run and
- * shared through LDS; extents multiples of 16, 8 and the run), streaming
+ * tiles 16 sites wide, 3 to 6: LDS-staged 16 x 8 tiles over runs of 16 or 8
+ * z planes, with three (3, 4) or two (5, 6) planes of links in LDS, FP32
+ * storage only, extents multiples of 16, 8 and the run), streaming
  * (1: nontemporal loads/stores of E and the
  * new links) and two_row (1: neighbour links read as two rows, the third
  * rebuilt as the conjugate cross product, exact for SU(3)).  The first
@@ -399,13 +399,15 @@ int nrq_w2t(double *data, int64_t n, double dt);
  * wide, 3^3 periodic kernels, ReLU, no biases, BF16 weights and activations
  * with FP32 accumulation on matrix cores.  Its 18 real input channels per
  * site are one colour-vector field v (3 colours x re/im) and its covariant
- * features beta K v and (beta K)^2 v, with K the covariant Laplacian of the
- * CN operator 1 + beta K: the inputs transform like v under gauge
- * transformations and do not depend on the mass.  FGMRES uses
- * z = v - beta K v + (beta K)^2 v + N(v, beta K v, (beta K)^2 v) (output
- * channels 0..5) as its right preconditioner: the second-order Neumann
- * polynomial of (1 + beta K)^-1 corrected by the network (with all weights 0,
- * the polynomial alone).  The lattice x extent must be a multiple of 16. */
+ * features K v / kmax and K^2 v / kmax^2, with K the covariant Laplacian of
+ * the CN operator 1 + beta K: the inputs transform like v under gauge
+ * transformations.  FGMRES uses z = v - beta K v + (beta K)^2 v
+ * - (beta kmax)^3 N(v, K v / kmax, K^2 v / kmax^2) (output channels 0..5,
+ * kmax = sum_i 2 / a_i^2) as its right preconditioner: the second-order
+ * Neumann polynomial of (1 + beta K)^-1 corrected by the network in units of
+ * the polynomial's leading remainder, so that one network serves every mass
+ * (with all weights 0, the polynomial alone).  The lattice x extent must be
+ * a multiple of 16. */
 typedef struct nrq_cnn nrq_cnn;
 #define NRQ_CNN_NWEIGHTS (4 * 27 * 32 * 32) /* [layer][tap][co][ci], tap = 9(dz+1)+3(dy+1)+(dx+1);
                                                channels past 18 of layer 1 inputs and
@@ -430,10 +432,11 @@ int nrq_cnn_loss_grad_trace(nrq_cnn *net, const float *in, const float *target, 
  * residual 1e-12) of CN systems on g at `mass` and `stepwidth` (c1 = 1):
  * each solve takes 12 random Gaussian fields psi (6 quark, 6 antiquark),
  * forms y = S psi and solves S^+ x = y; every field gives one pair: input
- * [y, beta K y, (beta K)^2 y] / |y|, target (x - (y - beta K y +
- * (beta K)^2 y)) / |y| (what the Neumann polynomial misses of the zero guess's
- * error) in channels 0..5 and zero in 6..17.  All pairs of one network come
- * from one lattice size. */
+ * [y, K y / kmax, K^2 y / kmax^2] / |y| (kmax = sum_i 2 / a_i^2), target
+ * (x - (y - beta K y + (beta K)^2 y)) / (-(beta kmax)^3 |y|) (what the
+ * Neumann polynomial misses of the zero guess's error, in units of its
+ * leading term) in channels 0..5 and zero in 6..17: the same functions of y
+ * at every mass.  All pairs of one network come from one lattice size. */
 int nrq_cnn_harvest(nrq_cnn *net, const nrq_gauge *g, double mass, double stepwidth, int nsolves, uint64_t seed);
 int nrq_cnn_dataset_size(const nrq_cnn *net, int64_t *count);
 /* `epochs` passes of Adam (learning rate lr, one pair per step, order

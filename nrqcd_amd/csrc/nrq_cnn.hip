@@ -247,18 +247,40 @@ __global__ void k_adam(float *__restrict__ w, const float *__restrict__ g, float
     wt[(int64_t)l * WL + ((int64_t)(26 - tap) * CH + ci) * CH + co] = f2bf(w[i]);
 }
 
+// Complex scale factors of one column with CN coefficient beta and kmax =
+// sum_i 2 / a_i^2 (the top of the spectrum of K): the network's inputs are v,
+// f1 s1 = K v / kmax and f2 s2 = K^2 v / kmax^2, and its output is the
+// remainder of the Neumann polynomial divided by c = -(beta kmax)^3, whose
+// leading term is c (K / kmax)^3 v.  Inputs and target are then the same
+// functions of v whatever the mass (beta).
+struct col_scale {
+    cplx<double> s1, s2, c, cinv;
+};
+__host__ inline col_scale make_col_scale(cplx<double> beta, double kmax)
+{
+    const std::complex<double> b(beta.re * kmax, beta.im * kmax);
+    const std::complex<double> i1 = 1.0 / b, i2 = i1 * i1, c = -(b * b * b), ci = 1.0 / c;
+    return {{i1.real(), i1.imag()}, {i2.real(), i2.imag()}, {c.real(), c.imag()}, {ci.real(), ci.imag()}};
+}
+__device__ __forceinline__ cplx<double> cmuld(cplx<double> a, cplx<double> b)
+{
+    return {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+}
+
 // BF16 network input of one colour field v (complex colour-major, 3V) and
 // its features f1 = beta K v, f2 = beta K f1: channel 6 q + 2 c + {re, im}
-// of feature q (v, f1, f2) and colour c; channels >= 18 zero.
+// of feature q (v, f1 s1, f2 s2) and colour c; channels >= 18 zero.
 template <typename T>
 __global__ void k_pack(const cplx<T> *__restrict__ v, const cplx<T> *__restrict__ f1, const cplx<T> *__restrict__ f2,
-                       bf16 *__restrict__ act, int64_t vol)
+                       bf16 *__restrict__ act, int64_t vol, col_scale sc)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x) {
         const cplx<T> *f[3] = {v, f1, f2};
+        const cplx<double> m[3] = {{1.0, 0.0}, sc.s1, sc.s2};
         for (int q = 0; q < 3; ++q)
             for (int c = 0; c < 3; ++c) {
-                const cplx<T> a = f[q][(int64_t)c * vol + s];
+                const cplx<T> a0 = f[q][(int64_t)c * vol + s];
+                const cplx<double> a = cmuld({(double)a0.re, (double)a0.im}, m[q]);
                 act[s * CH + 6 * q + 2 * c] = f2bf((float)a.re);
                 act[s * CH + 6 * q + 2 * c + 1] = f2bf((float)a.im);
             }
@@ -266,43 +288,50 @@ __global__ void k_pack(const cplx<T> *__restrict__ v, const cplx<T> *__restrict_
     }
 }
 
-// z = v - f1 + f2 + N(v, f1, f2): the second-order Neumann polynomial of
-// (1 + beta K)^-1 plus the network's correction from output channels 0..5
-// (colour c at 2 c, 2 c + 1)
+// z = v - f1 + f2 + c N(v, f1 s1, f2 s2): the second-order Neumann
+// polynomial of (1 + beta K)^-1 plus the network's correction, output
+// channels 0..5 (colour c at 2 c, 2 c + 1) scaled by c = -(beta kmax)^3
 template <typename T>
 __global__ void k_unpack(const float *__restrict__ o, const cplx<T> *__restrict__ v, const cplx<T> *__restrict__ f1,
-                         const cplx<T> *__restrict__ f2, cplx<T> *__restrict__ z, int64_t vol)
+                         const cplx<T> *__restrict__ f2, cplx<T> *__restrict__ z, int64_t vol, col_scale sc)
 {
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x)
         for (int c = 0; c < 3; ++c) {
             const int64_t at = (int64_t)c * vol + s;
             const cplx<T> a = v[at], b = f1[at], d = f2[at];
-            z[at] = {(T)((a.re - b.re + d.re) + (T)o[s * CH + 2 * c]), (T)((a.im - b.im + d.im) + (T)o[s * CH + 2 * c + 1])};
+            const cplx<double> n = cmuld({(double)o[s * CH + 2 * c], (double)o[s * CH + 2 * c + 1]}, sc.c);
+            z[at] = {(T)((a.re - b.re + d.re) + n.re), (T)((a.im - b.im + d.im) + n.im)};
         }
 }
 
 // Training sample of one column of a CN solve: right-hand side y, solution
-// x = A^-1 y and the features f1 = beta K y, f2 = beta K f1; input
-// [y, f1, f2] / |y| (BF16), target (x - (y - f1 + f2)) / |y| (what the
-// second-order Neumann polynomial misses) in channels 0..5 and zero in 6..17
-// (FP32 [V][18]).
+// x = A^-1 y, the features f1 = beta K y, f2 = beta K f1 and the column's
+// scales (col_scale); input [y, f1 s1, f2 s2] / |y| (BF16), target
+// (x - (y - f1 + f2)) / (c |y|) (what the second-order Neumann polynomial
+// misses, in units of its leading term's scale) in channels 0..5 and zero in
+// 6..17 (FP32 [V][18]).
 __global__ void k_make_sample(const cplx<double> *__restrict__ y, const cplx<double> *__restrict__ x,
                               const cplx<double> *__restrict__ f1, const cplx<double> *__restrict__ f2, double inv,
-                              bf16 *__restrict__ act, float *__restrict__ tgt, int64_t vol)
+                              col_scale sc, bf16 *__restrict__ act, float *__restrict__ tgt, int64_t vol)
 {
+    const cplx<double> cinv = {sc.cinv.re * inv, sc.cinv.im * inv};
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < vol; s += (int64_t)gridDim.x * blockDim.x) {
         const cplx<double> *f[3] = {y, f1, f2};
+        const cplx<double> m[3] = {{inv, 0.0}, {sc.s1.re * inv, sc.s1.im * inv}, {sc.s2.re * inv, sc.s2.im * inv}};
         for (int q = 0; q < 3; ++q)
             for (int c = 0; c < 3; ++c) {
-                const cplx<double> a = f[q][(int64_t)c * vol + s];
-                act[s * CH + 6 * q + 2 * c] = f2bf((float)(a.re * inv));
-                act[s * CH + 6 * q + 2 * c + 1] = f2bf((float)(a.im * inv));
+                const cplx<double> a = cmuld(f[q][(int64_t)c * vol + s], m[q]);
+                act[s * CH + 6 * q + 2 * c] = f2bf((float)a.re);
+                act[s * CH + 6 * q + 2 * c + 1] = f2bf((float)a.im);
             }
         for (int c = CIN; c < CH; ++c) act[s * CH + c] = 0;
         for (int c = 0; c < 3; ++c) {
             const int64_t at = (int64_t)c * vol + s;
-            tgt[s * CIN + 2 * c] = (float)((x[at].re - (y[at].re - f1[at].re + f2[at].re)) * inv);
-            tgt[s * CIN + 2 * c + 1] = (float)((x[at].im - (y[at].im - f1[at].im + f2[at].im)) * inv);
+            const cplx<double> r = {x[at].re - (y[at].re - f1[at].re + f2[at].re),
+                                    x[at].im - (y[at].im - f1[at].im + f2[at].im)};
+            const cplx<double> t = cmuld(r, cinv);
+            tgt[s * CIN + 2 * c] = (float)t.re;
+            tgt[s * CIN + 2 * c + 1] = (float)t.im;
         }
         for (int c = 6; c < CIN; ++c) tgt[s * CIN + c] = 0.f;
     }
@@ -448,26 +477,35 @@ static int upload_weights(nrq_cnn *n)
     return NRQ_OK;
 }
 
+static double spectrum_top(const geom &g)
+{
+    return 2.0 / (g.a[1] * g.a[1]) + 2.0 / (g.a[2] * g.a[2]) + 2.0 / (g.a[3] * g.a[3]);
+}
+
 template <typename T>
 int cnn_precondition(nrq_cnn *net, const cplx<T> *const *v, const cplx<T> *const *f1, const cplx<T> *const *f2,
-                     cplx<T> *const *z, int R, const int *act)
+                     cplx<T> *const *z, int R, const int *act, const cplx<double> *beta)
 {
     const geom g = lat().g;
     if (int e = ensure_geom(net, g)) return e;
+    const double kmax = spectrum_top(g);
     for (int r = 0; r < R; ++r) {
         if (act && !act[r]) continue;
-        hipLaunchKernelGGL(k_pack<T>, dim3(1024), dim3(256), 0, stream(), v[r], f1[r], f2[r], net->act[0], g.vol);
+        const col_scale sc = make_col_scale(beta[r], kmax);
+        hipLaunchKernelGGL(k_pack<T>, dim3(1024), dim3(256), 0, stream(), v[r], f1[r], f2[r], net->act[0], g.vol, sc);
         if (int e = forward(net)) return e;
         hipLaunchKernelGGL(k_unpack<T>, dim3(1024), dim3(256), 0, stream(), (const float *)net->out, v[r], f1[r], f2[r],
-                           z[r], g.vol);
+                           z[r], g.vol, sc);
         NRQ_HIP(hipGetLastError());
     }
     return NRQ_OK;
 }
 template int cnn_precondition<double>(nrq_cnn *, const cplx<double> *const *, const cplx<double> *const *,
-                                      const cplx<double> *const *, cplx<double> *const *, int, const int *);
+                                      const cplx<double> *const *, cplx<double> *const *, int, const int *,
+                                      const cplx<double> *);
 template int cnn_precondition<float>(nrq_cnn *, const cplx<float> *const *, const cplx<float> *const *,
-                                     const cplx<float> *const *, cplx<float> *const *, int, const int *);
+                                     const cplx<float> *const *, cplx<float> *const *, int, const int *,
+                                     const cplx<double> *);
 
 } // namespace nrq
 
@@ -672,8 +710,8 @@ extern "C" int nrq_cnn_harvest(nrq_cnn *n, const nrq_gauge *gf, double mass, dou
             }
             const int64_t off = (int64_t)q * fs;
             hipLaunchKernelGGL(k_make_sample, dim3(1024), dim3(256), 0, stream(), y + off,
-                               (const cplx<double> *)psi + off, f1 + off, f2 + off, 1.0 / std::sqrt(yn2[q]), smp.act,
-                               smp.tgt, g.vol);
+                               (const cplx<double> *)psi + off, f1 + off, f2 + off, 1.0 / std::sqrt(yn2[q]),
+                               make_col_scale(beta[q], spectrum_top(g)), smp.act, smp.tgt, g.vol);
             n->data.push_back(smp);
         }
     }

@@ -872,6 +872,9 @@ int krylov_rhs_features(const nrq_gauge *gf, int R, const cplx<double> **f1, con
 
 // column solves handed from FGMRES to plain GMRES since the library loaded
 static int64_t g_fallbacks = 0;
+// CN coefficients beta of the slots of the current step, on the host (the
+// learned preconditioner scales its inputs and output per slot)
+static cplx<double> g_beta_host[NSLOTS];
 
 // ---------------------------------------------------------------------------
 // GMRES(m) core: solves (1 + beta_r K) x_r = b_r for the slots with act[r],
@@ -960,7 +963,7 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
                     f2p[r] = res + (size_t)r * n;
                     zp[r] = zj + (size_t)r * n;
                 }
-                if (int e = cnn_precondition<T>(net, vp, f1p, f2p, zp, R, cycle.data())) return e;
+                if (int e = cnn_precondition<T>(net, vp, f1p, f2p, zp, R, cycle.data(), g_beta_host)) return e;
                 if (int e = apply_t<RL, T>(links, zj, w, R)) return e;
             } else if (int e = apply_t<RL, T>(links, vj, w, R)) {
                 return e;
@@ -1495,6 +1498,7 @@ int krylov_cn_step(const nrq_gauge *gf, cplx<double> *const *fields, int R, cons
         a[r] = zc(alpha[r].re, alpha[r].im);
         b[r] = zc(beta[r].re, beta[r].im);
         par[r] = alpha[r];
+        g_beta_host[r] = beta[r];
         const zc g = a[r] - b[r];
         par[NSLOTS + r] = {g.real(), g.imag()};
     }
