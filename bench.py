@@ -65,7 +65,7 @@ CN_MIXED_CHEB_BYTES = 2448.0 + 648 + 792 + 792 + 1080 + 2232 + 1656 + 2 * 1872
 CN_MIXED_CHEB_ITS = 6.0
 TRAFFIC_FILE = ROOT / "profiles" / "leapfrog_fused_128cubed_traffic.txt"
 TILINGS = ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles", "LDS tiles, 16-plane runs",
-           "LDS tiles, 8-plane runs", "LDS tiles (two planes), 16-plane runs", "LDS tiles (two planes), 8-plane runs")
+           "LDS tiles, 8-plane runs")
 
 
 def _shape_key(precision: int, shape: dict) -> str:

@@ -221,9 +221,9 @@ int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval);
 /* Launch shape the leapfrog autotuner chose for this configuration's
  * precision and lattice size: threads per block, XCD remap factor (1 = none),
  * tiling (0: consecutive sites, 1: z-sweep tiles of full x-rows, 2: z-sweep
- * tiles 16 sites wide, 3 to 6: LDS-staged 16 x 8 tiles over runs of 16 or 8
- * z planes, with three (3, 4) or two (5, 6) planes of links in LDS, FP32
- * storage only, extents multiples of 16, 8 and the run), streaming
+ * tiles 16 sites wide, 3 and 4: LDS-staged 16 x 8 tiles over runs of 16 or 8
+ * z planes, three planes of links in LDS, FP32 storage only, extents
+ * multiples of 16, 8 and the run), streaming
  * (1: nontemporal loads/stores of E and the
  * new links) and two_row (1: neighbour links read as two rows, the third
  * rebuilt as the conjugate cross product, exact for SU(3)).  The first

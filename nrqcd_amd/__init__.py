@@ -524,7 +524,7 @@ class GaugeConfiguration:
 # nrqcd
 # ---------------------------------------------------------------------------
 _TILINGS = ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles", "LDS tiles, 16-plane runs",
-            "LDS tiles, 8-plane runs", "LDS tiles (two planes), 16-plane runs", "LDS tiles (two planes), 8-plane runs")
+            "LDS tiles, 8-plane runs")
 
 
 def leapfrog_shapes() -> list:

@@ -232,27 +232,23 @@ __global__ __launch_bounds__(BS) void k_fused(lf_batch<R> bt, geom g, R dt, lf_m
 // The fused pass with the neighbour links staged in LDS.  A block owns a
 // TX x TY tile of (x, y) and ZC consecutive z planes of one configuration
 // (blockIdx.y); thread (site of the tile, direction i) works on (x, y, z, i)
-// for z in its run.  LDS holds a ring of z planes of links, each the tile
-// with a one-site halo in x and y, all three directions and nine elements
-// ([slot][dir * 9 + e][halo site]).  NP = 3: planes z - 1, z, z + 1 serve
-// every staple of plane z (the 13 links of (x, i) lie in them).  NP = 2:
-// planes z and z + 1 only, at half the LDS (two blocks per CU); the three
-// links that directions x and y need from plane z - 1 (U_z(x - z),
-// U_i(x - z), U_z(x + i - z)) are read from memory, where the block read
-// that plane one step before.  The next plane streams into registers while
-// plane z is computed and replaces the oldest plane afterwards, so each link
-// is read from HBM once per run of ZC planes (plus the x/y halo, from L2, and
-// the planes at the ends of the run).  Per (site, dir) the arithmetic is that
+// for z in its run.  LDS holds a ring of three z planes of links, each the
+// tile with a one-site halo in x and y, all three directions and nine
+// elements ([slot][dir * 9 + e][halo site]); planes z - 1, z, z + 1 serve
+// every staple of plane z (the 13 links of (x, i) lie in them).  The next
+// plane streams into registers while plane z is computed and replaces plane
+// z - 1 afterwards, so each link is read from HBM once per run of ZC planes
+// (plus the x/y halo, from L2, and two planes per run).  Per (site, dir) the arithmetic is that
 // of k_fused with full neighbour links.  The lattice extents must be
 // multiples of TX, TY and ZC.
-template <typename R, int TX, int TY, int ZC, int NP, bool NT>
-__global__ __launch_bounds__(TX *TY * 3, NP == 2 ? 3 : 1) void k_fused_tile(lf_batch<R> bt, geom g, R dt, int xcd)
+template <typename R, int TX, int TY, int ZC, bool NT>
+__global__ __launch_bounds__(TX *TY * 3) void k_fused_tile(lf_batch<R> bt, geom g, R dt, int xcd)
 {
     constexpr int HX = TX + 2, HY = TY + 2, HS = HX * HY;
     constexpr int NTH = TX * TY * 3;
     constexpr int PL = 27 * HS;                 // complex numbers per plane
     constexpr int PER = (PL + NTH - 1) / NTH;   // per thread and plane
-    __shared__ cplx<R> lds[NP * PL];
+    __shared__ cplx<R> lds[3 * PL];
     const cplx<R> *__restrict__ uin = bt.uin[blockIdx.y];
     cplx<R> *__restrict__ uout = bt.uout[blockIdx.y];
     R *__restrict__ ef = bt.ef[blockIdx.y];
@@ -290,8 +286,8 @@ __global__ __launch_bounds__(TX *TY * 3, NP == 2 ? 3 : 1) void k_fused_tile(lf_b
             if (q < PL) lds[slot * PL + q] = pre[k];
         }
     };
-    for (int p = 0; p < NP; ++p) {
-        gload(z0 - (NP == 3 ? 1 : 0) + p);
+    for (int p = 0; p < 3; ++p) {
+        gload(z0 - 1 + p);
         sstore(p);
     }
     __syncthreads();
@@ -301,14 +297,8 @@ __global__ __launch_bounds__(TX *TY * 3, NP == 2 ? 3 : 1) void k_fused_tile(lf_b
     const int tx = r % TX, ty = r / TX;
     // link of direction d at offset (dx, dy) in plane z + pz (pz = -1, 0, 1)
     int slot_of[3];
-    auto lnk = [&](int pz, int d, int dx, int dy, int z) {
+    auto lnk = [&](int pz, int d, int dx, int dy) {
         mat3<R> m;
-        if (NP == 2 && pz < 0) {
-            int gx = x0 + tx + dx, gy = y0 + ty + dy;
-            gx = gx < 0 ? gx + g.n0 : (gx >= g.n0 ? gx - g.n0 : gx);
-            gy = gy < 0 ? gy + g.n1 : (gy >= g.n1 ? gy - g.n1 : gy);
-            return load_link(uin, vol, d, gx + (int64_t)g.n0 * gy + plane * wrapz(z - 1));
-        }
         const cplx<R> *b = lds + slot_of[pz + 1] * PL + d * 9 * HS + (tx + 1 + dx) + HX * (ty + 1 + dy);
 #pragma unroll
         for (int e = 0; e < 9; ++e) m.m[e] = b[e * HS];
@@ -318,15 +308,9 @@ __global__ __launch_bounds__(TX *TY * 3, NP == 2 ? 3 : 1) void k_fused_tile(lf_b
         const int z = z0 + kz;
         const bool more = kz + 1 < ZC;
         if (more) gload(z + 2);
-        if (NP == 3) {
-            slot_of[0] = kz % 3;
-            slot_of[1] = (kz + 1) % 3;
-            slot_of[2] = (kz + 2) % 3;
-        } else {
-            slot_of[0] = 0; // unused: plane z - 1 comes from memory
-            slot_of[1] = kz & 1;
-            slot_of[2] = (kz + 1) & 1;
-        }
+        slot_of[0] = kz % 3;
+        slot_of[1] = (kz + 1) % 3;
+        slot_of[2] = (kz + 2) % 3;
         // staples, k in increasing order as in the reference's sum
         mat3<R> sum;
         set_zero(sum);
@@ -338,10 +322,10 @@ __global__ __launch_bounds__(TX *TY * 3, NP == 2 ? 3 : 1) void k_fused_tile(lf_b
             const int pk = k == 2 ? 1 : 0;                   // plane of x + k
             const int pmk = k == 2 ? -1 : 0;                 // plane of x - k
             const int pik = (i == 2 ? 1 : 0) + (k == 2 ? -1 : 0); // plane of x + i - k
-            mat3<R> su = mul_nd(lnk(pi, k, ix, iy, z), lnk(pk, i, kx, ky, z));
-            su = mul_nd(su, lnk(0, k, 0, 0, z));
-            mat3<R> sd = mul_dd(lnk(pik, k, ix - kx, iy - ky, z), lnk(pmk, i, -kx, -ky, z));
-            sd = mul(sd, lnk(pmk, k, -kx, -ky, z));
+            mat3<R> su = mul_nd(lnk(pi, k, ix, iy), lnk(pk, i, kx, ky));
+            su = mul_nd(su, lnk(0, k, 0, 0));
+            mat3<R> sd = mul_dd(lnk(pik, k, ix - kx, iy - ky), lnk(pmk, i, -kx, -ky));
+            sd = mul(sd, lnk(pmk, k, -kx, -ky));
 #pragma unroll
             for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * (su.m[m] + sd.m[m]);
         }
@@ -349,13 +333,13 @@ __global__ __launch_bounds__(TX *TY * 3, NP == 2 ? 3 : 1) void k_fused_tile(lf_b
         R e[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * vol + s);
-        const mat3<R> ui = lnk(0, i, 0, 0, z);
+        const mat3<R> ui = lnk(0, i, 0, 0);
         apply_force(ui, sum, e);
 #pragma unroll
         for (int a = 0; a < 8; ++a) st_e<NT>(ef + ((int64_t)i * 8 + a) * vol + s, e[a]);
         store_link_out<NT>(uout, vol, i, s, evolve_link(ui, e, dt));
         __syncthreads(); // the oldest plane is no longer read
-        if (more) sstore(NP == 3 ? kz % 3 : (kz & 1));
+        if (more) sstore(kz % 3);
         __syncthreads();
     }
 }
@@ -679,14 +663,14 @@ struct lf_variant {
     int rec; // neighbour links read as two rows (third rebuilt)
 };
 // Modes 3 and 4 are the LDS-tiled fused pass (k_fused_tile, FP32, 16 x 8
-// tiles, three planes in LDS, runs of 16 or 8 z planes), modes 5 and 6 the
-// same with two planes in LDS; bs is the tile's thread count.  Their first and
-// last half steps use the consecutive-site map.
+// tiles, three planes in LDS, runs of 16 or 8 z planes); bs is the tile's
+// thread count.  Their first and last half steps use the consecutive-site
+// map.
 static const lf_variant LF_VARIANTS[] = {
     {256, 8, 1, 1, 1}, {128, 8, 1, 1, 1}, {64, 8, 1, 1, 1}, {256, 8, 0, 1, 1}, {128, 8, 0, 1, 1}, {64, 8, 0, 1, 1},
     {256, 8, 2, 1, 1}, {256, 8, 2, 0, 1}, {256, 8, 0, 0, 1}, {64, 8, 0, 0, 1},  {256, 8, 1, 1, 0}, {256, 8, 0, 0, 0},
     {256, 8, 2, 0, 0}, {64, 8, 0, 0, 0},  {256, 1, 0, 0, 1}, {384, 8, 3, 0, 0}, {384, 8, 3, 1, 0}, {384, 8, 4, 0, 0},
-    {384, 8, 4, 1, 0}, {384, 8, 5, 0, 0}, {384, 8, 5, 1, 0}, {384, 8, 6, 0, 0}, {384, 8, 6, 1, 0}};
+    {384, 8, 4, 1, 0}};
 constexpr int LF_NVARIANTS = sizeof(LF_VARIANTS) / sizeof(LF_VARIANTS[0]);
 constexpr int LF_DEFAULT = 11; // {256, 8, 0, 0, 0}: used without tuning, valid on every lattice
 
@@ -707,7 +691,7 @@ static bool variant_ok(int variant, const geom &g, int prec)
     if (v.mode == 1) return g.n0 <= v.bs;
     if (v.mode == 2) return g.n0 >= 16;
     if (v.mode >= 3)
-        return prec == 32 && g.n0 % 16 == 0 && g.n1 % 8 == 0 && g.n2 % (v.mode == 3 || v.mode == 5 ? 16 : 8) == 0;
+        return prec == 32 && g.n0 % 16 == 0 && g.n1 % 8 == 0 && g.n2 % (v.mode == 3 ? 16 : 8) == 0;
     return true;
 }
 
@@ -737,13 +721,13 @@ static lf_map make_map(int variant, const geom &g)
     return make_lf_map(g, v.bs, v.xcd, v.mode, 3);
 }
 
-template <typename R, int ZC, int NP, bool NTV>
+template <typename R, int ZC, bool NTV>
 static void launch_tile(const lf_batch<R> &bt, int nb, const geom &g, R dt, int xcd)
 {
     constexpr int TX = 16, TY = 8;
     const unsigned total = (unsigned)(g.n0 / TX) * (unsigned)(g.n1 / TY) * (unsigned)(g.n2 / ZC);
-    hipLaunchKernelGGL((k_fused_tile<R, TX, TY, ZC, NP, NTV>), dim3(total, (unsigned)nb), dim3(TX * TY * 3), 0, stream(),
-                       bt, g, dt, xcd);
+    hipLaunchKernelGGL((k_fused_tile<R, TX, TY, ZC, NTV>), dim3(total, (unsigned)nb), dim3(TX * TY * 3), 0, stream(), bt,
+                       g, dt, xcd);
 }
 
 template <typename R> static void launch_links(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
@@ -772,12 +756,10 @@ template <typename R> static void launch_fused(int variant, const lf_batch<R> &b
 {
     const lf_variant tv = LF_VARIANTS[variant];
     if constexpr (sizeof(R) == 4) if (tv.mode >= 3) {
-        switch (tv.mode) {
-        case 3: tv.nt ? launch_tile<R, 16, 3, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 16, 3, false>(bt, nb, g, dt, tv.xcd); break;
-        case 4: tv.nt ? launch_tile<R, 8, 3, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 8, 3, false>(bt, nb, g, dt, tv.xcd); break;
-        case 5: tv.nt ? launch_tile<R, 16, 2, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 16, 2, false>(bt, nb, g, dt, tv.xcd); break;
-        default: tv.nt ? launch_tile<R, 8, 2, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 8, 2, false>(bt, nb, g, dt, tv.xcd); break;
-        }
+        if (tv.mode == 3)
+            tv.nt ? launch_tile<R, 16, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 16, false>(bt, nb, g, dt, tv.xcd);
+        else
+            tv.nt ? launch_tile<R, 8, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 8, false>(bt, nb, g, dt, tv.xcd);
         return;
     }
     const lf_map m = make_map(variant, g);
