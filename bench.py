@@ -380,7 +380,8 @@ def main() -> None:
             "traffic": _profiled_traffic(L, 32, shape),
             "traffic_source": f"rocprofv3 FETCH_SIZE + WRITE_SIZE per launch of this shape, "
                               f"{TRAFFIC_FILE.relative_to(ROOT)}",
-            "kernel": "k_fused<float> (E update + next link update per (site, dir)), FP32 storage",
+            "kernel": ("k_fused_tile<float>" if shape["tiling"].startswith("LDS") else "k_fused<float>")
+                      + " (E update + next link update per (site, dir)), FP32 storage",
             "launch_shape": shape,
             "bytes_per_launch": FUSED_BYTES[32] * V,
             "kernel_us": lf["fused_kernel_us"],
