@@ -733,7 +733,7 @@ static void launch_tile(const lf_batch<R> &bt, int nb, const geom &g, R dt, int 
 template <typename R> static void launch_links(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
 {
     const lf_map m = make_map(variant, g);
-    const dim3 grd((unsigned)(m.ngroup * m.nsb), (unsigned)nb);
+    const dim3 grd(lf_blocks(m), (unsigned)nb);
     switch (m.bs) {
     case 64: LF_NT(hipLaunchKernelGGL((k_links<R, 64, NTV>), grd, dim3(64), 0, stream(), bt, g, dt, m)); break;
     case 128: LF_NT(hipLaunchKernelGGL((k_links<R, 128, NTV>), grd, dim3(128), 0, stream(), bt, g, dt, m)); break;
@@ -744,7 +744,7 @@ template <typename R> static void launch_links(int variant, const lf_batch<R> &b
 template <typename R> static void launch_efield(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
 {
     const lf_map m = make_map(variant, g);
-    const dim3 grd((unsigned)(m.ngroup * m.nsb), (unsigned)nb);
+    const dim3 grd(lf_blocks(m), (unsigned)nb);
     switch (m.bs) {
     case 64: LF_NT(hipLaunchKernelGGL((k_efield<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), bt, g, dt, m)); break;
     case 128: LF_NT(hipLaunchKernelGGL((k_efield<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), bt, g, dt, m)); break;
@@ -763,7 +763,7 @@ template <typename R> static void launch_fused(int variant, const lf_batch<R> &b
         return;
     }
     const lf_map m = make_map(variant, g);
-    const dim3 grd((unsigned)(m.ngroup * m.nsb), (unsigned)nb);
+    const dim3 grd(lf_blocks(m), (unsigned)nb);
     switch (m.bs) {
     case 64: LF_NT(hipLaunchKernelGGL((k_fused<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), bt, g, dt, m)); break;
     case 128: LF_NT(hipLaunchKernelGGL((k_fused<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), bt, g, dt, m)); break;

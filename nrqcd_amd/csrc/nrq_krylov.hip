@@ -42,7 +42,11 @@ const void *gauge_links(const nrq_gauge *h);
 constexpr int CHEB_MAX = 64;  // Chebyshev steps per coefficient table
 constexpr int VBLOCKS = 256;  // blocks per slot in the vector kernels
 constexpr int VTHREADS = 256;
-constexpr int SBS = 256;      // threads per block in the stencil kernels
+constexpr int SBS = 256;      // threads per block in the stencil kernels (site-block maps)
+constexpr int SBS_WAVE = 512; // at most, with one work item per wavefront (wave-item map)
+#ifndef CN_WAVES
+#define CN_WAVES 4 // wavefronts per SIMD the stencil kernels' registers must allow
+#endif
 constexpr double FP32_CORR_ERR = 1e-6; // relative error assumed for an FP32 correction
 
 // ---------------------------------------------------------------------------
@@ -86,82 +90,109 @@ template <typename T> struct slots {
     cplx<T> *p[NSLOTS];
 };
 
-// Per-slot fields of one chunk, read in the compute type T.
-template <int NC, typename T, typename S> struct ld_fields {
+// Per-slot fields of one chunk, read in the compute type T (NT: streaming
+// loads, for values no other thread reads).  The slot pointers are the same
+// for a whole block, so every load is a scalar base plus a 32-bit offset.
+template <int NC, typename T, typename S, bool NT = false> struct ld_fields {
     const cplx<S> *f[NC];
-    __device__ __forceinline__ cplx<T> operator()(int q, int64_t at) const
+    __device__ __forceinline__ cplx<T> operator()(int q, int c, int64_t vol, uint32_t site) const
     {
-        const cplx<S> v = f[q][at];
+        const cplx<S> *p = at32(f[q] + (int64_t)c * vol, site);
+        const cplx<S> v = NT ? load_nt(p) : *p;
         return {(T)v.re, (T)v.im};
     }
 };
 
 // hop_q = sum_i (U_i(x) f_q(x+i) + U_i(x-i)^+ f_q(x-i)) / a_i^2 at x = s for
-// the slots q < NC of one chunk with on[q]; with dsum = sum_i 1/a_i^2 the
-// kinetic operator of nrqcd.f90:1210-1242 (without its coefficient) is
+// the slots q < NC of one chunk; with dsum = sum_i 1/a_i^2 the kinetic
+// operator of nrqcd.f90:1210-1242 (without its coefficient) is
 //   K f(s) = dsum f(s) - hop / 2
 //          = sum_i [f(s) - (U_i(s) f(s+i) + U_i(s-i)^+ f(s-i))/2] / a_i^2.
-// ld(q, at) returns component at = colour * V + site of slot q's field.
-// The links are loaded once for the whole chunk.
+// ld(q, c, V, site) returns colour c of slot q's field at site; it
+// must be readable for every q (a chunk's unused slots read a used slot's
+// field, and the caller drops their results).  The kernels are bound by
+// memory latency, so each of the six hops issues its 9 link loads and all
+// 3 NC field loads at once, with no branch between them: one round trip per
+// hop.
 template <int NC, typename T, typename RL, typename LD>
-__device__ __forceinline__ void hop_chunk(const cplx<RL> *__restrict__ u, const LD &ld, const bool (&on)[NC], const lsite &p,
-                                          const geom &g, cplx<T> (&hop)[NC][3])
+__device__ __forceinline__ void hop_chunk(const cplx<RL> *__restrict__ u, const LD &ld, const lsite &p, const geom &g,
+                                          cplx<T> (&hop)[NC][3])
 {
 #pragma unroll
     for (int q = 0; q < NC; ++q)
 #pragma unroll
         for (int c = 0; c < 3; ++c) hop[q][c] = {T(0), T(0)};
-    for (int i = 0; i < 3; ++i) {
+    // a loop, not unrolled: unrolled, the compiler hoists the loads of all six
+    // hops to the top and spills them
+#pragma unroll 1
+    for (int h = 0; h < 6; ++h) {
+        const int i = h >> 1;
+        const bool fwd = (h & 1) == 0;
         const T ia2 = T(1.0 / (g.a[i + 1] * g.a[i + 1]));
-        const int64_t s = p.s, sp = site_up(g, p, i).s, sm = site_dn(g, p, i).s;
-        cplx<T> up[9];
+        // forward: U_i(s) and f(s+i); backward: U_i(s-i) and f(s-i)
+        const int64_t sl = fwd ? p.s : site_dn(g, p, i).s;
+        const int64_t sf = fwd ? site_up(g, p, i).s : sl;
+        cplx<RL> lk[9];
+        cplx<T> f[NC][3];
 #pragma unroll
-        for (int e = 0; e < 9; ++e) {
-            const cplx<RL> a = u[((int64_t)i * 9 + e) * g.vol + s];
-            up[e] = {(T)(ia2 * a.re), (T)(ia2 * a.im)};
-        }
+        for (int e = 0; e < 9; ++e) lk[e] = *at32(u + ((int64_t)i * 9 + e) * g.vol, (uint32_t)sl);
 #pragma unroll
-        for (int q = 0; q < NC; ++q) {
-            if (!on[q]) continue;
-            cplx<T> vp[3];
+        for (int q = 0; q < NC; ++q)
 #pragma unroll
-            for (int c = 0; c < 3; ++c) vp[c] = ld(q, (int64_t)c * g.vol + sp);
+            for (int c = 0; c < 3; ++c) f[q][c] = ld(q, c, g.vol, (uint32_t)sf);
+        // hop += (U f) / a_i^2, the product in the compute type
+        cplx<T> w[9];
 #pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                mac(hop[q][c], up[3 * c], vp[0]);
-                mac(hop[q][c], up[3 * c + 1], vp[1]);
-                mac(hop[q][c], up[3 * c + 2], vp[2]);
-            }
-        }
+        for (int e = 0; e < 9; ++e) w[e] = {(T)lk[e].re, (T)lk[e].im};
 #pragma unroll
-        for (int e = 0; e < 9; ++e) {
-            const cplx<RL> a = u[((int64_t)i * 9 + e) * g.vol + sm];
-            up[e] = {(T)(ia2 * a.re), (T)(ia2 * a.im)};
-        }
-#pragma unroll
-        for (int q = 0; q < NC; ++q) {
-            if (!on[q]) continue;
-            cplx<T> vm[3];
-#pragma unroll
-            for (int c = 0; c < 3; ++c) vm[c] = ld(q, (int64_t)c * g.vol + sm);
+        for (int q = 0; q < NC; ++q)
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                mac_cn(hop[q][c], up[c], vm[0]);
-                mac_cn(hop[q][c], up[3 + c], vm[1]);
-                mac_cn(hop[q][c], up[6 + c], vm[2]);
+                cplx<T> t = {T(0), T(0)};
+                if (fwd) {
+                    mac(t, w[3 * c], f[q][0]);
+                    mac(t, w[3 * c + 1], f[q][1]);
+                    mac(t, w[3 * c + 2], f[q][2]);
+                } else {
+                    mac_cn(t, w[c], f[q][0]);
+                    mac_cn(t, w[3 + c], f[q][1]);
+                    mac_cn(t, w[6 + c], f[q][2]);
+                }
+                hop[q][c].re = fmx(ia2, t.re, hop[q][c].re);
+                hop[q][c].im = fmx(ia2, t.im, hop[q][c].im);
             }
-        }
     }
 }
 
-// Per-slot block sums of v[q]; every thread of the block must call this.
-// Thread q < NC writes partial[(r0 + q) * nsb + t] for slots r0 + q < nf.
+// The chunk's own values of slot fields: v[q][c] = ld(q, c, V, s), all
+// issued before any store of the caller.
+template <int NC, typename T, typename LD>
+__device__ __forceinline__ void own_chunk(const LD &ld, int64_t s, int64_t vol, cplx<T> (&v)[NC][3])
+{
+#pragma unroll
+    for (int q = 0; q < NC; ++q)
+#pragma unroll
+        for (int c = 0; c < 3; ++c) v[q][c] = ld(q, c, vol, (uint32_t)s);
+}
+
+// Per-slot sums of v[q] over the block (wave = false; every thread of the
+// block must call this) or over the wavefront (wave = true: the wave-item
+// map, one chunk per wavefront).  The sum of slot r0 + q < nf goes to
+// partial[(r0 + q) * nsb + t].
 template <int NC>
 __device__ __forceinline__ void chunk_partials(double (&v)[NC], int r0, int nf, int64_t t, int64_t nsb,
-                                               double *__restrict__ partial)
+                                               double *__restrict__ partial, bool wave)
 {
-    __shared__ double sh[NC][SBS / 64];
     const int lane = threadIdx.x & 63, w = threadIdx.x >> 6;
+    if (wave) {
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            const double x = wsum(v[q]);
+            if (lane == 0 && r0 + q < nf) partial[(int64_t)(r0 + q) * nsb + t] = x;
+        }
+        return;
+    }
+    __shared__ double sh[NC][SBS / 64];
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         const double x = wsum(v[q]);
@@ -180,10 +211,28 @@ __device__ __forceinline__ void chunk_partials(double (&v)[NC], int r0, int nf, 
 // stencil kernels (grid: lf_map with one work item per chunk of NC slots)
 // ---------------------------------------------------------------------------
 
+// The slots of chunk r0 that a stencil kernel computes: on[q] for slot
+// r0 + q, and every load pointer of an unused slot aimed at the first used
+// one (same addresses, so its loads hit the cache).  False when no slot of
+// the chunk is used.
+template <int NC>
+__device__ __forceinline__ bool chunk_slots(int r0, int nf, const int *__restrict__ active, bool (&on)[NC], int (&rs)[NC])
+{
+    int first = -1;
+#pragma unroll
+    for (int q = 0; q < NC; ++q) {
+        on[q] = r0 + q < nf && (!active || active[r0 + q]);
+        if (on[q] && first < 0) first = r0 + q;
+    }
+#pragma unroll
+    for (int q = 0; q < NC; ++q) rs[q] = on[q] ? r0 + q : first;
+    return first >= 0;
+}
+
 // out_r = in_r + beta_r K in_r for the active slots; with gco (not null)
 // instead out_r = -beta_r gco_r K in_r.
 template <int NC, typename RL, typename T>
-__global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u, slots<T> in, slots<T> out,
+__global__ __launch_bounds__(SBS_WAVE, CN_WAVES) void k_stencil(const cplx<RL> *__restrict__ u, slots<T> in, slots<T> out,
                                                  const cplx<double> *__restrict__ beta, const int *__restrict__ active,
                                                  int nf, geom g, lf_map mp, const cplx<double> *__restrict__ gco)
 {
@@ -193,34 +242,31 @@ __global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u,
     const int64_t s = p.s;
     const int r0 = chunk * NC;
     bool on[NC];
+    int rs[NC];
+    if (!chunk_slots<NC>(r0, nf, active, on, rs)) return;
     ld_fields<NC, T, T> ld;
 #pragma unroll
-    for (int q = 0; q < NC; ++q) {
-        on[q] = (r0 + q < nf) && active[r0 + q];
-        ld.f[q] = in.p[on[q] ? r0 + q : 0];
-    }
+    for (int q = 0; q < NC; ++q) ld.f[q] = in.p[rs[q]];
     cplx<T> hop[NC][3];
-    hop_chunk<NC, T>(u, ld, on, p, g, hop);
+    hop_chunk<NC, T>(u, ld, p, g, hop);
+    cplx<T> v0[NC][3];
+    own_chunk<NC, T>(ld, s, g.vol, v0);
     const T dsum = T(g_dsum(g));
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
         if (!on[q]) continue;
-        cplx<T> *o = out.p[r0 + q] + s;
+        cplx<T> *o = out.p[r0 + q];
+        cplx<T> cf;
         if (gco) {
             const cplx<double> b = beta[r0 + q], gc = gco[r0 + q];
-            const cplx<T> cf = to<T>(cplx<double>{-(b.re * gc.re - b.im * gc.im), -(b.re * gc.im + b.im * gc.re)});
-#pragma unroll
-            for (int c = 0; c < 3; ++c) {
-                const cplx<T> v0 = ld(q, (int64_t)c * g.vol + s);
-                store_nt(o + (int64_t)c * g.vol, cf * (dsum * v0 - T(0.5) * hop[q][c]));
-            }
-            continue;
+            cf = to<T>(cplx<double>{-(b.re * gc.re - b.im * gc.im), -(b.re * gc.im + b.im * gc.re)});
+        } else {
+            cf = to<T>(beta[r0 + q]);
         }
-        const cplx<T> bt = to<T>(beta[r0 + q]);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const cplx<T> v0 = ld(q, (int64_t)c * g.vol + s);
-            store_nt(o + (int64_t)c * g.vol, v0 + bt * (dsum * v0 - T(0.5) * hop[q][c]));
+            const cplx<T> kf = dsum * v0[q][c] - T(0.5) * hop[q][c];
+            store_nt(at32(o + (int64_t)c * g.vol, (uint32_t)s), gco ? cf * kf : v0[q][c] + cf * kf);
         }
     }
 }
@@ -233,7 +279,7 @@ __global__ __launch_bounds__(SBS) void k_stencil(const cplx<RL> *__restrict__ u,
 // K psi_r also goes to kout in FP32: the guess's residual is exactly
 // y - (1 + beta K) x = -beta g K^2 psi.
 template <int NC, typename RL>
-__global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, slots<double> psi, slots<double> y,
+__global__ __launch_bounds__(SBS_WAVE, CN_WAVES) void k_cn_rhs(const cplx<RL> *__restrict__ u, slots<double> psi, slots<double> y,
                                                 slots<double> x, const cplx<double> *__restrict__ alpha,
                                                 const cplx<double> *__restrict__ gco, int guess, int nf, geom g, lf_map mp,
                                                 double *__restrict__ partial, slots<float> kout, int wk)
@@ -244,17 +290,17 @@ __global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, 
     const bool valid = lf_site(mp, g, p, chunk, t);
     const int64_t s = p.s;
     const int r0 = chunk * NC;
-    bool on[NC];
-    ld_fields<NC, double, double> ld;
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-        on[q] = valid && (r0 + q < nf);
-        ld.f[q] = psi.p[on[q] ? r0 + q : 0];
-    }
     double nrm[NC] = {};
-    if (valid) {
+    bool on[NC];
+    int rs[NC];
+    if (valid && chunk_slots<NC>(r0, nf, nullptr, on, rs)) {
+        ld_fields<NC, double, double> ld;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) ld.f[q] = psi.p[rs[q]];
         cplx<double> hop[NC][3];
-        hop_chunk<NC, double>(u, ld, on, p, g, hop);
+        hop_chunk<NC, double>(u, ld, p, g, hop);
+        cplx<double> v0[NC][3];
+        own_chunk<NC, double>(ld, s, g.vol, v0);
         const double dsum = g_dsum(g);
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
@@ -263,24 +309,23 @@ __global__ __launch_bounds__(SBS) void k_cn_rhs(const cplx<RL> *__restrict__ u, 
             const cplx<double> a = alpha[r], gc = gco[r];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const int64_t at = (int64_t)c * g.vol + s;
-                const cplx<double> v0 = ld(q, at);
-                const cplx<double> kf = dsum * v0 - 0.5 * hop[q][c];
-                const cplx<double> yv = v0 + a * kf;
-                store_nt(y.p[r] + at, yv);
-                store_nt(x.p[r] + at, guess ? v0 + gc * kf : cplx<double>{0.0, 0.0});
-                if (wk) store_nt(kout.p[r] + at, cplx<float>{(float)kf.re, (float)kf.im});
+                const int64_t cv = (int64_t)c * g.vol;
+                const cplx<double> kf = dsum * v0[q][c] - 0.5 * hop[q][c];
+                const cplx<double> yv = v0[q][c] + a * kf;
+                store_nt(at32(y.p[r] + cv, (uint32_t)s), yv);
+                store_nt(at32(x.p[r] + cv, (uint32_t)s), guess ? v0[q][c] + gc * kf : cplx<double>{0.0, 0.0});
+                if (wk) store_nt(at32(kout.p[r] + cv, (uint32_t)s), cplx<float>{(float)kf.re, (float)kf.im});
                 nrm[q] += nrm2(yv);
             }
         }
     }
-    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial);
+    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial, mp.mode == 3);
 }
 
 // FP64 true residual r_r = y_r - (1 + beta_r K) x_r, stored to res as type
 // TR when WRITE_RES; partial ||r_r||^2 per site block.
 template <int NC, typename RL, typename TR, bool WRITE_RES>
-__global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u, slots<double> x, slots<double> y,
+__global__ __launch_bounds__(SBS_WAVE, CN_WAVES) void k_cn_resid(const cplx<RL> *__restrict__ u, slots<double> x, slots<double> y,
                                                   slots<TR> res, const cplx<double> *__restrict__ beta, int nf, geom g,
                                                   lf_map mp, double *__restrict__ partial)
 {
@@ -290,17 +335,22 @@ __global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u
     const bool valid = lf_site(mp, g, p, chunk, t);
     const int64_t s = p.s;
     const int r0 = chunk * NC;
-    bool on[NC];
-    ld_fields<NC, double, double> ld;
-#pragma unroll
-    for (int q = 0; q < NC; ++q) {
-        on[q] = valid && (r0 + q < nf);
-        ld.f[q] = x.p[on[q] ? r0 + q : 0];
-    }
     double nrm[NC] = {};
-    if (valid) {
+    bool on[NC];
+    int rs[NC];
+    if (valid && chunk_slots<NC>(r0, nf, nullptr, on, rs)) {
+        ld_fields<NC, double, double> ld;
+        ld_fields<NC, double, double, true> ly;
+#pragma unroll
+        for (int q = 0; q < NC; ++q) {
+            ld.f[q] = x.p[rs[q]];
+            ly.f[q] = y.p[rs[q]];
+        }
         cplx<double> hop[NC][3];
-        hop_chunk<NC, double>(u, ld, on, p, g, hop);
+        hop_chunk<NC, double>(u, ld, p, g, hop);
+        cplx<double> v0[NC][3], yv[NC][3];
+        own_chunk<NC, double>(ld, s, g.vol, v0);
+        own_chunk<NC, double>(ly, s, g.vol, yv);
         const double dsum = g_dsum(g);
 #pragma unroll
         for (int q = 0; q < NC; ++q) {
@@ -309,15 +359,13 @@ __global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u
             const cplx<double> bt = beta[r];
 #pragma unroll
             for (int c = 0; c < 3; ++c) {
-                const int64_t at = (int64_t)c * g.vol + s;
-                const cplx<double> v0 = ld(q, at);
-                const cplx<double> rr = load_nt(y.p[r] + at) - (v0 + bt * (dsum * v0 - 0.5 * hop[q][c]));
-                if (WRITE_RES) store_nt(res.p[r] + at, cplx<TR>{(TR)rr.re, (TR)rr.im});
+                const cplx<double> rr = yv[q][c] - (v0[q][c] + bt * (dsum * v0[q][c] - 0.5 * hop[q][c]));
+                if (WRITE_RES) store_nt(at32(res.p[r] + (int64_t)c * g.vol, (uint32_t)s), cplx<TR>{(TR)rr.re, (TR)rr.im});
                 nrm[q] += nrm2(rr);
             }
         }
     }
-    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial);
+    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial, mp.mode == 3);
 }
 
 // One Chebyshev step on the segment [1, 1 + beta_r kmax] of the spectrum of
@@ -328,8 +376,10 @@ __global__ __launch_bounds__(SBS) void k_cn_resid(const cplx<RL> *__restrict__ u
 // x_k = b / c implicit (xk is b) and x_{k-1} = 0; 3: x_{k-1} = b / c implicit.
 // x_{k+1} goes to xnext, or with fin to fout = fin + x_{k+1} in FP64 (the
 // last step of a correction solve adds the correction to the FP64 iterate).
+// xprev and xnext may be the same fields: each thread reads its own values of
+// x_{k-1} before it writes them.
 template <int NC, typename RL, typename T>
-__global__ __launch_bounds__(SBS) void k_cheb(const cplx<RL> *__restrict__ u, slots<T> xk, slots<T> xprev, slots<T> xnext,
+__global__ __launch_bounds__(SBS_WAVE, CN_WAVES) void k_cheb(const cplx<RL> *__restrict__ u, slots<T> xk, slots<T> xprev, slots<T> xnext,
                                               slots<T> b, const cplx<double> *__restrict__ beta,
                                               const cplx<double> *__restrict__ coef, const int *__restrict__ active, int nf,
                                               geom g, lf_map mp, int mode, int fin, slots<double> fin_in,
@@ -341,14 +391,23 @@ __global__ __launch_bounds__(SBS) void k_cheb(const cplx<RL> *__restrict__ u, sl
     const int64_t s = p.s;
     const int r0 = chunk * NC;
     bool on[NC];
+    int rs[NC];
+    if (!chunk_slots<NC>(r0, nf, active, on, rs)) return;
     ld_fields<NC, T, T> ld;
+    ld_fields<NC, T, T, true> lb, lp;
+    ld_fields<NC, double, double, true> lf;
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
-        on[q] = (r0 + q < nf) && active[r0 + q];
-        ld.f[q] = xk.p[on[q] ? r0 + q : 0];
+        ld.f[q] = xk.p[rs[q]];
+        lb.f[q] = b.p[rs[q]];
+        lp.f[q] = xprev.p[rs[q]];
+        lf.f[q] = fin_in.p[rs[q]];
     }
     cplx<T> hop[NC][3];
-    hop_chunk<NC, T>(u, ld, on, p, g, hop);
+    hop_chunk<NC, T>(u, ld, p, g, hop);
+    cplx<T> v0[NC][3], bs[NC][3];
+    own_chunk<NC, T>(ld, s, g.vol, v0);
+    if (mode != 2) own_chunk<NC, T>(lb, s, g.vol, bs);
     const T dsum = T(g_dsum(g));
 #pragma unroll
     for (int q = 0; q < NC; ++q) {
@@ -358,23 +417,30 @@ __global__ __launch_bounds__(SBS) void k_cheb(const cplx<RL> *__restrict__ u, sl
         const cplx<T> ic = to<T>(coef[2 * r]); // 1 / c_r
         const cplx<T> rho = to<T>(coef[2 * r + 1]);
         const cplx<T> one_m_rho = {T(1) - rho.re, -rho.im};
+        // x_{k-1} and the FP64 iterate per slot (xprev may be xnext)
+        cplx<T> xp[3];
+        cplx<double> xo[3];
+        if (mode == 0)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xp[c] = lp(q, c, g.vol, (uint32_t)s);
+        if (fin)
+#pragma unroll
+            for (int c = 0; c < 3; ++c) xo[c] = lf(q, c, g.vol, (uint32_t)s);
 #pragma unroll
         for (int c = 0; c < 3; ++c) {
-            const int64_t at = (int64_t)c * g.vol + s;
-            const cplx<T> v0 = ld(q, at);
-            const cplx<T> kf = dsum * v0 - T(0.5) * hop[q][c];
-            const cplx<T> bs = mode == 2 ? v0 : load_nt(b.p[r] + at);
-            const cplx<T> x0 = mode == 2 ? ic * v0 : v0;
+            const int64_t cv = (int64_t)c * g.vol;
+            const cplx<T> kf = dsum * v0[q][c] - T(0.5) * hop[q][c];
+            const cplx<T> bq = mode == 2 ? v0[q][c] : bs[q][c];
+            const cplx<T> x0 = mode == 2 ? ic * v0[q][c] : v0[q][c];
             const cplx<T> ax = mode == 2 ? ic * kf : kf;
-            cplx<T> nx = rho * (x0 + ic * (bs - (x0 + bt * ax)));
-            if (mode == 3) nx = nx + one_m_rho * (ic * bs);
-            else if (mode == 0) nx = nx + one_m_rho * load_nt(xprev.p[r] + at);
-            if (fin) {
-                const cplx<double> xo = load_nt(fin_in.p[r] + at);
-                store_nt(fin_out.p[r] + at, cplx<double>{xo.re + (double)nx.re, xo.im + (double)nx.im});
-            } else {
-                store_nt(xnext.p[r] + at, nx);
-            }
+            cplx<T> nx = rho * (x0 + ic * (bq - (x0 + bt * ax)));
+            if (mode == 3) nx = nx + one_m_rho * (ic * bq);
+            else if (mode == 0) nx = nx + one_m_rho * xp[c];
+            if (fin)
+                store_nt(at32(fin_out.p[r] + cv, (uint32_t)s),
+                         cplx<double>{xo[c].re + (double)nx.re, xo[c].im + (double)nx.im});
+            else
+                store_nt(at32(xnext.p[r] + cv, (uint32_t)s), nx);
         }
     }
 }
@@ -620,7 +686,7 @@ struct workspace {
         const size_t f64 = (size_t)3 * (size_t)v * sizeof(cplx<double>) * rc;
         const size_t f32 = f64 / 2;
         int64_t sb = 0;
-        for (int md = 0; md <= 2; ++md) sb = std::max(sb, make_lf_map(lg, SBS, 8, md, 1).nsb);
+        for (int md = 0; md <= 3; ++md) sb = std::max(sb, make_lf_map(lg, md == 3 ? SBS_WAVE : SBS, 8, md, 1).nsb);
         auto dm = [](void **p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
         bool ok = dm(&basis, f64 * (size_t)(mm + 1)) && dm((void **)&y, f64) && dm((void **)&x, f64) &&
                   dm((void **)&t, f64) && dm((void **)&res, f64) && dm((void **)&y32, f32) &&
@@ -695,7 +761,9 @@ static int upload_active(const std::vector<int> &act) { return upload_ints(g_ws.
 // the initial value, nrq_cn_set_shape changes it.  Every shape computes the
 // same sums in the same order per slot.
 struct cn_shape {
-    int nc64 = 2, nc32 = 4, mode = 0, xcd = 8;
+    // measured fastest at 128^3 with 12 fields: one work item per wavefront,
+    // no XCD remap
+    int nc64 = 2, nc32 = 4, mode = 3, xcd = 1;
 };
 static bool cn_nc_ok(int v) { return v == 1 || v == 2 || v == 4; }
 static cn_shape read_shape()
@@ -706,7 +774,7 @@ static cn_shape read_shape()
         if (sscanf(e, "%d,%d,%d,%d", &a, &b, &c, &d) == 4) {
             if (cn_nc_ok(a)) sh.nc64 = a;
             if (cn_nc_ok(b)) sh.nc32 = b;
-            if (c >= 0 && c <= 2) sh.mode = c;
+            if (c >= 0 && c <= 3) sh.mode = c;
             if (d == 1 || d == 8) sh.xcd = d;
         }
     }
@@ -723,7 +791,7 @@ template <typename T> static int nc_of() { return sizeof(T) == 8 ? shape().nc64 
 // stencil launch map for nf slots, nc per thread
 static lf_map cn_map(int nf, int nc)
 {
-    return make_lf_map(lat().g, SBS, shape().xcd, shape().mode, (nf + nc - 1) / nc);
+    return make_lf_map(lat().g, shape().mode == 3 ? SBS_WAVE : SBS, shape().xcd, shape().mode, (nf + nc - 1) / nc);
 }
 
 // Runs the statements after `nc` with the compile-time chunk width NCV equal
@@ -742,7 +810,7 @@ static lf_map cn_map(int nf, int nc)
         }                                                                                                                \
     } while (0)
 
-static unsigned grid_of(const lf_map &mp) { return (unsigned)(mp.ngroup * mp.nsb); }
+static unsigned grid_of(const lf_map &mp) { return lf_blocks(mp); }
 
 // slot r -> base + r * 3V (workspace fields)
 template <typename T> static slots<T> ws_slots(cplx<T> *base, int R)
@@ -784,7 +852,7 @@ static void launch_stencil(const cplx<RL> *u, const slots<T> &in, const slots<T>
 {
     const int nc = nc_of<T>();
     const lf_map mp = cn_map(nf, nc);
-    NC_DISPATCH(nc, hipLaunchKernelGGL((k_stencil<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, in, out,
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_stencil<NCV, RL, T>), dim3(grid_of(mp)), dim3(mp.bs), 0, stream(), u, in, out,
                                        g_ws.beta, g_ws.active, nf, lat().g, mp, gco));
 }
 
@@ -795,7 +863,7 @@ static void launch_cheb(const cplx<RL> *links, const slots<T> &xk, const slots<T
 {
     const int nc = nc_of<T>();
     const lf_map mp = cn_map(R, nc);
-    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cheb<NCV, RL, T>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), links, xk, xnext,
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cheb<NCV, RL, T>), dim3(grid_of(mp)), dim3(mp.bs), 0, stream(), links, xk, xnext,
                                        xnext, b, g_ws.beta, cf, g_ws.active, R, lat().g, mp, mode, fin, fi, fo));
 }
 
@@ -807,7 +875,7 @@ static int64_t launch_resid(const cplx<RL> *u, int R, const slots<double> &x, co
     const slots<double> ty = ws_slots(g_ws.y, R);
     const int nc = shape().nc64;
     const lf_map mp = cn_map(R, nc);
-    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, TR, WRITE>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, x, ty,
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_resid<NCV, RL, TR, WRITE>), dim3(grid_of(mp)), dim3(mp.bs), 0, stream(), u, x, ty,
                                        res, g_ws.beta, R, lat().g, mp, g_ws.spart));
     return mp.nsb;
 }
@@ -820,7 +888,7 @@ static int64_t launch_rhs(const cplx<RL> *u, int R, const slots<double> &psi, co
 {
     const int nc = shape().nc64;
     const lf_map mp = cn_map(R, nc);
-    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(SBS), 0, stream(), u, psi, y, x,
+    NC_DISPATCH(nc, hipLaunchKernelGGL((k_cn_rhs<NCV, RL>), dim3(grid_of(mp)), dim3(mp.bs), 0, stream(), u, psi, y, x,
                                        g_ws.cpar, g_ws.cpar + NSLOTS, guess, R, lat().g, mp, g_ws.spart, kout, wk));
     return mp.nsb;
 }
@@ -1516,7 +1584,7 @@ using namespace nrq;
 extern "C" int nrq_cn_set_shape(int nc64, int nc32, int tiling, int xcd_remap)
 {
     if (!cn_nc_ok(nc64) || !cn_nc_ok(nc32)) return fail(NRQ_ERR_ARG, "CN shape: slots per thread must be 1, 2 or 4");
-    if (tiling < 0 || tiling > 2) return fail(NRQ_ERR_ARG, "CN shape: tiling %d out of range", tiling);
+    if (tiling < 0 || tiling > 3) return fail(NRQ_ERR_ARG, "CN shape: tiling %d out of range", tiling);
     if (xcd_remap != 1 && xcd_remap != 8) return fail(NRQ_ERR_ARG, "CN shape: XCD remap must be 1 or 8");
     cn_shape &sh = shape_mut();
     sh.nc64 = nc64;

@@ -11,6 +11,12 @@
 // Site blocks are either `bs` consecutive sites (mode 0) or z-sweep tiles
 // (mode 1): tile t = yband * n2 + z holds ty full x-rows at one z, so the
 // z +- 1 neighbours of a tile are the tiles just before and after it.
+//
+// Mode 3 (wave items): a site block is 64 consecutive sites, one per lane,
+// and each wavefront of a block takes another work item of the same 64
+// sites (up to wpb items per block, more blocks per site block beyond that).
+// The items of a site read the same links, and in one block those loads are
+// served by the CU's L1 instead of L2.
 #pragma once
 #include "nrq_su3.h"
 
@@ -25,7 +31,15 @@ struct lf_map {
     int ngroup;  // work items per site block
     int tx;      // mode 2: x extent of a tile
     int nxb;     // mode 2: tiles along x
+    int wpb;     // mode 3: work items (wavefronts) per block
 };
+
+// Blocks of a launch.
+__host__ __device__ __forceinline__ unsigned lf_blocks(const lf_map &m)
+{
+    if (m.mode == 3) return (unsigned)m.nsb * (unsigned)((m.ngroup + m.wpb - 1) / m.wpb);
+    return (unsigned)m.ngroup * (unsigned)m.nsb;
+}
 
 // Logical block of hardware block blockIdx.x among `total` when XCD x
 // (= blockIdx.x mod xcd) is to walk the x-th contiguous range of logical
@@ -42,7 +56,7 @@ __device__ __forceinline__ unsigned xcd_block(unsigned total, int xcdi)
 
 __device__ __forceinline__ unsigned lf_logical_block(const lf_map &m)
 {
-    return xcd_block((unsigned)m.ngroup * (unsigned)m.nsb, m.xcd);
+    return xcd_block(lf_blocks(m), m.xcd);
 }
 
 // Site of this thread, the work item (direction / chunk) of its block and the
@@ -50,6 +64,16 @@ __device__ __forceinline__ unsigned lf_logical_block(const lf_map &m)
 __device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p, int &item, int64_t &t)
 {
     const unsigned lb = lf_logical_block(m);
+    if (m.mode == 3) {
+        const unsigned nbg = (unsigned)((m.ngroup + m.wpb - 1) / m.wpb); // blocks per site block
+        const unsigned tt = lb / nbg;
+        item = (int)(lb - tt * nbg) * m.wpb + (int)(threadIdx.x >> 6);
+        t = tt;
+        const int64_t s = (int64_t)tt * 64 + (threadIdx.x & 63);
+        if (s >= g.vol || item >= m.ngroup) return false;
+        p = make_site(g, s);
+        return true;
+    }
     const unsigned ng = (unsigned)m.ngroup;
     const unsigned tt = lb / ng;
     item = (int)(lb - tt * ng);
@@ -84,14 +108,21 @@ __device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p
 }
 
 // Host: map for `bs` threads per block.  Mode 1 needs n0 <= bs; mode 2
-// uses 16-wide tiles (bs a multiple of 16); otherwise consecutive sites.
+// uses 16-wide tiles (bs a multiple of 16); mode 3 takes up to bs / 64 work
+// items per block (bs is then 64 times the items per block); otherwise
+// consecutive sites.
 inline lf_map make_lf_map(const geom &g, int bs, int xcd, int mode, int ngroup)
 {
     lf_map m{};
     m.bs = bs;
     m.xcd = xcd;
     m.ngroup = ngroup;
-    if (mode == 1 && g.n0 <= bs) {
+    if (mode == 3 && bs >= 64) {
+        m.mode = 3;
+        m.wpb = ngroup < bs / 64 ? ngroup : bs / 64;
+        m.bs = 64 * m.wpb;
+        m.nsb = (g.vol + 63) / 64;
+    } else if (mode == 1 && g.n0 <= bs) {
         m.mode = 1;
         m.ty = bs / g.n0 < g.n1 ? bs / g.n0 : g.n1;
         m.nsb = (int64_t)((g.n1 + m.ty - 1) / m.ty) * g.n2;
