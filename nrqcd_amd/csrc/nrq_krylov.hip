@@ -449,7 +449,11 @@ __global__ __launch_bounds__(SBS_WAVE, CN_WAVES) void k_cheb(const cplx<RL> *__r
 // vector kernels (grid: VBLOCKS x slots)
 // ---------------------------------------------------------------------------
 
-// partial[r][i][blk] = this block's share of <V_i,r, w_r>, i < nv (FP64 sums)
+// partial[r][i][blk] = this block's share of <V_i,r, w_r>, i < nv (FP64 sums).
+// The basis vectors go in groups of MDOT_G, each group in one pass that reads
+// w_r once (PETSc's VecMDot); per thread and per block the sums run in the
+// same order for every grouping.
+constexpr int MDOT_G = 8;
 template <typename T>
 __global__ __launch_bounds__(VTHREADS) void k_mdot(const cplx<T> *__restrict__ basis, int nv, const cplx<T> *__restrict__ w,
                                                    int64_t n, int nf, const int *__restrict__ active,
@@ -457,19 +461,41 @@ __global__ __launch_bounds__(VTHREADS) void k_mdot(const cplx<T> *__restrict__ b
 {
     const int r = blockIdx.y;
     if (!active[r]) return;
-    __shared__ double sh[8];
+    __shared__ double sh[VTHREADS / 64][2 * MDOT_G];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const cplx<T> *wr = w + (int64_t)r * n;
-    for (int i = 0; i < nv; ++i) {
-        const cplx<T> *vr = basis + ((int64_t)i * nf + r) * n;
-        double re = 0, im = 0;
+    for (int i0 = 0; i0 < nv; i0 += MDOT_G) {
+        const int gn = nv - i0 < MDOT_G ? nv - i0 : MDOT_G;
+        double re[MDOT_G], im[MDOT_G];
+#pragma unroll
+        for (int j = 0; j < MDOT_G; ++j) re[j] = im[j] = 0;
         for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x) {
-            const cplx<T> a = vr[k], b = wr[k];
-            re += (double)a.re * b.re + (double)a.im * b.im;
-            im += (double)a.re * b.im - (double)a.im * b.re;
+            const cplx<T> b = wr[k];
+#pragma unroll
+            for (int j = 0; j < MDOT_G; ++j) {
+                if (j >= gn) break;
+                const cplx<T> a = basis[((int64_t)(i0 + j) * nf + r) * n + k];
+                re[j] += (double)a.re * b.re + (double)a.im * b.im;
+                im[j] += (double)a.re * b.im - (double)a.im * b.re;
+            }
         }
-        re = block_sum(re, sh);
-        im = block_sum(im, sh);
-        if (threadIdx.x == 0) partial[((int64_t)r * nv + i) * gridDim.x + blockIdx.x] = {re, im};
+        __syncthreads(); // sh is free (previous group read)
+#pragma unroll
+        for (int j = 0; j < MDOT_G; ++j) {
+            const double x = wsum(re[j]), y = wsum(im[j]);
+            if (lane == 0) {
+                sh[wv][2 * j] = x;
+                sh[wv][2 * j + 1] = y;
+            }
+        }
+        __syncthreads();
+        if (threadIdx.x < 2 * gn) {
+            double t = 0;
+            for (int k = 0; k < VTHREADS / 64; ++k) t += sh[k][threadIdx.x];
+            const int j = threadIdx.x >> 1;
+            double *o = (double *)&partial[((int64_t)r * nv + i0 + j) * gridDim.x + blockIdx.x];
+            o[threadIdx.x & 1] = t;
+        }
     }
 }
 
