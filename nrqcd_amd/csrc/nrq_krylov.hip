@@ -499,29 +499,47 @@ __global__ __launch_bounds__(VTHREADS) void k_mdot(const cplx<T> *__restrict__ b
     }
 }
 
-// out[r*count + i] = sum over blocks (fixed order)
-__global__ void k_fold_c(const cplx<double> *__restrict__ partial, int nblk, int count, int nf,
-                         const int *__restrict__ active, cplx<double> *__restrict__ out)
+// out[t] = sum over the nblk block partials of t = r * count + i, one
+// wavefront per t (lane l sums partials l, l + 64, ..., then the wavefront
+// sum): a fixed order, and 64 loads in flight instead of one.
+__global__ __launch_bounds__(256) void k_fold_c(const cplx<double> *__restrict__ partial, int nblk, int count, int nf,
+                                                const int *__restrict__ active, cplx<double> *__restrict__ out)
 {
-    const int t = blockIdx.x * blockDim.x + threadIdx.x;
+    const int t = blockIdx.x * 4 + (threadIdx.x >> 6), lane = threadIdx.x & 63;
     if (t >= count * nf) return;
     if (!active[t / count]) return;
-    cplx<double> s = {0.0, 0.0};
-    for (int b = 0; b < nblk; ++b) s = s + partial[(int64_t)t * nblk + b];
-    out[t] = s;
+    double re = 0, im = 0;
+    for (int b = lane; b < nblk; b += 64) {
+        const cplx<double> v = partial[(int64_t)t * nblk + b];
+        re += v.re;
+        im += v.im;
+    }
+    re = wsum(re);
+    im = wsum(im);
+    if (lane == 0) out[t] = {re, im};
 }
 
 // out[r] = sum_b partial[r * nblk + b] for the slot r = blockIdx.x if
-// active[r] (active may be null); fixed summation order.
-__global__ __launch_bounds__(256) void k_fold_slots(const double *__restrict__ partial, int64_t nblk,
-                                                    const int *__restrict__ active, double *__restrict__ out)
+// active[r] (active may be null); fixed summation order.  FOLD_THREADS
+// threads with four independent sums each keep enough loads in flight for
+// the V / 64 partials of the wave-item map.
+constexpr int FOLD_THREADS = 1024;
+__global__ __launch_bounds__(FOLD_THREADS) void k_fold_slots(const double *__restrict__ partial, int64_t nblk,
+                                                             const int *__restrict__ active, double *__restrict__ out)
 {
     const int r = blockIdx.x;
     if (active && !active[r]) return;
-    __shared__ double sh[4];
-    double s = 0;
-    for (int64_t b = threadIdx.x; b < nblk; b += blockDim.x) s += partial[(int64_t)r * nblk + b];
-    s = block_sum(s, sh);
+    __shared__ double sh[FOLD_THREADS / 64];
+    const double *p = partial + (int64_t)r * nblk;
+    double s0 = 0, s1 = 0, s2 = 0, s3 = 0;
+    constexpr int64_t STEP = 4 * FOLD_THREADS;
+    for (int64_t b = threadIdx.x; b < nblk; b += STEP) {
+        s0 += p[b];
+        if (b + FOLD_THREADS < nblk) s1 += p[b + FOLD_THREADS];
+        if (b + 2 * FOLD_THREADS < nblk) s2 += p[b + 2 * FOLD_THREADS];
+        if (b + 3 * FOLD_THREADS < nblk) s3 += p[b + 3 * FOLD_THREADS];
+    }
+    double s = block_sum((s0 + s1) + (s2 + s3), sh);
     if (threadIdx.x == 0) out[r] = s;
 }
 
@@ -858,7 +876,7 @@ static slots<double> psi_slots(int R)
 // out[r] = sum of the nblk partials of slot r (r < nf; active may be null)
 static int fold_slots(const double *partial, int64_t nblk, int nf, const int *active, double *out)
 {
-    hipLaunchKernelGGL(k_fold_slots, dim3(nf), dim3(256), 0, stream(), partial, nblk, active, out);
+    hipLaunchKernelGGL(k_fold_slots, dim3(nf), dim3(FOLD_THREADS), 0, stream(), partial, nblk, active, out);
     NRQ_HIP(hipGetLastError());
     return NRQ_OK;
 }
@@ -1065,7 +1083,7 @@ static int gmres_core(const cplx<RL> *links, int R, const cplx<T> *b, cplx<T> *x
             const int nv = j + 1;
             hipLaunchKernelGGL(k_mdot<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<T> *)basis, nv,
                                (const cplx<T> *)w, n, R, g_ws.active, g_ws.cpart);
-            hipLaunchKernelGGL(k_fold_c, dim3((nv * R + 63) / 64), dim3(64), 0, stream(), g_ws.cpart, VBLOCKS, nv, R, g_ws.active,
+            hipLaunchKernelGGL(k_fold_c, dim3((nv * R + 3) / 4), dim3(256), 0, stream(), g_ws.cpart, VBLOCKS, nv, R, g_ws.active,
                                g_ws.hdev);
             hipLaunchKernelGGL(k_cgs_update<T>, dim3(VBLOCKS, R), dim3(VTHREADS), 0, stream(), (const cplx<T> *)basis, nv, w, n, R,
                                g_ws.active, g_ws.hdev, g_ws.rpart);

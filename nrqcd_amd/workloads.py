@@ -7,6 +7,8 @@
     python -m nrqcd_amd.workloads spectrum   # mode-31 rows at 64^3 (CN step, 8 correlators, norms) and
                                              # spectral functions of 1024 correlators of 4097 points
     python -m nrqcd_amd.workloads full       # CN steps with c = (1,1,1,1) and EquilibriumInit at 64^3
+    python -m nrqcd_amd.workloads cn         # CN steps/s per solver and stencil launch shape at 128^3,
+                                             # FP32-stored and FP64 links (--shape nc64,nc32,tiling,xcd ...)
 
 Each prints one JSON object.  Inputs are synthetic (the seeded box initial
 condition of gaugeconfiguration_su3.f90:906-951); all timing uses HIP events
@@ -21,7 +23,7 @@ import time
 
 import numpy as np
 
-from . import (GaugeConfiguration, GetNorm2Momentum_G, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
+from . import (cn_get_shape, cn_set_shape, GaugeConfiguration, GetNorm2Momentum_G, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
                NRQ_SOLVER_CHEBYSHEV_MIXED, NRQ_SOLVER_FGMRES_CNN, NRQ_SOLVER_GMRES_FP64, NRQ_SOLVER_GMRES_MIXED,
                PreconditionerNet, event_elapsed_ms, event_record, lattice_finalize, lattice_init, leapfrog_batch,
                nrqcd_finalize, nrqcd_init, nrqcd_set_preconditioner, nrqcd_set_solver, random_init, spectral_functions,
@@ -344,13 +346,63 @@ def full(size: int = 64, steps: int = 3, seed: int = 2, beta: float = 16.0, nequ
     return out
 
 
+def cn(size: int = 128, evolve: int = 50, seed: int = 3, cn_steps: int = 5, shapes=None,
+       solvers=("mixed_chebyshev", "plain_fp64_gmres")) -> dict:
+    """CN steps of point sources (Ma = 2, 12 fields, FP64 true relative
+    residual 1e-10) on the seeded box configuration evolved `evolve` steps,
+    stored in FP32 (reunitarised every 10 steps) and in FP64, for each
+    stencil launch shape (nc64, nc32, tiling, xcd) in turn; every shape twice,
+    interleaved, so that both passes see the same machine state."""
+    kinds = {"mixed_chebyshev": NRQ_SOLVER_CHEBYSHEV_MIXED, "plain_fp64_gmres": NRQ_SOLVER_GMRES_FP64,
+             "mixed_gmres": NRQ_SOLVER_GMRES_MIXED, "fp64_chebyshev": NRQ_SOLVER_CHEBYSHEV_FP64}
+    lattice_init((size,) * 3, SPACINGS)
+    confs = {}
+    for name, prec in (("fp32_links", NRQ_PRECISION_FP32), ("fp64_links", NRQ_PRECISION_FP64)):
+        random_init(seed)
+        g = GaugeConfiguration(prec)
+        g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+        if prec == NRQ_PRECISION_FP32:
+            g.SetReunitarisationInterval(10)
+        g.Update(1.0, evolve)
+        confs[name] = g
+    nrqcd_init(2.0, C1, 1.0, 1e-10)
+    default = cn_get_shape()
+    shapes = [tuple(sh) for sh in shapes] if shapes else [default]
+    out = {"size": size, "evolve": evolve, "cn_steps": cn_steps, "default_shape": list(default), "runs": []}
+    try:
+        for rep in range(2):
+            for sh in shapes:
+                cn_set_shape(*sh)
+                for conf, g in confs.items():
+                    for sv in solvers:
+                        if sv == "plain_fp64_gmres" and conf == "fp32_links":
+                            continue
+                        nrqcd_set_solver(kinds[sv], 30, 10000)
+                        h = NRQCDField()
+                        h.InitSinglePoint(1, 1)
+                        h.Update(g, 2.0, C1)
+                        n = cn_steps if "chebyshev" in sv else max(1, cn_steps // 2)
+                        its = []
+                        ms = _timed(lambda: its.extend(h.Update(g, 2.0, C1).iterations for _ in range(n)))
+                        out["runs"].append({"rep": rep, "shape": list(sh), "links": conf, "solver": sv,
+                                            "ms_per_step": ms / n, "iterations_per_step": sum(its) / n})
+                        del h
+    finally:
+        cn_set_shape(*default)
+        nrqcd_finalize()
+        confs.clear()
+        lattice_finalize()
+    return out
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon", "spectrum", "full"))
+    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon", "spectrum", "full", "cn"))
     ap.add_argument("--size", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--members", type=int, default=32)
     ap.add_argument("--cn-steps", type=int)
+    ap.add_argument("--shape", action="append", help="cn: stencil launch shape nc64,nc32,tiling,xcd (repeatable)")
     a = ap.parse_args(argv)
     kw = {k: v for k, v in (("size", a.size), ("steps", a.steps), ("cn_steps", a.cn_steps)) if v is not None}
     if a.workload == "mixed":
@@ -361,6 +413,9 @@ def main(argv=None) -> None:
         out = gluon(**{k: v for k, v in kw.items() if k in ("size", "steps")})
     elif a.workload == "full":
         out = full(**{k: v for k, v in kw.items() if k in ("size", "steps")})
+    elif a.workload == "cn":
+        shapes = [tuple(int(x) for x in sh.split(",")) for sh in a.shape] if a.shape else None
+        out = cn(shapes=shapes, **{k: v for k, v in kw.items() if k in ("size", "cn_steps")})
     elif a.workload == "spectrum":
         out = spectrum(**{k: v for k, v in kw.items() if k in ("size", "steps")})
     else:
