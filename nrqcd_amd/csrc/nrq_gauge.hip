@@ -292,7 +292,10 @@ __global__ __launch_bounds__(TX *TY * 3) void k_fused_tile(lf_batch<R> bt, geom 
     }
     __syncthreads();
 
-    const int i = t / (TX * TY);
+    // TX * TY is a multiple of 64, so the direction is the same for a whole
+    // wavefront: made scalar, every offset derived from it is scalar too
+    static_assert((TX * TY) % 64 == 0, "a wavefront must hold one direction");
+    const int i = __builtin_amdgcn_readfirstlane(t / (TX * TY));
     const int r = t - i * (TX * TY);
     const int tx = r % TX, ty = r / TX;
     // link of direction d at offset (dx, dy) in plane z + pz (pz = -1, 0, 1)
