@@ -676,10 +676,15 @@ __global__ void k_promote_add(cplx<double> *__restrict__ x, const cplx<float> *_
     }
 }
 
-__global__ void k_links_to_f32(const cplx<double> *__restrict__ u, cplx<float> *__restrict__ v, int64_t n)
+// FP32 copy of rows 1 and 2 of every link ([dir][e][site], e < 6): the FP32
+// stencils rebuild row 3, so rows 3 of the copy are neither written nor read.
+__global__ void k_links_to_f32(const cplx<double> *__restrict__ u, cplx<float> *__restrict__ v, int64_t vol)
 {
-    for (int64_t k = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; k < n; k += (int64_t)gridDim.x * blockDim.x)
+    const int64_t n6 = 6 * vol;
+    for (int64_t j = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; j < 3 * n6; j += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t k = j + (j / n6) * 3 * vol; // dir * 9V + e * V + site
         v[k] = {(float)u[k].re, (float)u[k].im};
+    }
 }
 
 // dst_r = src_r (n complex numbers per slot)
@@ -1385,7 +1390,7 @@ static int cheb_solve(const cplx<RL> *u, int R, const std::vector<zc> &alpha, co
     // time, in place afterwards)
     const cplx<float> *l32;
     if (sizeof(RL) == 8) {
-        hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)u, g_ws.links32, 27 * g_ws.vol);
+        hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)u, g_ws.links32, g_ws.vol);
         NRQ_HIP(hipGetLastError());
         l32 = g_ws.links32;
     } else {
@@ -1459,7 +1464,7 @@ static int gmres_mixed(const cplx<RL> *u, int R, std::vector<double> &bn, const 
     const int64_t n = 3 * g_ws.vol;
     const cplx<float> *l32;
     if (sizeof(RL) == 8) {
-        hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)u, g_ws.links32, 27 * g_ws.vol);
+        hipLaunchKernelGGL(k_links_to_f32, dim3(4096), dim3(256), 0, stream(), (const cplx<double> *)u, g_ws.links32, g_ws.vol);
         NRQ_HIP(hipGetLastError());
         l32 = g_ws.links32;
     } else {
