@@ -57,13 +57,17 @@ EPOCHS = 10           # training passes of the learned preconditioner
 FUSED_BYTES = {64: 1248.0, 32: 624.0}
 # Algorithmic HBM bytes per site of one mixed-Chebyshev CN step with 12 live
 # fields when every column takes the planned 6 Chebyshev steps (two FP32
-# corrections of 4 and 2 steps): right-hand side with y, x0 (FP64) and K psi
-# (FP32) 2448; FP32 link copy 648; K pass 792; Chebyshev launches 792, 1080,
-# 2232 (adds the correction to x0 in FP64) and 1656 (adds the second to psi);
-# two FP64 true-residual passes 2 x 1872.
-CN_MIXED_CHEB_BYTES = 2448.0 + 648 + 792 + 792 + 1080 + 2232 + 1656 + 2 * 1872
+# corrections of 4 and 2 steps), FP64 links: right-hand side with y, x0 (FP64)
+# and K psi (FP32) 2448; FP32 copy of two link rows 432; K pass 720; Chebyshev
+# launches 720, 1008, 2160 (adds the correction to x0 in FP64) and 1584 (adds
+# the second to psi); two FP64 true-residual passes 2 x 1872.  The FP32 passes
+# read two rows of each link (144 B per site).
+CN_MIXED_CHEB_BYTES = 2448.0 + 432 + 720 + 720 + 1008 + 2160 + 1584 + 2 * 1872
 CN_MIXED_CHEB_ITS = 6.0
 TRAFFIC_FILE = ROOT / "profiles" / "leapfrog_fused_128cubed_traffic.txt"
+# one network apply: 2 flop per multiply-add, 27 taps, 18 -> 32 -> 32 -> 32 -> 18 channels, per site
+CNN_FLOP_PER_SITE = 2 * 27 * (18 * 32 + 32 * 32 + 32 * 32 + 32 * 18)
+BF16_MFMA_PEAK_TFLOPS = 2500.0
 TILINGS = ("consecutive sites", "z-sweep tiles of x-rows", "z-sweep 16-wide tiles", "LDS tiles, 16-plane runs",
            "LDS tiles, 8-plane runs")
 
@@ -180,6 +184,7 @@ def _train(nrq, g_ic):
     info = {"configuration": f"seed {SEED} evolved 400 steps (t = 10/Qs)", "training_pairs": net.dataset_size(),
             "patches_16cubed": net.dataset_size() * (nrq.lattice_size() // 4096), "epochs": EPOCHS,
             "loss_first_epoch": loss0, "loss_last_epoch": loss1, "harvest_and_training_s": time.perf_counter() - t0}
+    info["network_apply_ms"] = net.time_forward(20)
     return net, g, info
 
 
@@ -326,6 +331,13 @@ def main() -> None:
         cn_roof = {"solver": "mixed_chebyshev", "bound": "hbm", "achieved": gbs, "peak": HBM_PEAK_GBS, "unit": "GB/s",
                    "frac": gbs / HBM_PEAK_GBS, "bytes_per_step": CN_MIXED_CHEB_BYTES * V}
 
+    cnn_roof = None
+    if learned is not None:
+        tf = CNN_FLOP_PER_SITE * V / (learned["network_apply_ms"] * 1e-3) / 1e12
+        cnn_roof = {"kernel": "k_conv x 4 layers (one network apply, one colour field and its features)", "bound": "mfma",
+                    "achieved": tf, "peak": BF16_MFMA_PEAK_TFLOPS, "unit": "TFLOP/s", "frac": tf / BF16_MFMA_PEAK_TFLOPS,
+                    "flop_per_apply": CNN_FLOP_PER_SITE * V, "ms_per_apply": learned["network_apply_ms"]}
+
     cpu = None
     if not args.no_cpu:
         U, E = g_ic.download()
@@ -369,6 +381,7 @@ def main() -> None:
         "cn_on_fp32_stored_links_speedup_vs_plain": {k: v["cn_steps_per_s"] / plain for k, v in cn_fp32_links.items()},
         "cn_mixed_vs_fp64_chebyshev": mc["cn_steps_per_s"] / cn["fp64_chebyshev"]["cn_steps_per_s"],
         "cn_roofline": cn_roof,
+        "cnn_roofline": cnn_roof,
         "learned": learned,
         "hbm_copy_probe_gbs": copy_gbs,
         "roofline": {

@@ -421,6 +421,12 @@ int nrq_cnn_get_weights(const nrq_cnn *net, float *w);
 int nrq_cnn_set_weights(nrq_cnn *net, const float *w);
 /* out = N(in) on the current lattice; in, out: [site][18] floats (host) */
 int nrq_cnn_forward(nrq_cnn *net, const float *in, float *out);
+/* Mean device time of one network apply (four convolution layers, one colour
+ * field with its features, inputs resident on the device) over `reps`
+ * applies after one warm-up, by HIP events on the library stream.  For
+ * reporting the network against the matrix-core peak; no reference
+ * counterpart. */
+int nrq_cnn_time_forward(nrq_cnn *net, int reps, double *ms);
 /* loss = sum (N(in) - target)^2 / norm and its gradient with respect to the
  * NRQ_CNN_NWEIGHTS weights (host arrays, [site][18] layouts) */
 int nrq_cnn_loss_grad(nrq_cnn *net, const float *in, const float *target, double norm, double *loss, float *grad);

@@ -159,6 +159,7 @@ def _declare(lib: ctypes.CDLL) -> None:
         "nrq_cnn_get_weights": (I, [P, P]),
         "nrq_cnn_set_weights": (I, [P, P]),
         "nrq_cnn_forward": (I, [P, P, P]),
+        "nrq_cnn_time_forward": (I, [P, I, P]),
         "nrq_cnn_loss_grad": (I, [P, P, P, D, P, P]),
         "nrq_cnn_loss_grad_trace": (I, [P, P, P, D, P, P, P, P]),
         "nrq_cnn_harvest": (I, [P, P, D, D, I, ctypes.c_uint64]),
@@ -726,6 +727,13 @@ class PreconditionerNet:
         out = np.zeros_like(inp)
         _check(library().nrq_cnn_forward(self._h, _ptr(inp), _ptr(out)))
         return out
+
+    def time_forward(self, reps: int = 20) -> float:
+        """Mean device milliseconds of one network apply on the current
+        lattice (inputs resident on the GPU)."""
+        ms = ctypes.c_double()
+        _check(library().nrq_cnn_time_forward(self._h, int(reps), ctypes.byref(ms)))
+        return ms.value
 
     def loss_grad(self, inp: np.ndarray, target: np.ndarray, norm: float = 1.0):
         inp = np.ascontiguousarray(inp, dtype=np.float32)

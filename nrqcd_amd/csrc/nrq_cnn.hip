@@ -11,8 +11,9 @@
 // commutes with the scale of its input.
 //
 // Convolutions are implicit GEMMs on BF16 matrix cores with FP32
-// accumulation: a wavefront computes 16 consecutive sites x 32 output
-// channels with one v_mfma_f32_16x16x32_bf16 per (tap, 16 output channels);
+// accumulation: a wavefront computes four runs of 16 consecutive sites x 32
+// output channels with one v_mfma_f32_16x16x32_bf16 per (run, tap, 16 output
+// channels), each tap's weights loaded once for the four runs;
 // K = 32 input channels per tap.  Activations are channels-last BF16
 // [site][32] (channels past 18 are zero in the outer layers), weights
 // [layer][tap][co][ci] with tap = 9 (dz+1) + 3 (dy+1) + (dx+1).  Backward:
@@ -60,58 +61,74 @@ __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >=
 // out[s][co] = sum_{tap, ci} w[tap][co][ci] in[s + off(tap)][ci] for one
 // layer.  out_f != null: FP32 output; else BF16 output, with ReLU when relu,
 // and zeroed where mask[s][co] <= 0 when mask != null (the ReLU derivative of
-// the backward pass).  Grid: 4 wavefronts per block, one per 16-site x run.
+// the backward pass).  Grid: 4 wavefronts per block; a wavefront computes
+// CONV_SEG consecutive 16-site x runs (in x, then y, then z order) and loads
+// each tap's weights once for all of them.
+constexpr int CONV_SEG = 4;
 __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const bf16 *__restrict__ w,
                                               bf16 *__restrict__ out_b, float *__restrict__ out_f,
                                               const bf16 *__restrict__ mask, int relu, geom g)
 {
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
     const int nsx = g.n0 >> 4;
-    const int64_t seg = (int64_t)blockIdx.x * 4 + wv;
-    if (seg >= (int64_t)nsx * g.n1 * g.n2) return;
-    const int xs = (int)(seg % nsx);
-    const int64_t yz = seg / nsx;
-    const int y = (int)(yz % g.n1), z = (int)(yz / g.n1);
+    const int64_t nseg = (int64_t)nsx * g.n1 * g.n2;
+    const int64_t seg0 = ((int64_t)blockIdx.x * 4 + wv) * CONV_SEG;
+    if (seg0 >= nseg) return;
     const int m = lane & 15, kq = lane >> 4;
-    const int x = xs * 16 + m;
-    f32x4 acc0 = {0.f, 0.f, 0.f, 0.f}, acc1 = {0.f, 0.f, 0.f, 0.f};
+    int xs[CONV_SEG], ys[CONV_SEG], zs[CONV_SEG];
+#pragma unroll
+    for (int j = 0; j < CONV_SEG; ++j) {
+        const int64_t sg = seg0 + j < nseg ? seg0 + j : seg0; // a run past the end repeats the first
+        xs[j] = (int)(sg % nsx);
+        const int64_t yz = sg / nsx;
+        ys[j] = (int)(yz % g.n1);
+        zs[j] = (int)(yz / g.n1);
+    }
+    f32x4 acc0[CONV_SEG], acc1[CONV_SEG];
+#pragma unroll
+    for (int j = 0; j < CONV_SEG; ++j) acc0[j] = acc1[j] = f32x4{0.f, 0.f, 0.f, 0.f};
     int tap = 0;
-    for (int dz = -1; dz <= 1; ++dz) {
-        const int zz = wrap(z + dz, g.n2);
-        for (int dy = -1; dy <= 1; ++dy) {
-            const int yy = wrap(y + dy, g.n1);
+    for (int dz = -1; dz <= 1; ++dz)
+        for (int dy = -1; dy <= 1; ++dy)
             for (int dx = -1; dx <= 1; ++dx, ++tap) {
-                const int xx = wrap(x + dx, g.n0);
-                const int64_t nb = xx + (int64_t)g.n0 * (yy + (int64_t)g.n1 * zz);
-                // A[m][k] = in[site m][ci k], B[k][n] = w[tap][co n][ci k]
-                const bf16x8 a = *(const bf16x8 *)(in + nb * CH + 8 * kq);
+                // B[k][n] = w[tap][co n][ci k], shared by the runs
                 const bf16x8 b0 = *(const bf16x8 *)(w + ((int64_t)tap * CH + m) * CH + 8 * kq);
                 const bf16x8 b1 = *(const bf16x8 *)(w + ((int64_t)tap * CH + 16 + m) * CH + 8 * kq);
-                acc0 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0, 0, 0, 0);
-                acc1 = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1, 0, 0, 0);
-            }
-        }
-    }
-    // lane holds D[site 4 kq + i][co m] and D[site 4 kq + i][co 16 + m]
-    const int64_t base = (int64_t)xs * 16 + (int64_t)g.n0 * (y + (int64_t)g.n1 * z);
 #pragma unroll
-    for (int i = 0; i < 4; ++i) {
-        const int64_t s = base + 4 * kq + i;
-        float v0 = acc0[i], v1 = acc1[i];
-        if (out_f) {
-            out_f[s * CH + m] = v0;
-            out_f[s * CH + 16 + m] = v1;
-        } else {
-            if (relu) {
-                v0 = fmaxf(v0, 0.f);
-                v1 = fmaxf(v1, 0.f);
+                for (int j = 0; j < CONV_SEG; ++j) {
+                    const int xx = wrap(xs[j] * 16 + m + dx, g.n0);
+                    const int yy = wrap(ys[j] + dy, g.n1), zz = wrap(zs[j] + dz, g.n2);
+                    const int64_t nb = xx + (int64_t)g.n0 * (yy + (int64_t)g.n1 * zz);
+                    // A[m][k] = in[site m][ci k]
+                    const bf16x8 a = *(const bf16x8 *)(in + nb * CH + 8 * kq);
+                    acc0[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0[j], 0, 0, 0);
+                    acc1[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1[j], 0, 0, 0);
+                }
             }
-            if (mask) {
-                if (!(bf2f(mask[s * CH + m]) > 0.f)) v0 = 0.f;
-                if (!(bf2f(mask[s * CH + 16 + m]) > 0.f)) v1 = 0.f;
+    // lane holds D[site 4 kq + i][co m] and D[site 4 kq + i][co 16 + m] of each run
+#pragma unroll
+    for (int j = 0; j < CONV_SEG; ++j) {
+        if (seg0 + j >= nseg) break;
+        const int64_t base = (int64_t)xs[j] * 16 + (int64_t)g.n0 * (ys[j] + (int64_t)g.n1 * zs[j]);
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int64_t s = base + 4 * kq + i;
+            float v0 = acc0[j][i], v1 = acc1[j][i];
+            if (out_f) {
+                out_f[s * CH + m] = v0;
+                out_f[s * CH + 16 + m] = v1;
+            } else {
+                if (relu) {
+                    v0 = fmaxf(v0, 0.f);
+                    v1 = fmaxf(v1, 0.f);
+                }
+                if (mask) {
+                    if (!(bf2f(mask[s * CH + m]) > 0.f)) v0 = 0.f;
+                    if (!(bf2f(mask[s * CH + 16 + m]) > 0.f)) v1 = 0.f;
+                }
+                out_b[s * CH + m] = f2bf(v0);
+                out_b[s * CH + 16 + m] = f2bf(v1);
             }
-            out_b[s * CH + m] = f2bf(v0);
-            out_b[s * CH + 16 + m] = f2bf(v1);
         }
     }
 }
@@ -420,7 +437,11 @@ static int ensure_geom(nrq_cnn *n, const geom &g)
     return NRQ_OK;
 }
 
-static unsigned conv_grid(const geom &g) { return (unsigned)(((int64_t)(g.n0 / 16) * g.n1 * g.n2 + 3) / 4); }
+static unsigned conv_grid(const geom &g)
+{
+    const int64_t waves = ((int64_t)(g.n0 / 16) * g.n1 * g.n2 + CONV_SEG - 1) / CONV_SEG;
+    return (unsigned)((waves + 3) / 4);
+}
 
 // forward pass from n->act[0]; FP32 output in n->out
 static int forward(nrq_cnn *n)
@@ -615,6 +636,35 @@ extern "C" int nrq_cnn_forward(nrq_cnn *n, const float *in, float *out)
     NRQ_HIP(hipStreamSynchronize(stream()));
     for (int64_t s = 0; s < g.vol; ++s)
         for (int c = 0; c < CIN; ++c) out[(size_t)s * CIN + c] = o[(size_t)s * CH + c];
+    return NRQ_OK;
+}
+
+extern "C" int nrq_cnn_time_forward(nrq_cnn *n, int reps, double *ms)
+{
+    NEED_NET(n);
+    if (!lat().initialised) return fail(NRQ_ERR_STATE, "lattice: not initialised");
+    if (!ms || reps < 1) return fail(NRQ_ERR_ARG, "cnn: bad argument");
+    const geom g = lat().g;
+    if (int e = ensure_geom(n, g)) return e;
+    // the time of the matrix-core passes does not depend on the values
+    NRQ_HIP(hipMemsetAsync(n->act[0], 0, (size_t)g.vol * CH * sizeof(bf16), stream()));
+    hipEvent_t ev[2];
+    NRQ_HIP(hipEventCreate(&ev[0]));
+    if (hipEventCreate(&ev[1]) != hipSuccess) {
+        (void)hipEventDestroy(ev[0]);
+        return fail(NRQ_ERR_HIP, "cnn: event creation failed");
+    }
+    int rc = forward(n); // warm-up
+    if (!rc && hipEventRecord(ev[0], stream()) != hipSuccess) rc = fail(NRQ_ERR_HIP, "cnn: event record failed");
+    for (int r = 0; r < reps && !rc; ++r) rc = forward(n);
+    float t = 0;
+    if (!rc && (hipEventRecord(ev[1], stream()) != hipSuccess || hipEventSynchronize(ev[1]) != hipSuccess ||
+                hipEventElapsedTime(&t, ev[0], ev[1]) != hipSuccess))
+        rc = fail(NRQ_ERR_HIP, "cnn: event timing failed");
+    (void)hipEventDestroy(ev[0]);
+    (void)hipEventDestroy(ev[1]);
+    if (rc) return rc;
+    *ms = (double)t / reps;
     return NRQ_OK;
 }
 

@@ -24,9 +24,11 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-def test_forward_matches_oracle(nrq, lattice):
-    ext = (16, 4, 3)
-    _say("cnn forward 16x4x3")
+@pytest.mark.parametrize("ext", [(16, 4, 3), (16, 5, 3), (32, 3, 5)])
+def test_forward_matches_oracle(nrq, lattice, ext):
+    """A wavefront convolves four 16-site runs: 12, 15 (the last wavefront
+    has three runs) and 30 runs (two per x row)."""
+    _say(f"cnn forward {ext}")
     lattice(ext, SP)
     rng = np.random.default_rng(4)
     v = ext[0] * ext[1] * ext[2]
@@ -291,3 +293,19 @@ def test_learned_preconditioner_at_128cubed(nrq, lattice):
         nrq.nrqcd_set_preconditioner(None)
     finally:
         nrq.nrqcd_finalize()
+
+
+def test_network_apply_timer(nrq, lattice):
+    """nrq_cnn_time_forward times the device-resident apply; it leaves the
+    weights alone and gives the same time scale for more applies."""
+    lattice((32, 16, 16), SP)
+    net = nrq.PreconditionerNet(5)
+    w = net.weights.copy()
+    a = net.time_forward(3)
+    b = net.time_forward(12)
+    _say(f"network apply at 32x16x16: {a:.4f} ms (3 applies), {b:.4f} ms (12 applies)")
+    assert np.isfinite(a) and np.isfinite(b) and a > 0 and b > 0
+    assert 0.2 < a / b < 5.0
+    assert np.array_equal(net.weights, w)
+    with pytest.raises(nrq.NRQError):
+        net.time_forward(0)
