@@ -11,9 +11,9 @@
 // commutes with the scale of its input.
 //
 // Convolutions are implicit GEMMs on BF16 matrix cores with FP32
-// accumulation: a wavefront computes four runs of 16 consecutive sites x 32
+// accumulation: a wavefront computes eight runs of 16 consecutive sites x 32
 // output channels with one v_mfma_f32_16x16x32_bf16 per (run, tap, 16 output
-// channels), each tap's weights loaded once for the four runs;
+// channels), each tap's weights loaded once for the eight runs;
 // K = 32 input channels per tap.  Activations are channels-last BF16
 // [site][32] (channels past 18 are zero in the outer layers), weights
 // [layer][tap][co][ci] with tap = 9 (dz+1) + 3 (dy+1) + (dx+1).  Backward:
@@ -64,7 +64,7 @@ __device__ __forceinline__ int wrap(int a, int n) { return a < 0 ? a + n : (a >=
 // the backward pass).  Grid: 4 wavefronts per block; a wavefront computes
 // CONV_SEG consecutive 16-site x runs (in x, then y, then z order) and loads
 // each tap's weights once for all of them.
-constexpr int CONV_SEG = 4;
+constexpr int CONV_SEG = 8;
 __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const bf16 *__restrict__ w,
                                               bf16 *__restrict__ out_b, float *__restrict__ out_f,
                                               const bf16 *__restrict__ mask, int relu, geom g)

@@ -26,8 +26,8 @@ def _rel(a, b):
 
 @pytest.mark.parametrize("ext", [(16, 4, 3), (16, 5, 3), (32, 3, 5)])
 def test_forward_matches_oracle(nrq, lattice, ext):
-    """A wavefront convolves four 16-site runs: 12, 15 (the last wavefront
-    has three runs) and 30 runs (two per x row)."""
+    """A wavefront convolves eight 16-site runs: 12 (the last wavefront has
+    four runs), 15 (seven) and 30 runs (two per x row, six in the last)."""
     _say(f"cnn forward {ext}")
     lattice(ext, SP)
     rng = np.random.default_rng(4)
