@@ -133,6 +133,109 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const
     }
 }
 
+// k_conv with the input staged in LDS, for lattices with n0 % 128 == 0, even
+// n1 and n2 % ZC == 0: a block of 4 wavefronts owns a 128 x 2 tile of (x, y)
+// and sweeps ZC planes in z.  LDS holds a ring of three input planes of the
+// tile with a one-site halo (130 x 4 sites x 32 BF16 channels each, 100 KB);
+// output plane z reads planes z - 1, z, z + 1 there instead of re-reading each
+// input site 27 times through L1, while plane z + 2 streams into registers
+// and then replaces plane z - 1.  Wavefront w computes row y0 + (w & 1), x runs
+// 4 (w >> 1) .. 4 (w >> 1) + 3.  Same arguments, same arithmetic and summation
+// order per output as k_conv.
+constexpr int LT_X = 128, LT_HX = LT_X + 2, LT_HY = 4;
+constexpr int LT_PLANE = LT_HX * LT_HY * 4;                     // bf16x8 per plane
+constexpr int LT_PER = (LT_PLANE + 255) / 256;                  // per thread and plane
+template <int ZC>
+__global__ __launch_bounds__(256) void k_conv_lds(const bf16 *__restrict__ in, const bf16 *__restrict__ w,
+                                                  bf16 *__restrict__ out_b, float *__restrict__ out_f,
+                                                  const bf16 *__restrict__ mask, int relu, geom g)
+{
+    __shared__ bf16x8 lds[3 * LT_PLANE];
+    const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    const int ntx = g.n0 / LT_X, nty = g.n1 / 2;
+    const int b = blockIdx.x;
+    const int x0 = (b % ntx) * LT_X, y0 = ((b / ntx) % nty) * 2, z0 = (b / (ntx * nty)) * ZC;
+    bf16x8 pre[LT_PER];
+    auto gload = [&](int z) {
+        const int gz = wrap(z, g.n2);
+#pragma unroll
+        for (int k = 0; k < LT_PER; ++k) {
+            const int q = threadIdx.x + 256 * k;
+            if (q < LT_PLANE) {
+                const int c4 = q & 3, site = q >> 2;
+                const int hx = site % LT_HX, hy = site / LT_HX;
+                const int gx = wrap(x0 - 1 + hx, g.n0), gy = wrap(y0 - 1 + hy, g.n1);
+                const int64_t gs = gx + (int64_t)g.n0 * (gy + (int64_t)g.n1 * gz);
+                pre[k] = *(const bf16x8 *)(in + gs * CH + 8 * c4);
+            }
+        }
+    };
+    auto sstore = [&](int slot) {
+#pragma unroll
+        for (int k = 0; k < LT_PER; ++k) {
+            const int q = threadIdx.x + 256 * k;
+            if (q < LT_PLANE) lds[slot * LT_PLANE + q] = pre[k];
+        }
+    };
+    for (int p = 0; p < 3; ++p) {
+        gload(z0 - 1 + p);
+        sstore(p);
+    }
+    __syncthreads();
+    const int m = lane & 15, kq = lane >> 4;
+    const int ry = wv & 1, xr0 = 4 * (wv >> 1);
+#pragma unroll 1
+    for (int kz = 0; kz < ZC; ++kz) {
+        const bool more = kz + 1 < ZC;
+        if (more) gload(z0 + kz + 2);
+        f32x4 acc0[4], acc1[4];
+#pragma unroll
+        for (int j = 0; j < 4; ++j) acc0[j] = acc1[j] = f32x4{0.f, 0.f, 0.f, 0.f};
+        int tap = 0;
+        for (int dz = -1; dz <= 1; ++dz) {
+            const int slot = (kz + 1 + dz) % 3; // slots hold planes z0 - 1 + p at p mod 3
+            for (int dy = -1; dy <= 1; ++dy)
+                for (int dx = -1; dx <= 1; ++dx, ++tap) {
+                    const bf16x8 b0 = *(const bf16x8 *)(w + ((int64_t)tap * CH + m) * CH + 8 * kq);
+                    const bf16x8 b1 = *(const bf16x8 *)(w + ((int64_t)tap * CH + 16 + m) * CH + 8 * kq);
+                    const bf16x8 *row = lds + slot * LT_PLANE + ((ry + dy + 1) * LT_HX + 16 * xr0 + m + dx + 1) * 4 + kq;
+#pragma unroll
+                    for (int j = 0; j < 4; ++j) {
+                        const bf16x8 a = row[16 * j * 4];
+                        acc0[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b0, acc0[j], 0, 0, 0);
+                        acc1[j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a, b1, acc1[j], 0, 0, 0);
+                    }
+                }
+        }
+        const int64_t base = x0 + 16 * xr0 + (int64_t)g.n0 * ((y0 + ry) + (int64_t)g.n1 * (z0 + kz));
+#pragma unroll
+        for (int j = 0; j < 4; ++j)
+#pragma unroll
+            for (int i = 0; i < 4; ++i) {
+                const int64_t s = base + 16 * j + 4 * kq + i;
+                float v0 = acc0[j][i], v1 = acc1[j][i];
+                if (out_f) {
+                    out_f[s * CH + m] = v0;
+                    out_f[s * CH + 16 + m] = v1;
+                } else {
+                    if (relu) {
+                        v0 = fmaxf(v0, 0.f);
+                        v1 = fmaxf(v1, 0.f);
+                    }
+                    if (mask) {
+                        if (!(bf2f(mask[s * CH + m]) > 0.f)) v0 = 0.f;
+                        if (!(bf2f(mask[s * CH + 16 + m]) > 0.f)) v1 = 0.f;
+                    }
+                    out_b[s * CH + m] = f2bf(v0);
+                    out_b[s * CH + 16 + m] = f2bf(v1);
+                }
+            }
+        __syncthreads(); // plane z - 1 is no longer read
+        if (more) sstore(kz % 3);
+        __syncthreads();
+    }
+}
+
 // Partial weight gradient of one block of DW_SITES sites, on matrix cores:
 //   part[blk][tap][co][ci] = sum_s d[s][co] in[s + off(tap)][ci]
 // a GEMM over sites, 32 per MFMA step: A[co][site] = d, B[site][ci] = the
@@ -443,6 +546,21 @@ static unsigned conv_grid(const geom &g)
     return (unsigned)((waves + 3) / 4);
 }
 
+// One convolution layer (k_conv arguments), LDS-staged where the extents
+// allow it.
+static void conv(const geom &g, const bf16 *in, const bf16 *w, bf16 *out_b, float *out_f, const bf16 *mask, int relu)
+{
+    if (g.n0 % LT_X == 0 && g.n1 % 2 == 0 && g.n2 % 16 == 0) {
+        const unsigned blocks = (unsigned)((g.n0 / LT_X) * (g.n1 / 2) * (g.n2 / 16));
+        hipLaunchKernelGGL(k_conv_lds<16>, dim3(blocks), dim3(256), 0, stream(), in, w, out_b, out_f, mask, relu, g);
+    } else if (g.n0 % LT_X == 0 && g.n1 % 2 == 0 && g.n2 % 8 == 0) {
+        const unsigned blocks = (unsigned)((g.n0 / LT_X) * (g.n1 / 2) * (g.n2 / 8));
+        hipLaunchKernelGGL(k_conv_lds<8>, dim3(blocks), dim3(256), 0, stream(), in, w, out_b, out_f, mask, relu, g);
+    } else {
+        hipLaunchKernelGGL(k_conv, dim3(conv_grid(g)), dim3(256), 0, stream(), in, w, out_b, out_f, mask, relu, g);
+    }
+}
+
 // forward pass from n->act[0]; FP32 output in n->out
 static int forward(nrq_cnn *n)
 {
@@ -450,11 +568,9 @@ static int forward(nrq_cnn *n)
     for (int l = 0; l < NL; ++l) {
         const bf16 *w = n->wb + (int64_t)l * WL;
         if (l < NL - 1)
-            hipLaunchKernelGGL(k_conv, dim3(conv_grid(g)), dim3(256), 0, stream(), n->act[l], w, n->act[l + 1], (float *)nullptr,
-                               (const bf16 *)nullptr, 1, g);
+            conv(g, n->act[l], w, n->act[l + 1], nullptr, nullptr, 1);
         else
-            hipLaunchKernelGGL(k_conv, dim3(conv_grid(g)), dim3(256), 0, stream(), n->act[l], w, (bf16 *)nullptr, n->out,
-                               (const bf16 *)nullptr, 0, g);
+            conv(g, n->act[l], w, nullptr, n->out, nullptr, 0);
     }
     NRQ_HIP(hipGetLastError());
     return NRQ_OK;
@@ -480,8 +596,7 @@ static int backward(nrq_cnn *n, const float *tgt, double norm, bf16 *dtrace = nu
                            n->grad + (int64_t)l * WL);
         if (l > 0) {
             // d(pre-activation of layer l-1) = conv(d, mirrored W_l^T) where act[l] > 0
-            hipLaunchKernelGGL(k_conv, dim3(conv_grid(g)), dim3(256), 0, stream(), (const bf16 *)d, n->wt + (int64_t)l * WL, dn,
-                               (float *)nullptr, (const bf16 *)n->act[l], 0, g);
+            conv(g, d, n->wt + (int64_t)l * WL, dn, nullptr, n->act[l], 0);
             std::swap(d, dn);
         }
     }

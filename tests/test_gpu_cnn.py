@@ -24,10 +24,13 @@ def _rel(a, b):
     return float(np.linalg.norm(a - b) / np.linalg.norm(b))
 
 
-@pytest.mark.parametrize("ext", [(16, 4, 3), (16, 5, 3), (32, 3, 5)])
+@pytest.mark.parametrize("ext", [(16, 4, 3), (16, 5, 3), (32, 3, 5), (128, 2, 8), (128, 4, 16)])
 def test_forward_matches_oracle(nrq, lattice, ext):
     """A wavefront convolves eight 16-site runs: 12 (the last wavefront has
-    four runs), 15 (seven) and 30 runs (two per x row, six in the last)."""
+    four runs), 15 (seven) and 30 runs (two per x row, six in the last); at
+    x extent 128, even y and z a multiple of 8 a block sweeps 8 planes of a
+    128 x 2 tile with its input in an LDS ring (halo wrapping in every
+    direction; one and two blocks along z)."""
     _say(f"cnn forward {ext}")
     lattice(ext, SP)
     rng = np.random.default_rng(4)
@@ -42,9 +45,9 @@ def test_forward_matches_oracle(nrq, lattice, ext):
     assert _rel(out, ref.reshape(v, co.CIN)) < 1e-2
 
 
-def test_loss_gradient_matches_oracle(nrq, lattice):
-    ext = (16, 3, 2)
-    _say("cnn loss/gradient 16x3x2")
+@pytest.mark.parametrize("ext", [(16, 3, 2), (128, 2, 8)])
+def test_loss_gradient_matches_oracle(nrq, lattice, ext):
+    _say(f"cnn loss/gradient {ext}")
     lattice(ext, SP)
     rng = np.random.default_rng(5)
     v = ext[0] * ext[1] * ext[2]
@@ -64,7 +67,8 @@ def test_loss_gradient_matches_oracle(nrq, lattice):
         assert _rel(g4[l], r4[l]) < 5e-2, l
 
 
-def test_each_layer_from_the_networks_own_values(nrq, lattice):
+@pytest.mark.parametrize("ext", [(16, 3, 2), (128, 2, 8)])
+def test_each_layer_from_the_networks_own_values(nrq, lattice, ext):
     """Layer by layer from the GPU's own BF16 values, so that a rounding
     branch taken differently in float64 cannot hide or fake an error: the
     weight gradient of layer l is sum_s d_l(s) x act_l(s + tap) (float64 here)
@@ -72,8 +76,7 @@ def test_each_layer_from_the_networks_own_values(nrq, lattice):
     convolution of the previous one, each output gradient the BF16 ReLU-masked
     transposed convolution of the next, and the last one 2 (N - t) / norm
     (bars 1e-2: one BF16 rounding)."""
-    ext = (16, 3, 2)
-    _say("cnn layers from the GPU's own activations 16x3x2")
+    _say(f"cnn layers from the GPU's own activations {ext}")
     lattice(ext, SP)
     rng = np.random.default_rng(6)
     v = ext[0] * ext[1] * ext[2]
