@@ -35,7 +35,7 @@ namespace {
 constexpr int CH = 32, TAPS = 27, NL = 4, CIN = 18;
 constexpr int WL = TAPS * CH * CH;   // weights per layer (padded)
 constexpr int NW = NL * WL;          // all weights
-constexpr int DW_SITES = 512;        // sites per block of the weight-gradient kernel
+constexpr int DW_SITES = 4096;       // sites per block of the weight-gradient kernel
 
 typedef uint16_t bf16;
 typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
