@@ -118,12 +118,11 @@ __device__ __forceinline__ mat3<R> staple_sum(const cplx<R> *__restrict__ u, con
         const int64_t xk = site_up(g, p, k).s;
         const int64_t xmk = pmk.s;
         const int64_t ximk = site_up(g, pmk, i).s;
-        mat3<R> su = mul_nd(load_nb<REC>(u, g.vol, k, xi), load_nb<REC>(u, g.vol, i, xk));
-        su = mul_nd(su, load_nb<REC>(u, g.vol, k, p.s));
-        mat3<R> sd = mul_dd(load_nb<REC>(u, g.vol, k, ximk), load_nb<REC>(u, g.vol, i, xmk));
-        sd = mul(sd, load_nb<REC>(u, g.vol, k, xmk));
+        const mat3<R> ss = staple_pair(load_nb<REC>(u, g.vol, k, xi), load_nb<REC>(u, g.vol, i, xk),
+                                       load_nb<REC>(u, g.vol, k, p.s), load_nb<REC>(u, g.vol, k, ximk),
+                                       load_nb<REC>(u, g.vol, i, xmk), load_nb<REC>(u, g.vol, k, xmk));
 #pragma unroll
-        for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * (su.m[m] + sd.m[m]);
+        for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * ss.m[m];
     }
     return sum;
 }
@@ -325,12 +324,10 @@ __global__ __launch_bounds__(TX *TY * 3) void k_fused_tile(lf_batch<R> bt, geom 
             const int pk = k == 2 ? 1 : 0;                   // plane of x + k
             const int pmk = k == 2 ? -1 : 0;                 // plane of x - k
             const int pik = (i == 2 ? 1 : 0) + (k == 2 ? -1 : 0); // plane of x + i - k
-            mat3<R> su = mul_nd(lnk(pi, k, ix, iy), lnk(pk, i, kx, ky));
-            su = mul_nd(su, lnk(0, k, 0, 0));
-            mat3<R> sd = mul_dd(lnk(pik, k, ix - kx, iy - ky), lnk(pmk, i, -kx, -ky));
-            sd = mul(sd, lnk(pmk, k, -kx, -ky));
+            const mat3<R> ss = staple_pair(lnk(pi, k, ix, iy), lnk(pk, i, kx, ky), lnk(0, k, 0, 0),
+                                           lnk(pik, k, ix - kx, iy - ky), lnk(pmk, i, -kx, -ky), lnk(pmk, k, -kx, -ky));
 #pragma unroll
-            for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * (su.m[m] + sd.m[m]);
+            for (int m = 0; m < 9; ++m) sum.m[m] = sum.m[m] + coef * ss.m[m];
         }
         const int64_t s = (x0 + tx) + (int64_t)g.n0 * (y0 + ty) + plane * z;
         R e[8];
