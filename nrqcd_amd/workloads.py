@@ -9,6 +9,8 @@
     python -m nrqcd_amd.workloads full       # CN steps with c = (1,1,1,1) and EquilibriumInit at 64^3
     python -m nrqcd_amd.workloads cn         # CN steps/s per solver and stencil launch shape at 128^3,
                                              # FP32-stored and FP64 links (--shape nc64,nc32,tiling,xcd ...)
+    python -m nrqcd_amd.workloads leapfrog   # one leapfrog Update call at 128^3 after the 1 GiB copy probe, for
+                                             # counter runs (--precision 32|64, NRQ_LF_SHAPE pins a launch shape)
 
 Each prints one JSON object.  Inputs are synthetic (the seeded box initial
 condition of gaugeconfiguration_su3.f90:906-951); all timing uses HIP events
@@ -23,7 +25,7 @@ import time
 
 import numpy as np
 
-from . import (cn_get_shape, cn_set_shape, GaugeConfiguration, GetNorm2Momentum_G, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
+from . import (bandwidth_probe, cn_get_shape, cn_set_shape, GaugeConfiguration, leapfrog_timer_start, leapfrog_timer_stop, GetNorm2Momentum_G, NRQCDField, NRQ_PRECISION_FP32, NRQ_PRECISION_FP64, NRQ_SOLVER_CHEBYSHEV_FP64,
                NRQ_SOLVER_CHEBYSHEV_MIXED, NRQ_SOLVER_FGMRES_CNN, NRQ_SOLVER_GMRES_FP64, NRQ_SOLVER_GMRES_MIXED,
                PreconditionerNet, event_elapsed_ms, event_record, lattice_finalize, lattice_init, leapfrog_batch,
                nrqcd_finalize, nrqcd_init, nrqcd_set_preconditioner, nrqcd_set_solver, random_init, spectral_functions,
@@ -395,14 +397,38 @@ def cn(size: int = 128, evolve: int = 50, seed: int = 3, cn_steps: int = 5, shap
     return out
 
 
+def leapfrog(size: int = 128, precision: int = 32, steps: int = 10, seed: int = 3) -> dict:
+    """The seeded box configuration (Qs = 0.5) at this storage precision, the
+    launch shape chosen (or pinned by NRQ_LF_SHAPE), the 1 GiB copy kernel
+    (k_stream_copy, a calibration point for HBM counters) and one
+    `steps`-step Update call (steps - 1 fused launches) timed per fused launch."""
+    lattice_init((size,) * 3, SPACINGS)
+    try:
+        random_init(seed)
+        g = GaugeConfiguration(NRQ_PRECISION_FP32 if precision == 32 else NRQ_PRECISION_FP64)
+        g.TransversePolarisedOccupiedInit_Box(0.5, 1.0, 1.0)
+        shape = g.launch_shape()
+        copy_gbs = bandwidth_probe(1 << 30, 2)
+        synchronize()
+        leapfrog_timer_start(steps + 1)
+        g.Update(1.0, steps)
+        t = leapfrog_timer_stop()
+        del g
+    finally:
+        lattice_finalize()
+    return {"size": size, "precision": precision, "steps": steps, "launch_shape": shape, "copy_gbs": copy_gbs,
+            "fused_launches": t["launches"], "fused_us": 1e3 * t["ms"] / max(t["launches"], 1)}
+
+
 def main(argv=None) -> None:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
-    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon", "spectrum", "full", "cn"))
+    ap.add_argument("workload", choices=("mixed", "ensemble", "learned", "gluon", "spectrum", "full", "cn", "leapfrog"))
     ap.add_argument("--size", type=int)
     ap.add_argument("--steps", type=int)
     ap.add_argument("--members", type=int, default=32)
     ap.add_argument("--cn-steps", type=int)
     ap.add_argument("--shape", action="append", help="cn: stencil launch shape nc64,nc32,tiling,xcd (repeatable)")
+    ap.add_argument("--precision", type=int, default=32, choices=(32, 64), help="leapfrog: storage precision")
     a = ap.parse_args(argv)
     kw = {k: v for k, v in (("size", a.size), ("steps", a.steps), ("cn_steps", a.cn_steps)) if v is not None}
     if a.workload == "mixed":
@@ -413,6 +439,8 @@ def main(argv=None) -> None:
         out = gluon(**{k: v for k, v in kw.items() if k in ("size", "steps")})
     elif a.workload == "full":
         out = full(**{k: v for k, v in kw.items() if k in ("size", "steps")})
+    elif a.workload == "leapfrog":
+        out = leapfrog(precision=a.precision, **{k: v for k, v in kw.items() if k in ("size", "steps")})
     elif a.workload == "cn":
         shapes = [tuple(int(x) for x in sh.split(",")) for sh in a.shape] if a.shape else None
         out = cn(shapes=shapes, **{k: v for k, v in kw.items() if k in ("size", "cn_steps")})
