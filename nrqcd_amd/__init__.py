@@ -555,9 +555,10 @@ def leapfrog_get_shape() -> int:
     return k.value
 
 
-def cn_set_shape(nc64: int, nc32: int, tiling: int = 3, xcd_remap: int = 1) -> None:
+def cn_set_shape(nc64: int, nc32: int, tiling: int = 4, xcd_remap: int = 8) -> None:
     """Launch shape of the CN stencil kernels (fields per thread in FP64 and
-    FP32, site tiling 0-3 with 3 the wave-item map, XCD remap)."""
+    FP32, site tiling 0-4 with 3 the wave-item map and 4 its banded z-sweep
+    order, XCD remap)."""
     _check(library().nrq_cn_set_shape(nc64, nc32, tiling, xcd_remap))
 
 

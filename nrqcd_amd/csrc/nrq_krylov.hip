@@ -334,7 +334,7 @@ __global__ __launch_bounds__(SBS_WAVE, CN_WAVES) void k_cn_rhs(const cplx<RL> *_
             }
         }
     }
-    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial, mp.mode == 3);
+    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial, mp.mode >= 3);
 }
 
 // FP64 true residual r_r = y_r - (1 + beta_r K) x_r, stored to res as type
@@ -380,7 +380,7 @@ __global__ __launch_bounds__(SBS_WAVE, CN_WAVES) void k_cn_resid(const cplx<RL> 
             }
         }
     }
-    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial, mp.mode == 3);
+    chunk_partials<NC>(nrm, r0, nf, t, mp.nsb, partial, mp.mode >= 3);
 }
 
 // One Chebyshev step on the segment [1, 1 + beta_r kmax] of the spectrum of
@@ -750,7 +750,7 @@ struct workspace {
         const size_t f64 = (size_t)3 * (size_t)v * sizeof(cplx<double>) * rc;
         const size_t f32 = f64 / 2;
         int64_t sb = 0;
-        for (int md = 0; md <= 3; ++md) sb = std::max(sb, make_lf_map(lg, md == 3 ? SBS_WAVE : SBS, 8, md, 1).nsb);
+        for (int md = 0; md <= 4; ++md) sb = std::max(sb, make_lf_map(lg, md >= 3 ? SBS_WAVE : SBS, 8, md, 1).nsb);
         auto dm = [](void **p, size_t b) { return hipMalloc(p, b) == hipSuccess; };
         bool ok = dm(&basis, f64 * (size_t)(mm + 1)) && dm((void **)&y, f64) && dm((void **)&x, f64) &&
                   dm((void **)&t, f64) && dm((void **)&res, f64) && dm((void **)&y32, f32) &&
@@ -826,8 +826,8 @@ static int upload_active(const std::vector<int> &act) { return upload_ints(g_ws.
 // same sums in the same order per slot.
 struct cn_shape {
     // measured fastest at 128^3 with 12 fields: one work item per wavefront,
-    // no XCD remap
-    int nc64 = 2, nc32 = 4, mode = 3, xcd = 1;
+    // y-band z-sweep order, each XCD walking whole bands
+    int nc64 = 2, nc32 = 4, mode = 4, xcd = 8;
 };
 static bool cn_nc_ok(int v) { return v == 1 || v == 2 || v == 4; }
 static cn_shape read_shape()
@@ -838,7 +838,7 @@ static cn_shape read_shape()
         if (sscanf(e, "%d,%d,%d,%d", &a, &b, &c, &d) == 4) {
             if (cn_nc_ok(a)) sh.nc64 = a;
             if (cn_nc_ok(b)) sh.nc32 = b;
-            if (c >= 0 && c <= 3) sh.mode = c;
+            if (c >= 0 && c <= 4) sh.mode = c;
             if (d == 1 || d == 8) sh.xcd = d;
         }
     }
@@ -855,7 +855,7 @@ template <typename T> static int nc_of() { return sizeof(T) == 8 ? shape().nc64 
 // stencil launch map for nf slots, nc per thread
 static lf_map cn_map(int nf, int nc)
 {
-    return make_lf_map(lat().g, shape().mode == 3 ? SBS_WAVE : SBS, shape().xcd, shape().mode, (nf + nc - 1) / nc);
+    return make_lf_map(lat().g, shape().mode >= 3 ? SBS_WAVE : SBS, shape().xcd, shape().mode, (nf + nc - 1) / nc);
 }
 
 // Runs the statements after `nc` with the compile-time chunk width NCV equal
@@ -1648,7 +1648,7 @@ using namespace nrq;
 extern "C" int nrq_cn_set_shape(int nc64, int nc32, int tiling, int xcd_remap)
 {
     if (!cn_nc_ok(nc64) || !cn_nc_ok(nc32)) return fail(NRQ_ERR_ARG, "CN shape: slots per thread must be 1, 2 or 4");
-    if (tiling < 0 || tiling > 3) return fail(NRQ_ERR_ARG, "CN shape: tiling %d out of range", tiling);
+    if (tiling < 0 || tiling > 4) return fail(NRQ_ERR_ARG, "CN shape: tiling %d out of range", tiling);
     if (xcd_remap != 1 && xcd_remap != 8) return fail(NRQ_ERR_ARG, "CN shape: XCD remap must be 1 or 8");
     cn_shape &sh = shape_mut();
     sh.nc64 = nc64;

@@ -17,6 +17,14 @@
 // sites (up to wpb items per block, more blocks per site block beyond that).
 // The items of a site read the same links, and in one block those loads are
 // served by the CU's L1 instead of L2.
+//
+// Mode 4 (banded wave items): as mode 3, but a site block is one 64-site
+// segment of an x-row and the segments are ordered by (y-band of ty rows, z,
+// row, segment).  With the XCD remap each XCD walks whole bands, one z-plane
+// of a band after the other: the y neighbours of a segment are in the same
+// band-plane (except at the band's edges) and its z neighbours in the
+// band-planes just before and after it, so the neighbour values a block reads
+// were fetched into the same L2 a few band-planes earlier.
 #pragma once
 #include "nrq_su3.h"
 
@@ -26,18 +34,19 @@ struct lf_map {
     int64_t nsb; // site blocks (mode 0) or tiles (mode 1)
     int bs;      // threads per block
     int xcd;     // XCD remap factor (1: none)
-    int mode;    // 0: bs consecutive sites; 1: z-sweep tiles of full x-rows; 2: z-sweep tx x ty tiles
-    int ty;      // modes 1, 2: y rows per tile
+    int mode;    // 0: bs consecutive sites; 1: z-sweep tiles of full x-rows; 2: z-sweep tx x ty tiles;
+                 // 3: wave items over 64 consecutive sites; 4: wave items over banded x-row segments
+    int ty;      // modes 1, 2: y rows per tile; mode 4: y rows per band
     int ngroup;  // work items per site block
     int tx;      // mode 2: x extent of a tile
-    int nxb;     // mode 2: tiles along x
-    int wpb;     // mode 3: work items (wavefronts) per block
+    int nxb;     // mode 2: tiles along x; mode 4: 64-site segments per x-row
+    int wpb;     // modes 3, 4: work items (wavefronts) per block
 };
 
 // Blocks of a launch.
 __host__ __device__ __forceinline__ unsigned lf_blocks(const lf_map &m)
 {
-    if (m.mode == 3) return (unsigned)m.nsb * (unsigned)((m.ngroup + m.wpb - 1) / m.wpb);
+    if (m.mode >= 3) return (unsigned)m.nsb * (unsigned)((m.ngroup + m.wpb - 1) / m.wpb);
     return (unsigned)m.ngroup * (unsigned)m.nsb;
 }
 
@@ -64,14 +73,25 @@ __device__ __forceinline__ unsigned lf_logical_block(const lf_map &m)
 __device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p, int &item, int64_t &t)
 {
     const unsigned lb = lf_logical_block(m);
-    if (m.mode == 3) {
+    if (m.mode >= 3) {
         const unsigned nbg = (unsigned)((m.ngroup + m.wpb - 1) / m.wpb); // blocks per site block
         const unsigned tt = lb / nbg;
         item = (int)(lb - tt * nbg) * m.wpb + (int)(threadIdx.x >> 6);
         t = tt;
-        const int64_t s = (int64_t)tt * 64 + (threadIdx.x & 63);
-        if (s >= g.vol || item >= m.ngroup) return false;
-        p = make_site(g, s);
+        if (m.mode == 3) {
+            const int64_t s = (int64_t)tt * 64 + (threadIdx.x & 63);
+            if (s >= g.vol || item >= m.ngroup) return false;
+            p = make_site(g, s);
+            return true;
+        }
+        const unsigned per = (unsigned)m.ty * (unsigned)m.nxb; // segments per band-plane
+        const unsigned pb = per * (unsigned)g.n2;              // segments per band
+        const unsigned band = tt / pb, r1 = tt - band * pb;
+        const unsigned z = r1 / per, r2 = r1 - z * per;
+        const unsigned row = r2 / (unsigned)m.nxb, seg = r2 - row * (unsigned)m.nxb;
+        const int x = (int)seg * 64 + (int)(threadIdx.x & 63), y = (int)(band * (unsigned)m.ty + row);
+        if (x >= g.n0 || item >= m.ngroup) return false;
+        p = lsite{x + (int64_t)g.n0 * (y + (int64_t)g.n1 * (int)z), x, y, (int)z};
         return true;
     }
     const unsigned ng = (unsigned)m.ngroup;
@@ -108,8 +128,9 @@ __device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p
 }
 
 // Host: map for `bs` threads per block.  Mode 1 needs n0 <= bs; mode 2
-// uses 16-wide tiles (bs a multiple of 16); mode 3 takes up to bs / 64 work
-// items per block (bs is then 64 times the items per block); otherwise
+// uses 16-wide tiles (bs a multiple of 16); modes 3 and 4 take up to bs / 64
+// work items per block (bs is then 64 times the items per block), mode 4
+// with bands of the largest of 8, 4, 2, 1 rows that divides n1; otherwise
 // consecutive sites.
 inline lf_map make_lf_map(const geom &g, int bs, int xcd, int mode, int ngroup)
 {
@@ -117,11 +138,17 @@ inline lf_map make_lf_map(const geom &g, int bs, int xcd, int mode, int ngroup)
     m.bs = bs;
     m.xcd = xcd;
     m.ngroup = ngroup;
-    if (mode == 3 && bs >= 64) {
-        m.mode = 3;
+    if ((mode == 3 || mode == 4) && bs >= 64) {
+        m.mode = mode;
         m.wpb = ngroup < bs / 64 ? ngroup : bs / 64;
         m.bs = 64 * m.wpb;
-        m.nsb = (g.vol + 63) / 64;
+        if (mode == 3) {
+            m.nsb = (g.vol + 63) / 64;
+        } else {
+            m.ty = g.n1 % 8 == 0 ? 8 : g.n1 % 4 == 0 ? 4 : g.n1 % 2 == 0 ? 2 : 1;
+            m.nxb = (g.n0 + 63) / 64;
+            m.nsb = (int64_t)m.nxb * g.n1 * g.n2;
+        }
     } else if (mode == 1 && g.n0 <= bs) {
         m.mode = 1;
         m.ty = bs / g.n0 < g.n1 ? bs / g.n0 : g.n1;

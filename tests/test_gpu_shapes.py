@@ -117,9 +117,9 @@ def test_autotuned_shape_matches_oracle(nrq, oracle, lattice, pinned):
     _say(f"tuned shape: {g.launch_shape()}")
 
 
-CN_SHAPES = ([(nc, 4, t, x, nrq_kind) for nc in (1, 2, 4) for t in (0, 1, 2, 3) for x in (1, 8)
+CN_SHAPES = ([(nc, 4, t, x, nrq_kind) for nc in (1, 2, 4) for t in (0, 1, 2, 3, 4) for x in (1, 8)
               for nrq_kind in ("GMRES_FP64",)]
-             + [(2, nc, t, 8, "CHEBYSHEV_MIXED") for nc in (1, 2, 4) for t in (0, 1, 2, 3)])
+             + [(2, nc, t, 8, "CHEBYSHEV_MIXED") for nc in (1, 2, 4) for t in (0, 1, 2, 3, 4)])
 
 
 def test_every_cn_shape_matches_oracle(nrq, oracle, lattice):
