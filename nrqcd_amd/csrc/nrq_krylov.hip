@@ -113,10 +113,13 @@ template <int NC, typename T, typename S, bool NT = false> struct ld_fields {
 // field, and the caller drops their results).  The kernels are bound by
 // memory latency, so each of the six hops issues its 9 link loads and all
 // 3 NC field loads at once, with no branch between them: one round trip per
-// hop.  In FP32 arithmetic on FP32 links (the correction solves, whose
-// results the FP64 residual checks) only rows 1 and 2 of each link are read
-// and row 3 is rebuilt as conj(row1 x row2), exact for SU(3) up to rounding:
-// a third fewer link bytes, which are half of these kernels' memory time.
+// hop.  Only rows 1 and 2 of each link are read and row 3 is rebuilt as
+// conj(row1 x row2), exact for SU(3) up to the links' rounding, a third fewer
+// link loads, which are half of these kernels' memory time: for FP64 links
+// (rebuilt to FP64 rounding) and in FP32 arithmetic on FP32 links (the
+// correction solves, whose results the FP64 residual checks).  FP64
+// arithmetic on FP32-stored links reads whole links, so that the FP64
+// residual is that of the stored operator and not of its SU(3) projection.
 template <int NC, typename T, typename RL, typename LD>
 __device__ __forceinline__ void hop_chunk(const cplx<RL> *__restrict__ u, const LD &ld, const lsite &p, const geom &g,
                                           cplx<T> (&hop)[NC][3])
@@ -135,7 +138,7 @@ __device__ __forceinline__ void hop_chunk(const cplx<RL> *__restrict__ u, const 
         // forward: U_i(s) and f(s+i); backward: U_i(s-i) and f(s-i)
         const int64_t sl = fwd ? p.s : site_dn(g, p, i).s;
         const int64_t sf = fwd ? site_up(g, p, i).s : sl;
-        constexpr bool REC = sizeof(T) == 4 && sizeof(RL) == 4;
+        constexpr bool REC = sizeof(RL) == 8 || sizeof(T) == 4;
         constexpr int NE = REC ? 6 : 9;
         cplx<RL> lk[9];
         cplx<T> f[NC][3];
