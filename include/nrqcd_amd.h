@@ -307,9 +307,10 @@ int nrq_cn_set_solver(int kind, int restart, int64_t maxit);
  * 4), the site-block tiling (0: consecutive sites, 1: z-sweep tiles of
  * x-rows, 2: z-sweep 16-wide tiles, 3: wave items, 64 consecutive sites per
  * block with one field chunk per wavefront, so the chunks of a site share
- * their link loads in the CU's L1; 4: wave items over 64-site x-row
- * segments ordered by y-band, z and row, so that with the XCD remap each XCD
- * sweeps whole bands through z) and the XCD remap (1 or 8).  Every shape
+ * their link loads in the CU's L1; 4: wave items over 64-site blocks of
+ * band-planes, runs of up to 8 x-rows at one z, ordered by band and z, so
+ * that with the XCD remap each XCD sweeps whole bands through z) and the XCD
+ * remap (1 or 8).  Every shape
  * computes the same per-field sums in the same order.  NRQ_CN_SHAPE=
  * "nc64,nc32,tiling,xcd" in the environment sets the initial value (default
  * 2,4,4,8).  No reference counterpart. */

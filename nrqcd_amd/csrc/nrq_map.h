@@ -18,11 +18,12 @@
 // The items of a site read the same links, and in one block those loads are
 // served by the CU's L1 instead of L2.
 //
-// Mode 4 (banded wave items): as mode 3, but a site block is one 64-site
-// segment of an x-row and the segments are ordered by (y-band of ty rows, z,
-// row, segment).  With the XCD remap each XCD walks whole bands, one z-plane
-// of a band after the other: the y neighbours of a segment are in the same
-// band-plane (except at the band's edges) and its z neighbours in the
+// Mode 4 (banded wave items): as mode 3, but the sites are ordered by (y-band
+// of ty rows, z, site within the band-plane): a band-plane, ty x-rows at one
+// z, is a contiguous run of ty * n0 sites, cut into 64-site blocks (only the
+// last may be partial).  With the XCD remap each XCD walks whole bands, one
+// z-plane of a band after the other: the y neighbours of a site are in the
+// same band-plane (except at the band's edges) and its z neighbours in the
 // band-planes just before and after it, so the neighbour values a block reads
 // were fetched into the same L2 a few band-planes earlier.
 #pragma once
@@ -39,7 +40,7 @@ struct lf_map {
     int ty;      // modes 1, 2: y rows per tile; mode 4: y rows per band
     int ngroup;  // work items per site block
     int tx;      // mode 2: x extent of a tile
-    int nxb;     // mode 2: tiles along x; mode 4: 64-site segments per x-row
+    int nxb;     // mode 2: tiles along x; mode 4: 64-site blocks per band-plane
     int wpb;     // modes 3, 4: work items (wavefronts) per block
 };
 
@@ -84,13 +85,14 @@ __device__ __forceinline__ bool lf_site(const lf_map &m, const geom &g, lsite &p
             p = make_site(g, s);
             return true;
         }
-        const unsigned per = (unsigned)m.ty * (unsigned)m.nxb; // segments per band-plane
-        const unsigned pb = per * (unsigned)g.n2;              // segments per band
+        const unsigned per = (unsigned)m.nxb;     // blocks per band-plane
+        const unsigned pb = per * (unsigned)g.n2; // blocks per band
         const unsigned band = tt / pb, r1 = tt - band * pb;
-        const unsigned z = r1 / per, r2 = r1 - z * per;
-        const unsigned row = r2 / (unsigned)m.nxb, seg = r2 - row * (unsigned)m.nxb;
-        const int x = (int)seg * 64 + (int)(threadIdx.x & 63), y = (int)(band * (unsigned)m.ty + row);
-        if (x >= g.n0 || item >= m.ngroup) return false;
+        const unsigned z = r1 / per, seg = r1 - z * per;
+        const unsigned o = seg * 64u + (threadIdx.x & 63u); // site within the band-plane
+        if (o >= (unsigned)m.ty * (unsigned)g.n0 || item >= m.ngroup) return false;
+        const unsigned row = o / (unsigned)g.n0;
+        const int x = (int)(o - row * (unsigned)g.n0), y = (int)(band * (unsigned)m.ty + row);
         p = lsite{x + (int64_t)g.n0 * (y + (int64_t)g.n1 * (int)z), x, y, (int)z};
         return true;
     }
@@ -146,8 +148,8 @@ inline lf_map make_lf_map(const geom &g, int bs, int xcd, int mode, int ngroup)
             m.nsb = (g.vol + 63) / 64;
         } else {
             m.ty = g.n1 % 8 == 0 ? 8 : g.n1 % 4 == 0 ? 4 : g.n1 % 2 == 0 ? 2 : 1;
-            m.nxb = (g.n0 + 63) / 64;
-            m.nsb = (int64_t)m.nxb * g.n1 * g.n2;
+            m.nxb = (m.ty * g.n0 + 63) / 64;
+            m.nsb = (int64_t)m.nxb * (g.n1 / m.ty) * g.n2;
         }
     } else if (mode == 1 && g.n0 <= bs) {
         m.mode = 1;
