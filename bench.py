@@ -58,11 +58,11 @@ FUSED_BYTES = {64: 1248.0, 32: 624.0}
 # Algorithmic HBM bytes per site of one mixed-Chebyshev CN step with 12 live
 # fields when every column takes the planned 6 Chebyshev steps (two FP32
 # corrections of 4 and 2 steps), FP64 links: right-hand side with y, x0 (FP64)
-# and K psi (FP32) 2448; FP32 copy of two link rows 432; K pass 720; Chebyshev
+# and K psi (FP32) 2304; FP32 copy of two link rows 432; K pass 720; Chebyshev
 # launches 720, 1008, 2160 (adds the correction to x0 in FP64) and 1584 (adds
-# the second to psi); two FP64 true-residual passes 2 x 1872.  The FP32 passes
-# read two rows of each link (144 B per site).
-CN_MIXED_CHEB_BYTES = 2448.0 + 432 + 720 + 720 + 1008 + 2160 + 1584 + 2 * 1872
+# the second to psi); two FP64 true-residual passes 2 x 1728.  Every pass reads
+# two rows of each link (288 B per site in FP64, 144 B in FP32).
+CN_MIXED_CHEB_BYTES = 2304.0 + 432 + 720 + 720 + 1008 + 2160 + 1584 + 2 * 1728
 CN_MIXED_CHEB_ITS = 6.0
 TRAFFIC_FILE = ROOT / "profiles" / "leapfrog_fused_128cubed_traffic.txt"
 # one network apply: 2 flop per multiply-add, 27 taps, 18 -> 32 -> 32 -> 32 -> 18 channels, per site
