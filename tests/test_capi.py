@@ -62,3 +62,40 @@ def test_fortran_driver_built():
         pytest.skip("Fortran driver not built (amdflang missing)")
     r = subprocess.run([str(exe)], capture_output=True, text=True)
     assert r.returncode == 2 and "usage" in r.stdout
+
+
+def test_cn_shape_setter_validates_and_round_trips(nrq):
+    """nrq_cn_set_shape / nrq_cn_get_shape are host-only: the default is 2
+    FP64 and 4 FP32 fields per thread, the banded wave-item tiling (4) and the
+    XCD remap 8; out-of-range values are rejected and leave the shape as it
+    was."""
+    default = nrq.cn_get_shape()
+    assert default == (2, 4, 4, 8)
+    try:
+        for shape in ((1, 2, 0, 1), (4, 1, 3, 8), (2, 4, 4, 1)):
+            nrq.cn_set_shape(*shape)
+            assert nrq.cn_get_shape() == shape
+        for bad in ((3, 4, 4, 8), (2, 8, 4, 8), (2, 4, 5, 8), (2, 4, -1, 8), (2, 4, 4, 2)):
+            with pytest.raises(nrq.NRQError):
+                nrq.cn_set_shape(*bad)
+            assert nrq.cn_get_shape() == (2, 4, 4, 1)
+    finally:
+        nrq.cn_set_shape(*default)
+
+
+def test_leapfrog_shape_table(nrq, lattice):
+    """The autotuner's shape table: the default shape applies everywhere; the
+    LDS tiles only to FP32 storage on extents that are multiples of 16, 8 and
+    their run of z planes; pinning a shape outside the table is rejected."""
+    lattice((32, 16, 24), (0.05, 1, 1, 1))
+    shapes = nrq.leapfrog_shapes()
+    assert len(shapes) == 19
+    assert all(s["applies_fp32"] for s in shapes if not s["tiling"].startswith("LDS"))
+    tiles = [s for s in shapes if s["tiling"].startswith("LDS")]
+    assert tiles and not any(s["applies"] for s in tiles)
+    assert all(s["applies_fp32"] == s["tiling"].endswith("8-plane runs") for s in tiles)
+    lattice((16, 6, 5), (0.05, 1, 1, 1))
+    assert not any(s["applies_fp32"] for s in nrq.leapfrog_shapes() if s["tiling"].startswith("LDS"))
+    with pytest.raises(nrq.NRQError):
+        nrq.leapfrog_set_shape(len(shapes))
+    nrq.leapfrog_set_shape(-1)
