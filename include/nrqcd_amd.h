@@ -215,8 +215,12 @@ int nrq_gauge_plaquette(const nrq_gauge *h, int i, int j, int64_t idx, double P[
  * FP64 links and never reunitarises; used by the FP32 storage mode. */
 int nrq_gauge_reunitarise(nrq_gauge *h);
 /* Makes nrq_gauge_leapfrog reunitarise (as above) after every `interval`
- * steps, counted across calls (0: never, the default).  No reference
- * counterpart: it keeps FP32 link storage on the group over long runs. */
+ * steps, counted across calls (0: never, the default).  In a forward call a
+ * point inside the call is done by the fused pass that follows it (the links
+ * are projected after the E update has used them and before the next link
+ * update, the same result as a separate pass there); a point at the call's
+ * end by a separate pass.  No reference counterpart: it keeps FP32 link
+ * storage on the group over long runs. */
 int nrq_gauge_set_reunitarise_interval(nrq_gauge *h, int64_t interval);
 /* Launch shape the leapfrog autotuner chose for this configuration's
  * precision and lattice size: threads per block, XCD remap factor (1 = none),

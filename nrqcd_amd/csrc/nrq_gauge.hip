@@ -98,6 +98,37 @@ __device__ __forceinline__ mat3<R> evolve_link(const mat3<R> &u, const R *e, R d
     return mul(x, u);
 }
 
+// Projects a link onto SU(3) in FP64: normalise row 1, orthonormalise row 2
+// against it, row 3 = conj(row1 x row2); the result rounded to R.
+template <typename R> __device__ __forceinline__ void project_su3(mat3<R> &m)
+{
+    cplx<double> r0[3], r1[3];
+    for (int c = 0; c < 3; ++c) {
+        r0[c] = {(double)m.m[c].re, (double)m.m[c].im};
+        r1[c] = {(double)m.m[3 + c].re, (double)m.m[3 + c].im};
+    }
+    double n0 = 0;
+    for (int c = 0; c < 3; ++c) n0 += r0[c].re * r0[c].re + r0[c].im * r0[c].im;
+    n0 = 1.0 / sqrt(n0);
+    for (int c = 0; c < 3; ++c) r0[c] = n0 * r0[c];
+    cplx<double> pr = {0, 0};
+    for (int c = 0; c < 3; ++c) pr = pr + cmul(r0[c], r1[c]);
+    for (int c = 0; c < 3; ++c) r1[c] = r1[c] - pr * r0[c];
+    double n1 = 0;
+    for (int c = 0; c < 3; ++c) n1 += r1[c].re * r1[c].re + r1[c].im * r1[c].im;
+    n1 = 1.0 / sqrt(n1);
+    for (int c = 0; c < 3; ++c) r1[c] = n1 * r1[c];
+    cplx<double> r2[3];
+    r2[0] = conjg(r0[1] * r1[2] - r0[2] * r1[1]);
+    r2[1] = conjg(r0[2] * r1[0] - r0[0] * r1[2]);
+    r2[2] = conjg(r0[0] * r1[1] - r0[1] * r1[0]);
+    for (int c = 0; c < 3; ++c) {
+        m.m[c] = {(R)r0[c].re, (R)r0[c].im};
+        m.m[3 + c] = {(R)r1[c].re, (R)r1[c].im};
+        m.m[6 + c] = {(R)r2[c].re, (R)r2[c].im};
+    }
+}
+
 // Update_Efield_Leapfrog_atSite_Direction, gaugeconfiguration_su3.f90:
 // 2164-2198: E^a_i(x) -= 2 Im Tr[T^a U_i(x) sum_{k!=i} dt (a0/a_k)^2 (S^U+S^D)]
 // with S^U = U_k(x+i) U_i(x+k)^+ U_k(x)^+ and
@@ -210,8 +241,11 @@ __global__ __launch_bounds__(BS) void k_efield(lf_batch<R> bt, geom g, R dt, lf_
 // per-(site, dir) operations of the reference; fusing them only removes one
 // HBM round trip of E and U.  Neighbour links are read from uin, new links
 // go to uout (ping-pong), E is updated in place (each thread owns its E).
+// With reu the link is projected onto SU(3) (project_su3) after the E update
+// has used it and before the link update: the same result as a separate
+// reunitarisation pass between the two halves.
 template <typename R, int BS, bool NT, bool REC>
-__global__ __launch_bounds__(BS) void k_fused(lf_batch<R> bt, geom g, R dt, lf_map m)
+__global__ __launch_bounds__(BS) void k_fused(lf_batch<R> bt, geom g, R dt, lf_map m, int reu)
 {
     const cplx<R> *__restrict__ uin = bt.uin[blockIdx.y];
     cplx<R> *__restrict__ uout = bt.uout[blockIdx.y];
@@ -225,6 +259,7 @@ __global__ __launch_bounds__(BS) void k_fused(lf_batch<R> bt, geom g, R dt, lf_m
     efield_update<REC, NT>(uin, ef, g, i, p, dt, e, ui);
 #pragma unroll
     for (int a = 0; a < 8; ++a) st_e<NT>(ef + ((int64_t)i * 8 + a) * g.vol + s, e[a]);
+    if (reu) project_su3(ui);
     store_link_out<NT>(uout, g.vol, i, s, evolve_link(ui, e, dt));
 }
 
@@ -238,10 +273,10 @@ __global__ __launch_bounds__(BS) void k_fused(lf_batch<R> bt, geom g, R dt, lf_m
 // plane streams into registers while plane z is computed and replaces plane
 // z - 1 afterwards, so each link is read from HBM once per run of ZC planes
 // (plus the x/y halo, from L2, and two planes per run).  Per (site, dir) the arithmetic is that
-// of k_fused with full neighbour links.  The lattice extents must be
-// multiples of TX, TY and ZC.
+// of k_fused with full neighbour links (reu as there).  The lattice extents
+// must be multiples of TX, TY and ZC.
 template <typename R, int TX, int TY, int ZC, bool NT>
-__global__ __launch_bounds__(TX *TY * 3) void k_fused_tile(lf_batch<R> bt, geom g, R dt, int xcd)
+__global__ __launch_bounds__(TX *TY * 3) void k_fused_tile(lf_batch<R> bt, geom g, R dt, int xcd, int reu)
 {
     constexpr int HX = TX + 2, HY = TY + 2, HS = HX * HY;
     constexpr int NTH = TX * TY * 3;
@@ -333,10 +368,11 @@ __global__ __launch_bounds__(TX *TY * 3) void k_fused_tile(lf_batch<R> bt, geom 
         R e[8];
 #pragma unroll
         for (int a = 0; a < 8; ++a) e[a] = ld_e<NT>(ef + ((int64_t)i * 8 + a) * vol + s);
-        const mat3<R> ui = lnk(0, i, 0, 0);
+        mat3<R> ui = lnk(0, i, 0, 0);
         apply_force(ui, sum, e);
 #pragma unroll
         for (int a = 0; a < 8; ++a) st_e<NT>(ef + ((int64_t)i * 8 + a) * vol + s, e[a]);
+        if (reu) project_su3(ui);
         store_link_out<NT>(uout, vol, i, s, evolve_link(ui, e, dt));
         __syncthreads(); // the oldest plane is no longer read
         if (more) sstore(kz % 3);
@@ -506,31 +542,7 @@ template <typename R> __global__ void k_reunit(cplx<R> *u, int64_t vol)
     const int i = blockIdx.y;
     if (s >= vol) return;
     mat3<R> m = load_link(u, vol, i, s);
-    cplx<double> r0[3], r1[3];
-    for (int c = 0; c < 3; ++c) {
-        r0[c] = {(double)m.m[c].re, (double)m.m[c].im};
-        r1[c] = {(double)m.m[3 + c].re, (double)m.m[3 + c].im};
-    }
-    double n0 = 0;
-    for (int c = 0; c < 3; ++c) n0 += r0[c].re * r0[c].re + r0[c].im * r0[c].im;
-    n0 = 1.0 / sqrt(n0);
-    for (int c = 0; c < 3; ++c) r0[c] = n0 * r0[c];
-    cplx<double> pr = {0, 0};
-    for (int c = 0; c < 3; ++c) pr = pr + cmul(r0[c], r1[c]);
-    for (int c = 0; c < 3; ++c) r1[c] = r1[c] - pr * r0[c];
-    double n1 = 0;
-    for (int c = 0; c < 3; ++c) n1 += r1[c].re * r1[c].re + r1[c].im * r1[c].im;
-    n1 = 1.0 / sqrt(n1);
-    for (int c = 0; c < 3; ++c) r1[c] = n1 * r1[c];
-    cplx<double> r2[3];
-    r2[0] = conjg(r0[1] * r1[2] - r0[2] * r1[1]);
-    r2[1] = conjg(r0[2] * r1[0] - r0[0] * r1[2]);
-    r2[2] = conjg(r0[0] * r1[1] - r0[1] * r1[0]);
-    for (int c = 0; c < 3; ++c) {
-        m.m[c] = {(R)r0[c].re, (R)r0[c].im};
-        m.m[3 + c] = {(R)r1[c].re, (R)r1[c].im};
-        m.m[6 + c] = {(R)r2[c].re, (R)r2[c].im};
-    }
+    project_su3(m);
     store_link(u, vol, i, s, m);
 }
 
@@ -722,12 +734,12 @@ static lf_map make_map(int variant, const geom &g)
 }
 
 template <typename R, int ZC, bool NTV>
-static void launch_tile(const lf_batch<R> &bt, int nb, const geom &g, R dt, int xcd)
+static void launch_tile(const lf_batch<R> &bt, int nb, const geom &g, R dt, int xcd, int reu)
 {
     constexpr int TX = 16, TY = 8;
     const unsigned total = (unsigned)(g.n0 / TX) * (unsigned)(g.n1 / TY) * (unsigned)(g.n2 / ZC);
     hipLaunchKernelGGL((k_fused_tile<R, TX, TY, ZC, NTV>), dim3(total, (unsigned)nb), dim3(TX * TY * 3), 0, stream(), bt,
-                       g, dt, xcd);
+                       g, dt, xcd, reu);
 }
 
 template <typename R> static void launch_links(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
@@ -752,22 +764,24 @@ template <typename R> static void launch_efield(int variant, const lf_batch<R> &
     }
 }
 
-template <typename R> static void launch_fused(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt)
+// reu: project each link onto SU(3) between the E and the link update.
+template <typename R>
+static void launch_fused(int variant, const lf_batch<R> &bt, int nb, const geom &g, R dt, int reu = 0)
 {
     const lf_variant tv = LF_VARIANTS[variant];
     if constexpr (sizeof(R) == 4) if (tv.mode >= 3) {
         if (tv.mode == 3)
-            tv.nt ? launch_tile<R, 16, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 16, false>(bt, nb, g, dt, tv.xcd);
+            tv.nt ? launch_tile<R, 16, true>(bt, nb, g, dt, tv.xcd, reu) : launch_tile<R, 16, false>(bt, nb, g, dt, tv.xcd, reu);
         else
-            tv.nt ? launch_tile<R, 8, true>(bt, nb, g, dt, tv.xcd) : launch_tile<R, 8, false>(bt, nb, g, dt, tv.xcd);
+            tv.nt ? launch_tile<R, 8, true>(bt, nb, g, dt, tv.xcd, reu) : launch_tile<R, 8, false>(bt, nb, g, dt, tv.xcd, reu);
         return;
     }
     const lf_map m = make_map(variant, g);
     const dim3 grd(lf_blocks(m), (unsigned)nb);
     switch (m.bs) {
-    case 64: LF_NT(hipLaunchKernelGGL((k_fused<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), bt, g, dt, m)); break;
-    case 128: LF_NT(hipLaunchKernelGGL((k_fused<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), bt, g, dt, m)); break;
-    default: LF_NT(hipLaunchKernelGGL((k_fused<R, 256, NTV, RECV>), grd, dim3(256), 0, stream(), bt, g, dt, m)); break;
+    case 64: LF_NT(hipLaunchKernelGGL((k_fused<R, 64, NTV, RECV>), grd, dim3(64), 0, stream(), bt, g, dt, m, reu)); break;
+    case 128: LF_NT(hipLaunchKernelGGL((k_fused<R, 128, NTV, RECV>), grd, dim3(128), 0, stream(), bt, g, dt, m, reu)); break;
+    default: LF_NT(hipLaunchKernelGGL((k_fused<R, 256, NTV, RECV>), grd, dim3(256), 0, stream(), bt, g, dt, m, reu)); break;
     }
 }
 
@@ -980,7 +994,12 @@ struct lf_timer {
 };
 static lf_timer g_lf_timer;
 
-template <typename R> static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, double sw, int64_t n, bool graph)
+// n leapfrog steps of nb configurations.  Forward steps only: with
+// reu_every > 0 the links after steps reu_first, reu_first + reu_every, ...
+// (< n) are projected onto SU(3) inside the fused pass that follows them.
+template <typename R>
+static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, double sw, int64_t n, bool graph, int64_t reu_first = 0,
+                            int64_t reu_every = 0)
 {
     const geom g = lat().g;
     int v = 0;
@@ -1010,7 +1029,10 @@ template <typename R> static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, 
         if (sw > 0) launch_links<R>(v, table(c, false), nb, g, dt);
         if (rec) (void)hipEventRecord(tm.ev[2 * tm.used], stream());
         for (int64_t k = 0; k < nfused; ++k) {
-            launch_fused<R>(v, table(c, true), nb, g, dt);
+            // forward: this pass updates E of step k + 1, then the links of step k + 2
+            const int64_t st = k + 1;
+            const int reu = sw > 0 && reu_every > 0 && st >= reu_first && (st - reu_first) % reu_every == 0;
+            launch_fused<R>(v, table(c, true), nb, g, dt, reu);
             for (int &x : c) x ^= 1;
         }
         if (rec) {
@@ -1067,10 +1089,10 @@ template <typename R> static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, 
     return NRQ_OK;
 }
 
-template <typename R> static int leapfrog_t(nrq_gauge *h, double sw, int64_t n)
+template <typename R> static int leapfrog_t(nrq_gauge *h, double sw, int64_t n, int64_t reu_first = 0, int64_t reu_every = 0)
 {
     nrq_gauge *hs[1] = {h};
-    return leapfrog_batch_t<R>(hs, 1, sw, n, false);
+    return leapfrog_batch_t<R>(hs, 1, sw, n, false, reu_first, reu_every);
 }
 
 // Drops the captured leapfrog graphs (their launches carry the lattice by
@@ -1287,6 +1309,27 @@ extern "C" int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps
 {
     NEED_GAUGE(h);
     if (nsteps < 0 || !std::isfinite(stepwidth)) return fail(NRQ_ERR_ARG, "leapfrog: bad step count or width");
+    if (stepwidth > 0 && h->reunit_interval > 0 && nsteps > 0) {
+        // forward steps: the reunitarisation points inside the call are done by
+        // the fused passes that follow them (the links after step k are
+        // projected between the E update of step k and the link update of step
+        // k + 1, as a separate pass there would); one at the call's end by a
+        // separate pass.  Points after 1, 1 + interval, ... steps when the count
+        // has already reached the interval.
+        const int64_t iv = h->reunit_interval;
+        const int64_t first = std::max<int64_t>(1, iv - h->steps_since_reunit);
+        if (int r = h->precision == 64 ? leapfrog_t<double>(h, stepwidth, nsteps, first, iv)
+                                       : leapfrog_t<float>(h, stepwidth, nsteps, first, iv))
+            return r;
+        if (nsteps < first) {
+            h->steps_since_reunit += nsteps;
+            return NRQ_OK;
+        }
+        const int64_t last = first + (nsteps - first) / iv * iv; // last point <= nsteps
+        h->steps_since_reunit = nsteps - last;
+        if (last == nsteps) return nrq_gauge_reunitarise(h);
+        return NRQ_OK;
+    }
     while (nsteps > 0) {
         // with a reunitarisation interval, stop at each multiple of it
         int64_t n = nsteps;
