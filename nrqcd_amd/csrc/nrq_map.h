@@ -1,9 +1,9 @@
 // nrq_map.h -- block-to-work mapping shared by the lattice kernels.
 //
-// A launch is a 1-D grid of ngroup * nsb blocks: nsb site blocks times
-// `ngroup` work items per site block (the three link directions of the
-// leapfrog, the field chunks of the CN stencil), the items of one site block
-// adjacent.  Blocks are dealt round-robin to the XCDs, each with its own L2;
+// A launch is a 1-D grid of ngroup * nsb blocks (modes 3 and 4: nsb *
+// ceil(ngroup / wpb)): nsb site blocks times `ngroup` work items per site
+// block (the three link directions of the leapfrog, the field chunks of the
+// CN stencil), the items of one site block adjacent.  Blocks are dealt round-robin to the XCDs, each with its own L2;
 // with xcd > 1 the hardware block id is remapped so that each XCD walks one
 // contiguous range of site blocks, and the neighbour data a block needs was
 // fetched by the same XCD shortly before.
@@ -32,11 +32,11 @@
 namespace nrq {
 
 struct lf_map {
-    int64_t nsb; // site blocks (mode 0) or tiles (mode 1)
+    int64_t nsb; // site blocks (modes 0, 3, 4) or tiles (modes 1, 2)
     int bs;      // threads per block
     int xcd;     // XCD remap factor (1: none)
     int mode;    // 0: bs consecutive sites; 1: z-sweep tiles of full x-rows; 2: z-sweep tx x ty tiles;
-                 // 3: wave items over 64 consecutive sites; 4: wave items over banded x-row segments
+                 // 3: wave items over 64 consecutive sites; 4: wave items over 64-site blocks of band-planes
     int ty;      // modes 1, 2: y rows per tile; mode 4: y rows per band
     int ngroup;  // work items per site block
     int tx;      // mode 2: x extent of a tile
