@@ -291,6 +291,45 @@ def test_reunitarisation_interval(nrq, oracle, lattice):
         auto.SetReunitarisationInterval(-1)
 
 
+def test_reunitarisation_inside_the_lds_tile(nrq, oracle, lattice):
+    """The LDS-tiled fused pass projects its links onto SU(3) at the
+    reunitarisation points inside a call: with the tile shape pinned, calls of
+    7, 8 and 10 steps with interval 10 agree with calls ending at the points
+    plus explicit Reunitarise calls (to FP32 rounding: the half steps at the
+    call boundaries are not the tile's code), the stored links are unitary,
+    and both track the FP64 oracle."""
+    ext = (32, 16, 16)
+    _say("reunitarisation inside the LDS tile, 32x16x16")
+    lattice(ext, SPACING_DT005)
+    oracle.lattice_init(ext, SPACING_DT005)
+    U, E = oracle.box_init(7, 0.5, 1.0, 1.0)
+    tiles = [k for k, s in enumerate(nrq.leapfrog_shapes()) if s["tiling"].startswith("LDS") and s["applies_fp32"]]
+    assert tiles
+    nrq.leapfrog_set_shape(tiles[0])
+    try:
+        auto = nrq.GaugeConfiguration(nrq.NRQ_PRECISION_FP32)
+        auto.upload(U, E)
+        auto.SetReunitarisationInterval(10)
+        manual = auto.copy()
+        manual.SetReunitarisationInterval(0)
+        for n in (7, 8, 10):  # 25 steps: reunitarised after 10 (inside a call) and 20 (inside a call)
+            auto.Update(1.0, n)
+        for n in (10, 10):
+            manual.Update(1.0, n)
+            manual.Reunitarise()
+        manual.Update(1.0, 5)
+        Ua, Ea = auto.download()
+        Um, Em = manual.download()
+    finally:
+        nrq.leapfrog_set_shape(-1)
+    assert np.max(np.abs(Ua - Um)) < 1e-5 and np.max(np.abs(Ea - Em)) < 1e-5 * max(1.0, np.max(np.abs(Em)))
+    M = np.moveaxis(Ua.reshape(3, 3, 3, -1), (0, 1), (-2, -1))
+    assert np.abs(M @ np.conj(np.swapaxes(M, -1, -2)) - np.eye(3)).max() < 1e-5
+    oracle.leapfrog(U, E, 1.0, 25)
+    assert np.max(np.abs(Ua - U)) < 1e-4
+    assert np.max(np.abs(Ea - E)) < 1e-4 * max(1.0, np.max(np.abs(E)))
+
+
 def test_ensemble_48cubed_batch_matches_single(nrq, lattice):
     """The ensemble workload's size: four FP32 48^3 members (seeds 100..103,
     Qs 0.3..0.6) with reunitarisation every 10 steps, 30 steps in one batched
