@@ -164,7 +164,9 @@ int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps);
  * and the launch sequence of nsteps steps is captured into a hipGraph and
  * replayed on later calls with the same buffers (NRQ_GRAPH=0 in the
  * environment launches directly).  Each configuration's reunitarisation
- * interval is honoured.  No reference counterpart: the reference evolves
+ * interval is honoured; when all members are at the same point of the same
+ * interval, forward steps project inside the fused passes as
+ * nrq_gauge_leapfrog does.  No reference counterpart: the reference evolves
  * one configuration per run (src/main.f90:2572-2620 loops over runs). */
 int nrq_gauge_leapfrog_batch(nrq_gauge *const *h, int n, double stepwidth, int64_t nsteps);
 /* Kernel timer of the fused leapfrog pass (measurement only, no reference

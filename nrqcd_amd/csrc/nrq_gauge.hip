@@ -1051,7 +1051,8 @@ static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, double sw, int64_t n, 
         // coefficients), the shape and the buffers by value
         std::vector<const void *> key = {(const void *)(intptr_t)g.n0, (const void *)(intptr_t)g.n1,
                                          (const void *)(intptr_t)g.n2, (const void *)(intptr_t)v,
-                                         (const void *)(intptr_t)sizeof(R)};
+                                         (const void *)(intptr_t)sizeof(R), (const void *)(intptr_t)reu_first,
+                                         (const void *)(intptr_t)reu_every};
         for (int k = 0; k < 4; ++k) {
             uint64_t bits;
             memcpy(&bits, &g.a[k], sizeof bits);
@@ -1389,6 +1390,33 @@ extern "C" int nrq_gauge_leapfrog_batch(nrq_gauge *const *h, int n, double stepw
     }
     const char *env = getenv("NRQ_GRAPH");
     const bool graph = !(env && env[0] == '0');
+    bool uniform = stepwidth > 0 && nsteps > 0 && h[0]->reunit_interval > 0;
+    for (int i = 1; i < n && uniform; ++i)
+        uniform = h[i]->reunit_interval == h[0]->reunit_interval && h[i]->steps_since_reunit == h[0]->steps_since_reunit;
+    if (uniform) {
+        // every member at the same point of the same interval (forward steps):
+        // the points inside the call are done by the fused passes, as in
+        // nrq_gauge_leapfrog, one at the call's end by a separate pass
+        const int64_t iv = h[0]->reunit_interval;
+        const int64_t first = std::max<int64_t>(1, iv - h[0]->steps_since_reunit);
+        for (int i0 = 0; i0 < n; i0 += LF_MAXB) {
+            const int nb = std::min(LF_MAXB, n - i0);
+            if (int r = h[0]->precision == 64 ? leapfrog_batch_t<double>(h + i0, nb, stepwidth, nsteps, graph, first, iv)
+                                              : leapfrog_batch_t<float>(h + i0, nb, stepwidth, nsteps, graph, first, iv))
+                return r;
+        }
+        for (int i = 0; i < n; ++i) {
+            if (nsteps < first) {
+                h[i]->steps_since_reunit += nsteps;
+                continue;
+            }
+            const int64_t last = first + (nsteps - first) / iv * iv;
+            h[i]->steps_since_reunit = nsteps - last;
+            if (last == nsteps)
+                if (int r = nrq_gauge_reunitarise(h[i])) return r;
+        }
+        return NRQ_OK;
+    }
     while (nsteps > 0) {
         // stop at the next reunitarisation point of any configuration
         int64_t m = nsteps;
