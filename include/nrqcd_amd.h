@@ -161,9 +161,9 @@ int nrq_gauge_get_efield(const nrq_gauge *h, int dir, int64_t idx, double e[8]);
 int nrq_gauge_leapfrog(nrq_gauge *h, double stepwidth, int64_t nsteps);
 /* nrq_gauge_leapfrog for n configurations of one storage precision on the
  * current lattice (an ensemble): each pass over all of them is one launch,
- * and the launch sequence of nsteps steps is captured into a hipGraph and
- * replayed on later calls with the same buffers (NRQ_GRAPH=0 in the
- * environment launches directly).  Each configuration's reunitarisation
+ * and the launch sequence of nsteps steps (up to 256) is captured into a
+ * hipGraph and replayed on later calls with the same buffers (NRQ_GRAPH=0 in
+ * the environment launches directly, as do longer sequences).  Each configuration's reunitarisation
  * interval is honoured; when all members are at the same point of the same
  * interval, forward steps project inside the fused passes as
  * nrq_gauge_leapfrog does.  No reference counterpart: the reference evolves

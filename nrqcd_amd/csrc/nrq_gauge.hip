@@ -982,6 +982,9 @@ struct lf_graph {
 };
 static std::vector<lf_graph> g_lf_graphs;
 constexpr size_t LF_GRAPH_CACHE = 8;
+// Longer sequences are launched directly: a pass takes far longer than its
+// launch, and a graph of thousands of nodes costs memory and capture time.
+constexpr int64_t LF_GRAPH_MAX_STEPS = 256;
 
 // Span timer of the fused leapfrog launches (measurement only): while it is
 // on, every leapfrog call that is not graph-captured records one event pair
@@ -1043,7 +1046,7 @@ static int leapfrog_batch_t(nrq_gauge *const *hs, int nb, double sw, int64_t n, 
         }
         if (sw > 0) launch_efield<R>(v, table(c, false), nb, g, dt);
     };
-    if (!graph || n < 2) {
+    if (!graph || n < 2 || n > LF_GRAPH_MAX_STEPS) {
         issue(true);
         NRQ_HIP(hipGetLastError());
     } else {
