@@ -135,8 +135,10 @@ __global__ __launch_bounds__(256) void k_conv(const bf16 *__restrict__ in, const
 
 // k_conv with the input staged in LDS, for lattices with n0 % 128 == 0, even
 // n1 and n2 % ZC == 0: a block of 4 wavefronts owns a 128 x 2 tile of (x, y)
-// and sweeps ZC planes in z.  LDS holds a ring of three input planes of the
-// tile with a one-site halo (130 x 4 sites x 32 BF16 channels each, 100 KB);
+// and sweeps ZC planes in z.  LDS holds the layer's weights in the lane order
+// of the MFMA B operands (55 KB, read once per block instead of from L2 at
+// every tap) and a ring of three input planes of the tile with a one-site
+// halo (130 x 4 sites x 32 BF16 channels each, 100 KB);
 // output plane z reads planes z - 1, z, z + 1 there instead of re-reading each
 // input site 27 times through L1, while plane z + 2 streams into registers
 // and then replaces plane z - 1.  Wavefront w computes row y0 + (w & 1), x runs
@@ -151,7 +153,13 @@ __global__ __launch_bounds__(256) void k_conv_lds(const bf16 *__restrict__ in, c
                                                   const bf16 *__restrict__ mask, int relu, geom g)
 {
     __shared__ bf16x8 lds[3 * LT_PLANE];
+    // the layer's weights as the MFMA B operands: [tap][half][lane]
+    __shared__ bf16x8 wl[27 * 2 * 64];
     const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+    for (int q = threadIdx.x; q < 27 * 2 * 64; q += 256) {
+        const int l = q & 63, hf = (q >> 6) & 1, tp = q >> 7;
+        wl[q] = *(const bf16x8 *)(w + ((int64_t)tp * CH + 16 * hf + (l & 15)) * CH + 8 * (l >> 4));
+    }
     const int ntx = g.n0 / LT_X, nty = g.n1 / 2;
     const int b = blockIdx.x;
     const int x0 = (b % ntx) * LT_X, y0 = ((b / ntx) % nty) * 2, z0 = (b / (ntx * nty)) * ZC;
@@ -196,8 +204,8 @@ __global__ __launch_bounds__(256) void k_conv_lds(const bf16 *__restrict__ in, c
             const int slot = (kz + 1 + dz) % 3; // slots hold planes z0 - 1 + p at p mod 3
             for (int dy = -1; dy <= 1; ++dy)
                 for (int dx = -1; dx <= 1; ++dx, ++tap) {
-                    const bf16x8 b0 = *(const bf16x8 *)(w + ((int64_t)tap * CH + m) * CH + 8 * kq);
-                    const bf16x8 b1 = *(const bf16x8 *)(w + ((int64_t)tap * CH + 16 + m) * CH + 8 * kq);
+                    const bf16x8 b0 = wl[(tap * 2 + 0) * 64 + lane];
+                    const bf16x8 b1 = wl[(tap * 2 + 1) * 64 + lane];
                     const bf16x8 *row = lds + slot * LT_PLANE + ((ry + dy + 1) * LT_HX + 16 * xr0 + m + dx + 1) * 4 + kq;
 #pragma unroll
                     for (int j = 0; j < 4; ++j) {
