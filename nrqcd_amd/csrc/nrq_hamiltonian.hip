@@ -34,7 +34,9 @@
 // once.
 #include "nrq_internal.h"
 #include "nrq_grouplog.h"
+#include "nrq_map.h"
 #include <algorithm>
+#include <array>
 #include <cmath>
 #include <complex>
 #include <vector>
@@ -47,6 +49,7 @@ constexpr int HAM_THREADS = 256;
 constexpr int HAM_FLD = 36;   // doubles per site: F0_j (3 x 9), Bsum (9)
 constexpr int HAM_BLK = 13;   // colour matrices per site: Dsum, G[i][sign][p]
 constexpr int HAM_MAXCOL = 12; // columns per apply launch (6 per propagator)
+constexpr size_t HAM_BATCH_BYTES = (size_t)16 << 30; // Krylov storage of the columns solved together
 constexpr int HAM_DOT_BLOCKS = 512;
 
 template <typename R> __device__ __forceinline__ mat3<double> ham_link(const cplx<R> *u, int64_t vol, int dir, int64_t s)
@@ -153,6 +156,7 @@ struct ham_cols {
     const cplx<double> *in[HAM_MAXCOL];
     cplx<double> *out[HAM_MAXCOL];
     cplx<double> a[HAM_MAXCOL][4];
+    int n; // columns
 };
 
 // Pauli matrix entry sigma_k[r][c] of the reference's table (su2.f90:68-88):
@@ -212,14 +216,19 @@ __device__ __forceinline__ void ham_kron(const mat3<double> &G, int k, const cpl
     ham_colour<ADJ>(G, w, acc);
 }
 
-// y = S x (ADJ = false) or S^+ x for the columns of `cols` (blockIdx.y).
+// y = S x (ADJ = false) or S^+ x for the columns of `cols`.  The grid is
+// site blocks x columns with the column fastest, and each XCD walks one
+// contiguous range of it: the columns of a site block run on one XCD one
+// after the other and share its links, field strengths and colour blocks
+// (2.4 KB per site) through that XCD's L2.
 template <typename R, bool ADJ>
 __global__ __launch_bounds__(HAM_THREADS) void k_ham_apply(const cplx<R> *__restrict__ u, const double *__restrict__ fld,
                                                           const cplx<double> *__restrict__ blk, geom g, ham_cols cols)
 {
-    const int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const unsigned lb = xcd_block(gridDim.x, 8);
+    const int col = (int)(lb % (unsigned)cols.n);
+    const int64_t s = (int64_t)(lb / (unsigned)cols.n) * blockDim.x + threadIdx.x;
     if (s >= g.vol) return;
-    const int col = blockIdx.y;
     const int64_t V = g.vol;
     const cplx<double> *x = cols.in[col];
     cplx<double> a[4];
@@ -400,7 +409,8 @@ static int ham_apply(const ham_operator &op, bool adj, int ncol, const cplx<doub
         cols.out[c] = out[c];
         for (int t = 0; t < 4; ++t) cols.a[c][t] = a[c][t];
     }
-    const dim3 grd((unsigned)((op.g.vol + HAM_THREADS - 1) / HAM_THREADS), (unsigned)ncol);
+    cols.n = ncol;
+    const dim3 grd((unsigned)((op.g.vol + HAM_THREADS - 1) / HAM_THREADS) * (unsigned)ncol);
     if (op.precision == 64) {
         if (adj)
             hipLaunchKernelGGL((k_ham_apply<double, true>), grd, dim3(HAM_THREADS), 0, stream(), (const cplx<double> *)op.u, op.fld, op.blk, op.g, cols);
@@ -416,118 +426,200 @@ static int ham_apply(const ham_operator &op, bool adj, int ncol, const cplx<doub
     return NRQ_OK;
 }
 
-// ||w||^2 to the host
-static int ham_norm2(const cplx<double> *w, int64_t n, double *red, double *out_dev, double *host, double *res)
+// GMRES(m) for S^+ x_c = b_c on nc columns (n = 6V complex each), zero
+// initial guess, stopping on ||b_c - S^+ x_c|| <= tol ||b_c||.  The columns
+// run in lockstep, each through exactly the operations (and summation order)
+// it would take alone: one operator launch for all columns of an Arnoldi
+// step and one host synchronisation per step for all their Gram-Schmidt
+// coefficients.  Column c uses basis block c of the workspace.  A column
+// that reaches maxit leaves NRQ_ERR_NOCONV as the result; the others finish.
+static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4], cplx<double> *const *b,
+                     cplx<double> *const *x, int64_t n, int m, int64_t maxit, double tol, nrq_solve_stats *st)
 {
-    hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), w, n, red);
-    hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red, HAM_DOT_BLOCKS, out_dev);
-    NRQ_HIP(hipGetLastError());
-    NRQ_HIP(hipMemcpyAsync(host, out_dev, sizeof(double), hipMemcpyDeviceToHost, stream()));
-    NRQ_HIP(hipStreamSynchronize(stream()));
-    *res = host[0];
-    return NRQ_OK;
-}
-
-// GMRES(m) for S^+ x = b on one column (n = 6V complex), zero initial guess,
-// stopping on ||b - S^+ x|| <= tol ||b||.
-static int ham_gmres(const ham_operator &op, const cplx<double> (&a)[4], const cplx<double> *b, cplx<double> *x, int64_t n,
-                     int m, int64_t maxit, double tol, nrq_solve_stats *st)
-{
-    cplx<double> *basis = (cplx<double> *)g_ham.basis.ptr;
-    cplx<double> *w = (cplx<double> *)g_ham.vec.ptr, *r = w + n;
-    double *red = (double *)g_ham.red.ptr;
-    double *dsmall = (double *)g_ham.small.ptr; // 2 (m + 2) doubles
-    double *host = g_ham.host;
-    const cplx<double> coef[1][4] = {{a[0], a[1], a[2], a[3]}};
+    const int64_t ldb = (int64_t)(m + 1) * n;                 // basis block per column
+    const int64_t lred = 2 * (int64_t)(m + 2) * HAM_DOT_BLOCKS; // reduction partials per column
+    const int lsm = 2 * (m + 2);                               // small device / pinned host slots per column
+    auto basis = [&](int c) { return (cplx<double> *)g_ham.basis.ptr + c * ldb; };
+    auto rvec = [&](int c) { return (cplx<double> *)g_ham.vec.ptr + (int64_t)c * n; };
+    auto red = [&](int c) { return (double *)g_ham.red.ptr + c * lred; };
+    auto dsm = [&](int c) { return (double *)g_ham.small.ptr + (int64_t)c * lsm; };
+    auto hst = [&](int c) { return g_ham.host + (int64_t)c * lsm; };
     const unsigned nb = ham_blocks(n);
-    double bn2;
-    if (int e = ham_norm2(b, n, red, dsmall, host, &bn2)) return e;
-    NRQ_HIP(hipMemsetAsync(x, 0, sizeof(cplx<double>) * (size_t)n, stream()));
-    if (bn2 == 0.0) return NRQ_OK; // x = 0, as KSPSolve returns for b = 0
-    ++st->solves;
-    const double bnorm = std::sqrt(bn2);
-    // r = b (x = 0)
-    hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), r, b, 1.0, b, 0.0, n);
-    double rnorm = bnorm;
-    int64_t its = 0;
-    std::vector<std::complex<double>> H((size_t)(m + 1) * m), cs(m), sn(m), g(m + 1), y(m);
+    using zc = std::complex<double>;
+    struct column {
+        std::vector<zc> H, cs, sn, g, y;
+        double bnorm = 0, rnorm = 0;
+        int64_t its = 0;
+        int k = 0;           // Arnoldi steps of the current cycle
+        bool active = false; // not yet converged
+        bool cycling = false;
+        bool failed = false;
+    };
+    std::vector<column> col(nc);
+    // ||b||^2 of every column, one synchronisation
+    for (int c = 0; c < nc; ++c) {
+        hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), b[c], n, red(c));
+        hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red(c), HAM_DOT_BLOCKS, dsm(c));
+        NRQ_HIP(hipMemcpyAsync(hst(c), dsm(c), sizeof(double), hipMemcpyDeviceToHost, stream()));
+    }
+    NRQ_HIP(hipGetLastError());
+    NRQ_HIP(hipStreamSynchronize(stream()));
+    for (int c = 0; c < nc; ++c) {
+        column &C = col[c];
+        const double bn2 = hst(c)[0];
+        NRQ_HIP(hipMemsetAsync(x[c], 0, sizeof(cplx<double>) * (size_t)n, stream()));
+        if (bn2 == 0.0) continue; // x = 0, as KSPSolve returns for b = 0
+        ++st->solves;
+        C.active = true;
+        C.bnorm = C.rnorm = std::sqrt(bn2);
+        C.H.assign((size_t)(m + 1) * m, 0.0);
+        C.cs.assign(m, 0.0);
+        C.sn.assign(m, 0.0);
+        C.g.assign(m + 1, 0.0);
+        C.y.assign(m, 0.0);
+        // r = b (x = 0)
+        hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), rvec(c), b[c], 1.0, b[c], 0.0, n);
+    }
+    const cplx<double> *ins[HAM_MAXCOL];
+    cplx<double> *outs[HAM_MAXCOL];
+    cplx<double> cof[HAM_MAXCOL][4];
+    int sel[HAM_MAXCOL];
     while (true) {
-        ++st->restarts;
-        // V_0 = r / ||r||
-        hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), basis, r, 1.0 / rnorm, r, 0.0, n);
-        std::fill(g.begin(), g.end(), 0.0);
-        g[0] = rnorm;
-        int k = 0;
-        for (; k < m && its < maxit; ++k) {
-            ++its;
-            cplx<double> *vk = basis + (int64_t)k * n, *vk1 = basis + (int64_t)(k + 1) * n;
-            if (int e = ham_apply(op, true, 1, (const cplx<double> *const *)&vk, &vk1, coef)) return e;
-            // classical Gram-Schmidt: h = V^+ w, w -= V h
-            hipLaunchKernelGGL(k_ham_dots, dim3(HAM_DOT_BLOCKS, k + 1), dim3(HAM_THREADS), 0, stream(), basis, n, vk1, red);
-            hipLaunchKernelGGL(k_ham_fold, dim3(2 * (k + 1)), dim3(HAM_THREADS), 0, stream(), red, HAM_DOT_BLOCKS, dsmall);
-            hipLaunchKernelGGL(k_ham_combine, dim3(nb), dim3(HAM_THREADS), 0, stream(), vk1, basis, n, dsmall, k + 1, -1.0);
-            hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), vk1, n, red);
-            hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red, HAM_DOT_BLOCKS, dsmall + 2 * (k + 1));
+        int nact = 0;
+        for (int c = 0; c < nc; ++c) nact += col[c].active;
+        if (!nact) break;
+        // new cycle: V_0 = r / ||r||
+        for (int c = 0; c < nc; ++c) {
+            column &C = col[c];
+            if (!C.active) continue;
+            ++st->restarts;
+            hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), basis(c), rvec(c), 1.0 / C.rnorm,
+                               rvec(c), 0.0, n);
+            std::fill(C.g.begin(), C.g.end(), 0.0);
+            C.g[0] = C.rnorm;
+            C.k = 0;
+            C.cycling = true;
+        }
+        for (int k = 0; k < m; ++k) {
+            int ns = 0;
+            for (int c = 0; c < nc; ++c) {
+                column &C = col[c];
+                if (C.active && C.cycling && C.its >= maxit) C.cycling = false;
+                if (!(C.active && C.cycling)) continue;
+                ++C.its;
+                sel[ns] = c;
+                ins[ns] = basis(c) + (int64_t)k * n;
+                outs[ns] = basis(c) + (int64_t)(k + 1) * n;
+                for (int t = 0; t < 4; ++t) cof[ns][t] = a[c][t];
+                ++ns;
+            }
+            if (!ns) break;
+            if (int e = ham_apply(op, true, ns, ins, outs, cof)) return e;
+            // classical Gram-Schmidt per column: h = V^+ w, w -= V h, ||w||^2
+            for (int q = 0; q < ns; ++q) {
+                const int c = sel[q];
+                cplx<double> *vk1 = outs[q];
+                hipLaunchKernelGGL(k_ham_dots, dim3(HAM_DOT_BLOCKS, k + 1), dim3(HAM_THREADS), 0, stream(), basis(c), n, vk1,
+                                   red(c));
+                hipLaunchKernelGGL(k_ham_fold, dim3(2 * (k + 1)), dim3(HAM_THREADS), 0, stream(), red(c), HAM_DOT_BLOCKS,
+                                   dsm(c));
+                hipLaunchKernelGGL(k_ham_combine, dim3(nb), dim3(HAM_THREADS), 0, stream(), vk1, basis(c), n, dsm(c), k + 1,
+                                   -1.0);
+                hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), vk1, n, red(c));
+                hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red(c), HAM_DOT_BLOCKS,
+                                   dsm(c) + 2 * (k + 1));
+                NRQ_HIP(hipMemcpyAsync(hst(c), dsm(c), sizeof(double) * (2 * (k + 1) + 1), hipMemcpyDeviceToHost, stream()));
+            }
             NRQ_HIP(hipGetLastError());
-            NRQ_HIP(hipMemcpyAsync(host, dsmall, sizeof(double) * (2 * (k + 1) + 1), hipMemcpyDeviceToHost, stream()));
             NRQ_HIP(hipStreamSynchronize(stream()));
-            for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] = {host[2 * i], host[2 * i + 1]};
-            const double hk1 = std::sqrt(host[2 * (k + 1)]);
-            H[(size_t)(k + 1) * m + k] = hk1;
-            if (hk1 > 0)
-                hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), vk1, vk1, 1.0 / hk1, vk1, 0.0, n);
-            // Givens rotations
+            for (int q = 0; q < ns; ++q) {
+                const int c = sel[q];
+                column &C = col[c];
+                const double *host = hst(c);
+                std::vector<zc> &H = C.H;
+                for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] = {host[2 * i], host[2 * i + 1]};
+                const double hk1 = std::sqrt(host[2 * (k + 1)]);
+                H[(size_t)(k + 1) * m + k] = hk1;
+                if (hk1 > 0)
+                    hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), outs[q], outs[q], 1.0 / hk1,
+                                       outs[q], 0.0, n);
+                // Givens rotations
+                for (int i = 0; i < k; ++i) {
+                    const zc t = C.cs[i] * H[(size_t)i * m + k] + C.sn[i] * H[(size_t)(i + 1) * m + k];
+                    H[(size_t)(i + 1) * m + k] = -std::conj(C.sn[i]) * H[(size_t)i * m + k] + C.cs[i] * H[(size_t)(i + 1) * m + k];
+                    H[(size_t)i * m + k] = t;
+                }
+                const zc hkk = H[(size_t)k * m + k];
+                const double den = std::sqrt(std::norm(hkk) + hk1 * hk1);
+                if (den == 0.0) {
+                    C.cs[k] = 1.0;
+                    C.sn[k] = 0.0;
+                } else {
+                    C.cs[k] = std::abs(hkk) / den;
+                    const zc ph = std::abs(hkk) > 0 ? hkk / std::abs(hkk) : zc(1.0, 0.0);
+                    C.sn[k] = ph * hk1 / den;
+                }
+                H[(size_t)k * m + k] = C.cs[k] * hkk + C.sn[k] * hk1;
+                H[(size_t)(k + 1) * m + k] = 0.0;
+                C.g[k + 1] = -std::conj(C.sn[k]) * C.g[k];
+                C.g[k] = C.cs[k] * C.g[k];
+                C.k = k + 1;
+                if (std::abs(C.g[k + 1]) <= 0.5 * tol * C.bnorm || hk1 == 0.0) C.cycling = false;
+            }
+        }
+        // end of the cycle: y = H^-1 g (upper triangular, k x k), x += V y, then
+        // the true residual r = b - S^+ x of every column that ran
+        int ns = 0;
+        for (int c = 0; c < nc; ++c) {
+            column &C = col[c];
+            if (!C.active) continue;
+            const int k = C.k;
+            for (int i = k - 1; i >= 0; --i) {
+                zc t = C.g[i];
+                for (int j = i + 1; j < k; ++j) t -= C.H[(size_t)i * m + j] * C.y[j];
+                C.y[i] = t / C.H[(size_t)i * m + i];
+            }
+            double *host = hst(c);
             for (int i = 0; i < k; ++i) {
-                const std::complex<double> t = cs[i] * H[(size_t)i * m + k] + sn[i] * H[(size_t)(i + 1) * m + k];
-                H[(size_t)(i + 1) * m + k] = -std::conj(sn[i]) * H[(size_t)i * m + k] + cs[i] * H[(size_t)(i + 1) * m + k];
-                H[(size_t)i * m + k] = t;
+                host[2 * i] = C.y[i].real();
+                host[2 * i + 1] = C.y[i].imag();
             }
-            const std::complex<double> hkk = H[(size_t)k * m + k];
-            const double den = std::sqrt(std::norm(hkk) + hk1 * hk1);
-            if (den == 0.0) {
-                cs[k] = 1.0;
-                sn[k] = 0.0;
-            } else {
-                cs[k] = std::abs(hkk) / den;
-                const std::complex<double> ph = std::abs(hkk) > 0 ? hkk / std::abs(hkk) : std::complex<double>(1.0, 0.0);
-                sn[k] = ph * hk1 / den;
+            NRQ_HIP(hipMemcpyAsync(dsm(c), host, sizeof(double) * 2 * k, hipMemcpyHostToDevice, stream()));
+            hipLaunchKernelGGL(k_ham_combine, dim3(nb), dim3(HAM_THREADS), 0, stream(), x[c], basis(c), n, dsm(c), k, 1.0);
+            sel[ns] = c;
+            ins[ns] = x[c];
+            outs[ns] = rvec(c);
+            for (int t = 0; t < 4; ++t) cof[ns][t] = a[c][t];
+            ++ns;
+        }
+        if (int e = ham_apply(op, true, ns, ins, outs, cof)) return e;
+        for (int q = 0; q < ns; ++q) {
+            const int c = sel[q];
+            hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), rvec(c), b[c], 1.0, rvec(c), -1.0, n);
+            hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), rvec(c), n, red(c));
+            hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red(c), HAM_DOT_BLOCKS, dsm(c));
+            NRQ_HIP(hipMemcpyAsync(hst(c), dsm(c), sizeof(double), hipMemcpyDeviceToHost, stream()));
+        }
+        NRQ_HIP(hipGetLastError());
+        NRQ_HIP(hipStreamSynchronize(stream()));
+        for (int q = 0; q < ns; ++q) {
+            column &C = col[sel[q]];
+            C.rnorm = std::sqrt(hst(sel[q])[0]);
+            st->iterations += C.k;
+            if (C.rnorm <= tol * C.bnorm) {
+                C.active = false;
+                st->max_true_relres = std::max(st->max_true_relres, C.rnorm / C.bnorm);
+            } else if (C.its >= maxit) {
+                C.active = false;
+                C.failed = true;
+                st->max_true_relres = std::max(st->max_true_relres, C.rnorm / C.bnorm);
             }
-            H[(size_t)k * m + k] = cs[k] * hkk + sn[k] * hk1;
-            H[(size_t)(k + 1) * m + k] = 0.0;
-            g[k + 1] = -std::conj(sn[k]) * g[k];
-            g[k] = cs[k] * g[k];
-            if (std::abs(g[k + 1]) <= 0.5 * tol * bnorm || hk1 == 0.0) {
-                ++k;
-                break;
-            }
-        }
-        // y = H^-1 g (upper triangular, k x k), x += V y
-        for (int i = k - 1; i >= 0; --i) {
-            std::complex<double> t = g[i];
-            for (int j = i + 1; j < k; ++j) t -= H[(size_t)i * m + j] * y[j];
-            y[i] = t / H[(size_t)i * m + i];
-        }
-        for (int i = 0; i < k; ++i) {
-            host[2 * i] = y[i].real();
-            host[2 * i + 1] = y[i].imag();
-        }
-        NRQ_HIP(hipMemcpyAsync(dsmall, host, sizeof(double) * 2 * k, hipMemcpyHostToDevice, stream()));
-        hipLaunchKernelGGL(k_ham_combine, dim3(nb), dim3(HAM_THREADS), 0, stream(), x, basis, n, dsmall, k, 1.0);
-        // true residual r = b - S^+ x
-        if (int e = ham_apply(op, true, 1, (const cplx<double> *const *)&x, &r, coef)) return e;
-        hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), r, b, 1.0, r, -1.0, n);
-        double rn2;
-        if (int e = ham_norm2(r, n, red, dsmall, host, &rn2)) return e;
-        rnorm = std::sqrt(rn2);
-        st->iterations += k;
-        if (rnorm <= tol * bnorm) break;
-        if (its >= maxit) {
-            st->max_true_relres = std::max(st->max_true_relres, rnorm / bnorm);
-            return fail(NRQ_ERR_NOCONV, "CN (full Hamiltonian): no convergence after %lld iterations (relres %.3e)",
-                        (long long)its, rnorm / bnorm);
         }
     }
-    st->max_true_relres = std::max(st->max_true_relres, rnorm / bnorm);
+    for (int c = 0; c < nc; ++c)
+        if (col[c].failed)
+            return fail(NRQ_ERR_NOCONV, "CN (full Hamiltonian): no convergence after %lld iterations (relres %.3e)",
+                        (long long)col[c].its, col[c].rnorm / col[c].bnorm);
     return NRQ_OK;
 }
 
@@ -538,21 +630,26 @@ int ham_cn_update(nrq_heavy *const *hs, const double *mass, int n, const nrq_gau
     const int64_t V = g.vol;
     const int64_t nvec = 6 * V;
     const int m = std::max(1, restart);
+    // columns solved together: as many as fit HAM_BATCH_BYTES of Krylov basis,
+    // residual and right-hand side (m + 3 vectors each), at most HAM_MAXCOL
+    const size_t per_col = sizeof(cplx<double>) * (size_t)(m + 3) * (size_t)nvec;
+    const int ncb = (int)std::max<size_t>(1, std::min<size_t>(HAM_MAXCOL, HAM_BATCH_BYTES / per_col));
     if (int r = g_ham.fld.reserve(sizeof(double) * HAM_FLD * (size_t)V)) return r;
     if (int r = g_ham.blk.reserve(sizeof(cplx<double>) * 9 * HAM_BLK * (size_t)V)) return r;
-    if (int r = g_ham.basis.reserve(sizeof(cplx<double>) * (size_t)(m + 1) * (size_t)nvec)) return r;
-    if (int r = g_ham.vec.reserve(sizeof(cplx<double>) * 4 * (size_t)nvec)) return r;
-    if (int r = g_ham.red.reserve(sizeof(double) * 2 * (size_t)(m + 2) * HAM_DOT_BLOCKS)) return r;
-    if (int r = g_ham.small.reserve(sizeof(double) * 2 * (size_t)(m + 2))) return r;
-    if (g_ham.host_m < m) {
+    if (int r = g_ham.basis.reserve(sizeof(cplx<double>) * (size_t)(m + 1) * (size_t)nvec * ncb)) return r;
+    if (int r = g_ham.vec.reserve(sizeof(cplx<double>) * 2 * (size_t)nvec * ncb)) return r;
+    if (int r = g_ham.red.reserve(sizeof(double) * 2 * (size_t)(m + 2) * HAM_DOT_BLOCKS * ncb)) return r;
+    if (int r = g_ham.small.reserve(sizeof(double) * 2 * (size_t)(m + 2) * ncb)) return r;
+    const int host_need = (m + 2) * ncb;
+    if (g_ham.host_m < host_need) {
         if (g_ham.host) (void)hipHostFree(g_ham.host);
         g_ham.host = nullptr;
-        if (hipHostMalloc((void **)&g_ham.host, sizeof(double) * 2 * (size_t)(m + 2), hipHostMallocDefault) != hipSuccess) {
+        if (hipHostMalloc((void **)&g_ham.host, sizeof(double) * 2 * (size_t)host_need, hipHostMallocDefault) != hipSuccess) {
             (void)hipGetLastError();
             g_ham.host_m = -1;
             return fail(NRQ_ERR_ALLOC, "CN (full Hamiltonian): pinned buffer allocation failed");
         }
-        g_ham.host_m = m;
+        g_ham.host_m = host_need;
     }
     ham_operator op;
     op.u = gauge_links(gf);
@@ -575,11 +672,15 @@ int ham_cn_update(nrq_heavy *const *hs, const double *mass, int n, const nrq_gau
     NRQ_HIP(hipGetLastError());
     const double a0 = g.a[0];
     const std::complex<double> pre(0.0, -stepwidth / 2.0 * a0), I(0.0, 1.0);
-    cplx<double> *bvec = (cplx<double> *)g_ham.vec.ptr + 2 * nvec; // rhs
+    // right-hand sides after the ncb residual vectors
+    auto bvec = [&](int c) { return (cplx<double> *)g_ham.vec.ptr + (int64_t)(ncb + c) * nvec; };
     nrq_solve_stats st = {0, 0, 0, 0.0};
     int status = NRQ_OK;
     for (int hk = 0; hk < n; ++hk) {
         nrq_heavy *h = hs[hk];
+        // the live columns of both propagators, solved ncb at a time
+        std::vector<int> f0s;
+        std::vector<std::array<cplx<double>, 4>> coefs;
         for (int prop = 0; prop < 2; ++prop) {
             const double M = prop == 0 ? mass[hk] : -mass[hk];
             std::complex<double> cc[4];
@@ -589,23 +690,35 @@ int ham_cn_update(nrq_heavy *const *hs, const double *mass, int n, const nrq_gau
             }
             const std::complex<double> at[4] = {pre * cc[0] / M, pre * (-cc[1]) / (2.0 * M),
                                                 pre * (-cc[2]) / (8.0 * M * M), pre * (-cc[3]) * I / (8.0 * M * M)};
-            cplx<double> a[4];
+            std::array<cplx<double>, 4> a;
             for (int t = 0; t < 4; ++t) a[t] = {at[t].real(), at[t].imag()};
             for (int j = 0; j < 6; ++j) {
                 const int f0 = 12 * prop + 2 * j;
                 if (!((h->live >> f0) & 3u)) continue;
-                cplx<double> *xcol = (cplx<double> *)h->psi + (int64_t)f0 * 3 * V;
-                const cplx<double> coef[1][4] = {{a[0], a[1], a[2], a[3]}};
-                // rhs = S psi
-                if (int e = ham_apply(op, false, 1, (const cplx<double> *const *)&xcol, &bvec, coef)) return e;
-                const int r = ham_gmres(op, a, bvec, xcol, nvec, m, maxit, tol, &st);
-                h->live |= 3u << f0; // the spin-orbit term fills both spin rows
-                if (r == NRQ_ERR_NOCONV) {
-                    status = r;
-                    continue;
-                }
-                if (r != NRQ_OK) return r;
+                f0s.push_back(f0);
+                coefs.push_back(a);
             }
+        }
+        for (size_t c0 = 0; c0 < f0s.size(); c0 += (size_t)ncb) {
+            const int nc = (int)std::min<size_t>((size_t)ncb, f0s.size() - c0);
+            const cplx<double> *xin[HAM_MAXCOL];
+            cplx<double> *xcol[HAM_MAXCOL], *bcol[HAM_MAXCOL];
+            cplx<double> cof[HAM_MAXCOL][4];
+            for (int c = 0; c < nc; ++c) {
+                xcol[c] = (cplx<double> *)h->psi + (int64_t)f0s[c0 + c] * 3 * V;
+                xin[c] = xcol[c];
+                bcol[c] = bvec(c);
+                for (int t = 0; t < 4; ++t) cof[c][t] = coefs[c0 + c][t];
+            }
+            // rhs = S psi for the batch, then S^+ psi' = rhs
+            if (int e = ham_apply(op, false, nc, xin, bcol, cof)) return e;
+            const int r = ham_gmres(op, nc, cof, bcol, xcol, nvec, m, maxit, tol, &st);
+            for (int c = 0; c < nc; ++c) h->live |= 3u << f0s[c0 + c]; // the spin-orbit term fills both spin rows
+            if (r == NRQ_ERR_NOCONV) {
+                status = r;
+                continue;
+            }
+            if (r != NRQ_OK) return r;
         }
     }
     if (stats) *stats = st;
