@@ -712,16 +712,16 @@ static bool variant_ok(int variant, const geom &g, int prec)
     do {                                                                   \
         const lf_variant lv_ = LF_VARIANTS[variant];                       \
         if (lv_.nt && lv_.rec) {                                           \
-            constexpr bool NTV = true, RECV = true;                        \
+            constexpr bool NTV = true, RECV = true; (void)RECV;            \
             launch;                                                        \
         } else if (lv_.nt) {                                               \
-            constexpr bool NTV = true, RECV = false;                       \
+            constexpr bool NTV = true, RECV = false; (void)RECV;           \
             launch;                                                        \
         } else if (lv_.rec) {                                              \
-            constexpr bool NTV = false, RECV = true;                       \
+            constexpr bool NTV = false, RECV = true; (void)RECV;           \
             launch;                                                        \
         } else {                                                           \
-            constexpr bool NTV = false, RECV = false;                      \
+            constexpr bool NTV = false, RECV = false; (void)RECV;          \
             launch;                                                        \
         }                                                                  \
     } while (0)

@@ -507,7 +507,7 @@ __global__ __launch_bounds__(VTHREADS) void k_mdot(const cplx<T> *__restrict__ b
             }
         }
         __syncthreads();
-        if (threadIdx.x < 2 * gn) {
+        if ((int)threadIdx.x < 2 * gn) {
             double t = 0;
             for (int k = 0; k < VTHREADS / 64; ++k) t += sh[k][threadIdx.x];
             const int j = threadIdx.x >> 1;
