@@ -109,3 +109,26 @@ def test_spin_orbit_fills_the_other_spin_row(nrq, oracle, cn):
         assert np.max(np.abs(Ag - A)) < 1e-10 * np.max(np.abs(A))
         mixed = max(np.abs(Qg[d, j]).max() for j in range(6) for d in range(6) if d % 2 != j % 2)
         assert (mixed == 0.0) if c[3] == 0 else (mixed > 1e-8)
+
+
+def test_full_hamiltonian_reports_nonconvergence(nrq, oracle, cn):
+    """The columns of a step are solved together: with an iteration cap below
+    what they need, the step reports NRQ_ERR_NOCONV instead of returning the
+    unconverged fields as converged; with the cap lifted, the next step
+    converges every column again."""
+    ext, sp = (4, 3, 5), (0.05, 1.0, 0.9, 1.2)
+    _say("full Hamiltonian, iteration cap 2")
+    lat, g, U, E = _setup(nrq, oracle, cn, ext, sp)
+    h = nrq.NRQCDField()
+    h.InitSinglePoint(1, 1)
+    nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_GMRES_FP64, 30, 2)
+    try:
+        with pytest.raises(nrq.NRQError) as err:
+            h.Update(g, 2.0, [1, 1, 1, 1], 1.0)
+        assert err.value.code == -5  # NRQ_ERR_NOCONV
+    finally:
+        nrq.nrqcd_set_solver(nrq.NRQ_SOLVER_GMRES_FP64, 30, 10000)
+    Q, A = h.download()
+    assert np.all(np.isfinite(Q)) and np.all(np.isfinite(A))
+    st = h.Update(g, 2.0, [1, 1, 1, 1], 1.0)
+    assert st.max_true_relres <= 1e-12 and st.solves == 12
