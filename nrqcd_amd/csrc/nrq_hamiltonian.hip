@@ -315,12 +315,25 @@ __device__ __forceinline__ double ham_block_sum(double v, double *sh)
     return t;
 }
 
+// Per-column pointers of the Krylov vector kernels: column q of a launch
+// (blockIdx.z for k_ham_dots, blockIdx.y for the others) works on basis[q],
+// w[q], partial[q], small[q] with scale[q]; every column's sums are the same
+// as in a launch of its own.
+struct ham_vecs {
+    const cplx<double> *basis[HAM_MAXCOL];
+    cplx<double> *w[HAM_MAXCOL];
+    double *partial[HAM_MAXCOL];
+    double *small[HAM_MAXCOL];
+    double scale[HAM_MAXCOL];
+};
+
 // partial[(2 i + {0: re, 1: im}) * gridDim.x + block] of <V_i, w>, i = blockIdx.y
-__global__ __launch_bounds__(HAM_THREADS) void k_ham_dots(const cplx<double> *__restrict__ basis, int64_t n,
-                                                         const cplx<double> *__restrict__ w, double *__restrict__ partial)
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_dots(ham_vecs t, int64_t n)
 {
-    const int i = blockIdx.y;
-    const cplx<double> *vi = basis + (int64_t)i * n;
+    const int i = blockIdx.y, q = blockIdx.z;
+    const cplx<double> *vi = t.basis[q] + (int64_t)i * n;
+    const cplx<double> *__restrict__ w = t.w[q];
+    double *__restrict__ partial = t.partial[q];
     cplx<double> acc = {0.0, 0.0};
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
         mac_cn(acc, vi[e], w[e]);
@@ -334,21 +347,24 @@ __global__ __launch_bounds__(HAM_THREADS) void k_ham_dots(const cplx<double> *__
 }
 
 // partial[block] = sum |w|^2
-__global__ __launch_bounds__(HAM_THREADS) void k_ham_norm2(const cplx<double> *__restrict__ w, int64_t n,
-                                                          double *__restrict__ partial)
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_norm2(ham_vecs t, int64_t n)
 {
+    const cplx<double> *__restrict__ w = t.w[blockIdx.y];
+    double *__restrict__ partial = t.partial[blockIdx.y];
     double acc = 0;
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
         acc += w[e].re * w[e].re + w[e].im * w[e].im;
     __shared__ double sh[HAM_THREADS / 64];
-    const double t = ham_block_sum(acc, sh);
-    if (threadIdx.x == 0) partial[blockIdx.x] = t;
+    const double tot = ham_block_sum(acc, sh);
+    if (threadIdx.x == 0) partial[blockIdx.x] = tot;
 }
 
-// out[q] = sum_b partial[q * nb + b] for q = blockIdx.x (fixed order)
-__global__ __launch_bounds__(HAM_THREADS) void k_ham_fold(const double *__restrict__ partial, int nb,
-                                                         double *__restrict__ out)
+// out[q] = sum_b partial[q * nb + b] for q = blockIdx.x (fixed order), out =
+// small + offset of the column
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_fold(ham_vecs t, int nb, int offset)
 {
+    const double *__restrict__ partial = t.partial[blockIdx.y];
+    double *__restrict__ out = t.small[blockIdx.y] + offset;
     const int q = blockIdx.x;
     double s = 0;
     for (int b = threadIdx.x; b < nb; b += HAM_THREADS) s += partial[(int64_t)q * nb + b];
@@ -357,15 +373,27 @@ __global__ __launch_bounds__(HAM_THREADS) void k_ham_fold(const double *__restri
     if (threadIdx.x == 0) out[q] = s;
 }
 
-// w += sign sum_{i < k} c_i V_i (c: complex coefficients on the device)
-__global__ __launch_bounds__(HAM_THREADS) void k_ham_combine(cplx<double> *__restrict__ w, const cplx<double> *__restrict__ basis,
-                                                            int64_t n, const double *__restrict__ c, int k, double sign)
+// w += sign sum_{i < k} c_i V_i (c: complex coefficients on the device, the
+// column's small)
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_combine(ham_vecs t, int64_t n, int k, double sign)
 {
+    cplx<double> *__restrict__ w = t.w[blockIdx.y];
+    const cplx<double> *__restrict__ basis = t.basis[blockIdx.y];
+    const double *__restrict__ c = t.small[blockIdx.y];
     for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x) {
         cplx<double> acc = {0.0, 0.0};
         for (int i = 0; i < k; ++i) mac(acc, cplx<double>{c[2 * i], c[2 * i + 1]}, basis[(int64_t)i * n + e]);
         w[e] = {w[e].re + sign * acc.re, w[e].im + sign * acc.im};
     }
+}
+
+// w = scale w for the columns of the table
+__global__ __launch_bounds__(HAM_THREADS) void k_ham_scale(ham_vecs t, int64_t n)
+{
+    cplx<double> *__restrict__ w = t.w[blockIdx.y];
+    const double a = t.scale[blockIdx.y];
+    for (int64_t e = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; e < n; e += (int64_t)gridDim.x * blockDim.x)
+        w[e] = {a * w[e].re, a * w[e].im};
 }
 
 // y = a x + b z (real a, b)
@@ -445,6 +473,22 @@ static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4],
     auto dsm = [&](int c) { return (double *)g_ham.small.ptr + (int64_t)c * lsm; };
     auto hst = [&](int c) { return g_ham.host + (int64_t)c * lsm; };
     const unsigned nb = ham_blocks(n);
+    // the Krylov-kernel table of the columns sel[0 .. ns) with vectors wv
+    auto table = [&](int ns, const int *sl, cplx<double> *const *wv) {
+        ham_vecs t{};
+        for (int q = 0; q < ns; ++q) {
+            t.basis[q] = basis(sl[q]);
+            t.w[q] = wv[q];
+            t.partial[q] = red(sl[q]);
+            t.small[q] = dsm(sl[q]);
+            t.scale[q] = 1.0;
+        }
+        return t;
+    };
+    // the small device slots of all columns to the pinned host slots
+    auto smalls_to_host = [&]() {
+        return hipMemcpyAsync(hst(0), dsm(0), sizeof(double) * (size_t)lsm * (size_t)nc, hipMemcpyDeviceToHost, stream());
+    };
     using zc = std::complex<double>;
     struct column {
         std::vector<zc> H, cs, sn, g, y;
@@ -456,12 +500,21 @@ static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4],
         bool failed = false;
     };
     std::vector<column> col(nc);
+    const cplx<double> *ins[HAM_MAXCOL];
+    cplx<double> *outs[HAM_MAXCOL];
+    cplx<double> cof[HAM_MAXCOL][4];
+    int sel[HAM_MAXCOL];
     // ||b||^2 of every column, one synchronisation
     for (int c = 0; c < nc; ++c) {
-        hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), b[c], n, red(c));
-        hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red(c), HAM_DOT_BLOCKS, dsm(c));
-        NRQ_HIP(hipMemcpyAsync(hst(c), dsm(c), sizeof(double), hipMemcpyDeviceToHost, stream()));
+        sel[c] = c;
+        outs[c] = b[c];
     }
+    {
+        const ham_vecs t = table(nc, sel, outs);
+        hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS, nc), dim3(HAM_THREADS), 0, stream(), t, n);
+        hipLaunchKernelGGL(k_ham_fold, dim3(1, nc), dim3(HAM_THREADS), 0, stream(), t, HAM_DOT_BLOCKS, 0);
+    }
+    NRQ_HIP(smalls_to_host());
     NRQ_HIP(hipGetLastError());
     NRQ_HIP(hipStreamSynchronize(stream()));
     for (int c = 0; c < nc; ++c) {
@@ -480,10 +533,6 @@ static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4],
         // r = b (x = 0)
         hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), rvec(c), b[c], 1.0, b[c], 0.0, n);
     }
-    const cplx<double> *ins[HAM_MAXCOL];
-    cplx<double> *outs[HAM_MAXCOL];
-    cplx<double> cof[HAM_MAXCOL][4];
-    int sel[HAM_MAXCOL];
     while (true) {
         int nact = 0;
         for (int c = 0; c < nc; ++c) nact += col[c].active;
@@ -515,23 +564,17 @@ static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4],
             }
             if (!ns) break;
             if (int e = ham_apply(op, true, ns, ins, outs, cof)) return e;
-            // classical Gram-Schmidt per column: h = V^+ w, w -= V h, ||w||^2
-            for (int q = 0; q < ns; ++q) {
-                const int c = sel[q];
-                cplx<double> *vk1 = outs[q];
-                hipLaunchKernelGGL(k_ham_dots, dim3(HAM_DOT_BLOCKS, k + 1), dim3(HAM_THREADS), 0, stream(), basis(c), n, vk1,
-                                   red(c));
-                hipLaunchKernelGGL(k_ham_fold, dim3(2 * (k + 1)), dim3(HAM_THREADS), 0, stream(), red(c), HAM_DOT_BLOCKS,
-                                   dsm(c));
-                hipLaunchKernelGGL(k_ham_combine, dim3(nb), dim3(HAM_THREADS), 0, stream(), vk1, basis(c), n, dsm(c), k + 1,
-                                   -1.0);
-                hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), vk1, n, red(c));
-                hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red(c), HAM_DOT_BLOCKS,
-                                   dsm(c) + 2 * (k + 1));
-                NRQ_HIP(hipMemcpyAsync(hst(c), dsm(c), sizeof(double) * (2 * (k + 1) + 1), hipMemcpyDeviceToHost, stream()));
-            }
+            // classical Gram-Schmidt of every column: h = V^+ w, w -= V h, ||w||^2
+            ham_vecs tab = table(ns, sel, outs);
+            hipLaunchKernelGGL(k_ham_dots, dim3(HAM_DOT_BLOCKS, k + 1, ns), dim3(HAM_THREADS), 0, stream(), tab, n);
+            hipLaunchKernelGGL(k_ham_fold, dim3(2 * (k + 1), ns), dim3(HAM_THREADS), 0, stream(), tab, HAM_DOT_BLOCKS, 0);
+            hipLaunchKernelGGL(k_ham_combine, dim3(nb, ns), dim3(HAM_THREADS), 0, stream(), tab, n, k + 1, -1.0);
+            hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS, ns), dim3(HAM_THREADS), 0, stream(), tab, n);
+            hipLaunchKernelGGL(k_ham_fold, dim3(1, ns), dim3(HAM_THREADS), 0, stream(), tab, HAM_DOT_BLOCKS, 2 * (k + 1));
+            NRQ_HIP(smalls_to_host());
             NRQ_HIP(hipGetLastError());
             NRQ_HIP(hipStreamSynchronize(stream()));
+            int nsc = 0; // columns to normalise, in tab's first nsc slots
             for (int q = 0; q < ns; ++q) {
                 const int c = sel[q];
                 column &C = col[c];
@@ -540,9 +583,11 @@ static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4],
                 for (int i = 0; i <= k; ++i) H[(size_t)i * m + k] = {host[2 * i], host[2 * i + 1]};
                 const double hk1 = std::sqrt(host[2 * (k + 1)]);
                 H[(size_t)(k + 1) * m + k] = hk1;
-                if (hk1 > 0)
-                    hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), outs[q], outs[q], 1.0 / hk1,
-                                       outs[q], 0.0, n);
+                if (hk1 > 0) {
+                    tab.w[nsc] = outs[q];
+                    tab.scale[nsc] = 1.0 / hk1;
+                    ++nsc;
+                }
                 // Givens rotations
                 for (int i = 0; i < k; ++i) {
                     const zc t = C.cs[i] * H[(size_t)i * m + k] + C.sn[i] * H[(size_t)(i + 1) * m + k];
@@ -566,6 +611,7 @@ static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4],
                 C.k = k + 1;
                 if (std::abs(C.g[k + 1]) <= 0.5 * tol * C.bnorm || hk1 == 0.0) C.cycling = false;
             }
+            if (nsc) hipLaunchKernelGGL(k_ham_scale, dim3(nb, nsc), dim3(HAM_THREADS), 0, stream(), tab, n);
         }
         // end of the cycle: y = H^-1 g (upper triangular, k x k), x += V y, then
         // the true residual r = b - S^+ x of every column that ran
@@ -585,7 +631,10 @@ static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4],
                 host[2 * i + 1] = C.y[i].imag();
             }
             NRQ_HIP(hipMemcpyAsync(dsm(c), host, sizeof(double) * 2 * k, hipMemcpyHostToDevice, stream()));
-            hipLaunchKernelGGL(k_ham_combine, dim3(nb), dim3(HAM_THREADS), 0, stream(), x[c], basis(c), n, dsm(c), k, 1.0);
+            {
+                const int one[1] = {c};
+                hipLaunchKernelGGL(k_ham_combine, dim3(nb, 1), dim3(HAM_THREADS), 0, stream(), table(1, one, &x[c]), n, k, 1.0);
+            }
             sel[ns] = c;
             ins[ns] = x[c];
             outs[ns] = rvec(c);
@@ -596,10 +645,13 @@ static int ham_gmres(const ham_operator &op, int nc, const cplx<double> (*a)[4],
         for (int q = 0; q < ns; ++q) {
             const int c = sel[q];
             hipLaunchKernelGGL(k_ham_axpby, dim3(nb), dim3(HAM_THREADS), 0, stream(), rvec(c), b[c], 1.0, rvec(c), -1.0, n);
-            hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS), dim3(HAM_THREADS), 0, stream(), rvec(c), n, red(c));
-            hipLaunchKernelGGL(k_ham_fold, dim3(1), dim3(HAM_THREADS), 0, stream(), red(c), HAM_DOT_BLOCKS, dsm(c));
-            NRQ_HIP(hipMemcpyAsync(hst(c), dsm(c), sizeof(double), hipMemcpyDeviceToHost, stream()));
         }
+        {
+            const ham_vecs t = table(ns, sel, outs);
+            hipLaunchKernelGGL(k_ham_norm2, dim3(HAM_DOT_BLOCKS, ns), dim3(HAM_THREADS), 0, stream(), t, n);
+            hipLaunchKernelGGL(k_ham_fold, dim3(1, ns), dim3(HAM_THREADS), 0, stream(), t, HAM_DOT_BLOCKS, 0);
+        }
+        NRQ_HIP(smalls_to_host());
         NRQ_HIP(hipGetLastError());
         NRQ_HIP(hipStreamSynchronize(stream()));
         for (int q = 0; q < ns; ++q) {
