@@ -38,6 +38,7 @@
 #include <algorithm>
 #include <array>
 #include <cmath>
+#include <cstdlib>
 #include <complex>
 #include <vector>
 
@@ -682,10 +683,16 @@ int ham_cn_update(nrq_heavy *const *hs, const double *mass, int n, const nrq_gau
     const int64_t V = g.vol;
     const int64_t nvec = 6 * V;
     const int m = std::max(1, restart);
-    // columns solved together: as many as fit HAM_BATCH_BYTES of Krylov basis,
-    // residual and right-hand side (m + 3 vectors each), at most HAM_MAXCOL
+    // columns solved together: as many as fit HAM_BATCH_BYTES (NRQ_HAM_BATCH_BYTES
+    // in the environment overrides it) of Krylov basis, residual and right-hand
+    // side (m + 3 vectors each), at most HAM_MAXCOL
+    size_t budget = HAM_BATCH_BYTES;
+    if (const char *e = getenv("NRQ_HAM_BATCH_BYTES")) {
+        const long long v = atoll(e);
+        if (v > 0) budget = (size_t)v;
+    }
     const size_t per_col = sizeof(cplx<double>) * (size_t)(m + 3) * (size_t)nvec;
-    const int ncb = (int)std::max<size_t>(1, std::min<size_t>(HAM_MAXCOL, HAM_BATCH_BYTES / per_col));
+    const int ncb = (int)std::max<size_t>(1, std::min<size_t>(HAM_MAXCOL, budget / per_col));
     if (int r = g_ham.fld.reserve(sizeof(double) * HAM_FLD * (size_t)V)) return r;
     if (int r = g_ham.blk.reserve(sizeof(cplx<double>) * 9 * HAM_BLK * (size_t)V)) return r;
     if (int r = g_ham.basis.reserve(sizeof(cplx<double>) * (size_t)(m + 1) * (size_t)nvec * ncb)) return r;
